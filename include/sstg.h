@@ -1,0 +1,162 @@
+/*
+ * sstg.h -- C ABI of sstgraph_amd, an MI355X (gfx950) implementation of the
+ * SSTGraph hot path: tinyset batch edge insert/delete and the
+ * EdgeMap/VertexMap algorithms BFS, PageRank, BC, CC and TC.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * throws or exits, and returns an int status (SSTG_OK == 0, negative
+ * SSTG_E_* on failure; sstg_strerror() names it).  Each declaration cites
+ * the reference interface it replaces (paths relative to the SSTGraph tree;
+ * SM = include/SSTGraph/SparseMatrix.hpp, TST = include/SSTGraph/internal/
+ * test.hpp, TSS = include/SSTGraph/TinySet_small.hpp).
+ *
+ * Pointers: by default every array argument is HOST memory.  With the flag
+ * SSTG_DEVICE_PTRS the arrays are device memory of the graph's device.
+ *
+ * Threading: one mutating call (insert/remove/rebuild) in flight per graph;
+ * read-only calls on one graph may be issued from several host threads but
+ * are serialised on the graph's stream.  Different graphs are independent.
+ */
+#ifndef SSTG_H
+#define SSTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSTG_ABI_VERSION 1
+
+/* status codes */
+#define SSTG_OK 0
+#define SSTG_E_INVAL -1    /* null handle / bad argument */
+#define SSTG_E_RANGE -2    /* vertex id out of range (nothing was changed) */
+#define SSTG_E_NOMEM -3    /* device allocation failed or budget exceeded */
+#define SSTG_E_DEVICE -4   /* HIP runtime error */
+#define SSTG_E_NODEVICE -5 /* no usable gfx950 device */
+#define SSTG_E_UNSUPPORTED -6
+
+/* flags */
+#define SSTG_DEVICE_PTRS 0x1u  /* array arguments are device pointers */
+#define SSTG_SEMI_SORTED 0x2u  /* batch already grouped by row (a hint, SM:167) */
+#define SSTG_KEEP_SELF_LOOPS 0x4u /* insert x==y edges (SM:536-544 single insert) */
+
+typedef struct sstg_graph sstg_graph;
+
+const char *sstg_strerror(int status);
+int sstg_abi_version(void);
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int sstg_device_count(int *out);
+
+/* ---- graph lifetime -------------------------------------------------- */
+/* SparseMatrixV<true>(height, width), SM:73-87.  The graph lives on the
+ * current HIP device. */
+int sstg_create(uint32_t n_rows, uint32_t n_cols, uint32_t flags,
+                sstg_graph **out);
+/* ~SparseMatrixV, SM:141-146. */
+int sstg_destroy(sstg_graph *g);
+
+/* ---- updates ----------------------------------------------------------- */
+/* SparseMatrixV<true>::insert_batch(element_type *edges, n, semi_sorted),
+ * SM:556-643: contents become old U {(src[i], dst[i]) : src[i] != dst[i]}.
+ * Structure of arrays instead of std::tuple; the caller's arrays are not
+ * reordered.  Fails with SSTG_E_RANGE (graph unchanged) if an id is out of
+ * range. */
+int sstg_insert_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                      uint64_t n, uint32_t flags);
+/* SparseMatrixV<true>::remove_batch(tuple<el_t,el_t>*, n, semi_sorted),
+ * SM:645-718: contents become old \ batch; absent edges are no-ops and
+ * self-loops are not skipped. */
+int sstg_remove_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                      uint64_t n, uint32_t flags);
+/* SparseMatrixV::insert(row, col), SM:536-544 (no self-loop skip). */
+int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col);
+/* SparseMatrixV::remove(row, col), SM:546-554. */
+int sstg_remove(sstg_graph *g, uint32_t row, uint32_t col);
+
+/* ---- queries ----------------------------------------------------------- */
+/* SparseMatrixV::has(row, col), SM:720-726. */
+int sstg_has(sstg_graph *g, uint32_t row, uint32_t col, int *out);
+/* has() for n pairs at once; out[i] = 0/1. */
+int sstg_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
+                   uint64_t n, uint8_t *out, uint32_t flags);
+/* num_nodes() / get_rows(), SM:56,150-155. */
+int sstg_num_nodes(sstg_graph *g, uint32_t *out);
+/* M(), SM:518-525. */
+int sstg_num_edges(sstg_graph *g, uint64_t *out);
+/* getDegree(i) for every row, SM:57-59. */
+int sstg_degrees(sstg_graph *g, uint32_t *out, uint32_t flags);
+/* Ascending contents of every row as CSR (offsets[n_rows+1], cols[M()]):
+ * the rows' map order (TSS:295-353, map_range SM:306-315).  cols may be NULL
+ * to fetch offsets only. */
+int sstg_export_csr(sstg_graph *g, uint64_t *offsets, uint32_t *cols,
+                    uint32_t flags);
+/* get_memory_size(), SM:527-534: bytes of device memory held by the graph. */
+int sstg_memory_size(sstg_graph *g, uint64_t *out);
+/* touch_all_sum(), SM:436-454: sum of all stored column ids. */
+int sstg_touch_all_sum(sstg_graph *g, uint64_t *out);
+/* common_neighbors(u, v, early_end), SM:61-71 / TSS:517-612. */
+int sstg_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
+                          int early_end, uint64_t *out);
+
+/* ---- EdgeMap/VertexMap algorithms (the EdgeMapVertexMap:: calls at
+ * TST:476-618).  Outputs are arrays of n_rows entries. -------------------- */
+/* BFS(g, src), TST:486: parent[src] = src, -1 if unreached; level[v] = hop
+ * depth (TST:497-509), UINT32_MAX if unreached.  Either output may be NULL. */
+int sstg_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level,
+             uint32_t flags);
+/* PR_S<double>(g, iters), TST:527: x0 = 1/n, x' = damping * sum over
+ * neighbours u of x(u)/deg(u) + (1 - damping)/n, FP64. */
+int sstg_pagerank_f64(sstg_graph *g, uint32_t iters, double damping,
+                      double *out, uint32_t flags);
+/* The same PageRank with FP32 ranks and Kahan-compensated neighbour sums. */
+int sstg_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,
+                       float *out, uint32_t flags);
+/* BC(g, src), TST:547: single-source Brandes dependencies, FP64; out[src] =
+ * |R(src)| - 1; unreached vertices 0. */
+int sstg_bc(sstg_graph *g, uint32_t src, double *out, uint32_t flags);
+/* CC(g), TST:571: component label = smallest vertex id in the component. */
+int sstg_cc(sstg_graph *g, uint32_t *out, uint32_t flags);
+/* TC(g), TST:606: number of triangles (each counted once). */
+int sstg_tc(sstg_graph *g, uint64_t *out);
+/* BF(g, src), TST:618 on a binary graph (every edge weighs 1): shortest
+ * hop distances, INT32_MAX/2 for unreached vertices. */
+int sstg_bf(sstg_graph *g, uint32_t src, int32_t *out, uint32_t flags);
+/* TouchAll(g), TST:476: sum over all edges of the destination id. */
+int sstg_touch_all(sstg_graph *g, uint64_t *out);
+
+/* ---- workload generator (rmat_util.h:82-131 rMat<uint32_t>) ------------ */
+/* Edges i in [first, first+count) of rMat(n, seed, a, b, c) into src/dst
+ * (host or device arrays per flags).  n must be a power of two. */
+int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
+              uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
+              uint32_t flags);
+
+/* ---- device memory plumbing for callers without a HIP runtime of their
+ * own (tests, benchmarks, FFI hosts) ------------------------------------ */
+int sstg_device_alloc(uint64_t bytes, void **out);
+int sstg_device_free(void *p);
+int sstg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+int sstg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+int sstg_memcpy_d2d(void *dst, const void *src, uint64_t bytes);
+int sstg_device_synchronize(void);
+/* Free/total device memory in bytes (hipMemGetInfo). */
+int sstg_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ---- kernel timing (HIP events on the graph's stream) ------------------ */
+/* When enabled, every kernel launch of the library is bracketed by HIP
+ * events on the stream it is launched on. */
+int sstg_profile_enable(int on);
+int sstg_profile_reset(void);
+/* Accumulated time (ms) and launch count of kernels whose name contains
+ * `name`; forces pending events to complete. */
+int sstg_profile_query(const char *name, double *total_ms, uint64_t *launches);
+/* Writes "name launches total_ms" lines into buf (NUL terminated). */
+int sstg_profile_dump(char *buf, uint64_t buf_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSTG_H */

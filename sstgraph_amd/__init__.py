@@ -1,0 +1,421 @@
+"""sstgraph_amd -- SSTGraph's hot path on MI355X (gfx950).
+
+Python mirror of the reference's operator interface over the C ABI in
+include/sstg.h (libsstg.so next to this file):
+
+  SparseMatrixV(height, width)           SparseMatrix.hpp:73-87
+    .insert_batch(src, dst)               SparseMatrix.hpp:556-643
+    .remove_batch(src, dst)               SparseMatrix.hpp:645-718
+    .insert / .remove / .has              SparseMatrix.hpp:536-554, 720-726
+    .num_nodes / .getDegree / .M          SparseMatrix.hpp:56-59, 518-525
+    .get_memory_size / .touch_all_sum     SparseMatrix.hpp:527-534, 436-454
+    .common_neighbors(u, v, early_end)    SparseMatrix.hpp:61-71
+  EdgeMapVertexMap.BFS / PR_S / BC / CC / TC / BF / TouchAll
+                                          test.hpp:476-618 call sites
+
+All compute runs in the HIP library; there is no CPU fallback.  Importing
+works without a GPU (for inspection), but creating a graph raises
+DeviceUnavailable when no gfx950 device is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsstg.so")
+
+SSTG_OK = 0
+SSTG_E_INVAL = -1
+SSTG_E_RANGE = -2
+SSTG_E_NOMEM = -3
+SSTG_E_DEVICE = -4
+SSTG_E_NODEVICE = -5
+SSTG_E_UNSUPPORTED = -6
+
+SSTG_DEVICE_PTRS = 0x1
+SSTG_SEMI_SORTED = 0x2
+SSTG_KEEP_SELF_LOOPS = 0x4
+
+UNREACHED = 0xFFFFFFFF
+
+
+class SSTGError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {strerror(code)} ({code})")
+
+
+class DeviceUnavailable(SSTGError):
+    pass
+
+
+class IdOutOfRange(SSTGError, ValueError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libsstg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"sstgraph_amd: {LIB_PATH} is missing; build it with "
+            "`python -m sstgraph_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    P = ctypes.POINTER
+    sig = {
+        "sstg_strerror": (ctypes.c_char_p, [i32]),
+        "sstg_abi_version": (i32, []),
+        "sstg_device_count": (i32, [P(i32)]),
+        "sstg_create": (i32, [u32, u32, u32, P(vp)]),
+        "sstg_destroy": (i32, [vp]),
+        "sstg_insert_batch": (i32, [vp, vp, vp, u64, u32]),
+        "sstg_remove_batch": (i32, [vp, vp, vp, u64, u32]),
+        "sstg_insert": (i32, [vp, u32, u32]),
+        "sstg_remove": (i32, [vp, u32, u32]),
+        "sstg_has": (i32, [vp, u32, u32, P(i32)]),
+        "sstg_has_batch": (i32, [vp, vp, vp, u64, vp, u32]),
+        "sstg_num_nodes": (i32, [vp, P(u32)]),
+        "sstg_num_edges": (i32, [vp, P(u64)]),
+        "sstg_degrees": (i32, [vp, vp, u32]),
+        "sstg_export_csr": (i32, [vp, vp, vp, u32]),
+        "sstg_memory_size": (i32, [vp, P(u64)]),
+        "sstg_touch_all_sum": (i32, [vp, P(u64)]),
+        "sstg_touch_all": (i32, [vp, P(u64)]),
+        "sstg_common_neighbors": (i32, [vp, u32, u32, i32, P(u64)]),
+        "sstg_bfs": (i32, [vp, u32, vp, vp, u32]),
+        "sstg_pagerank_f64": (i32, [vp, u32, ctypes.c_double, vp, u32]),
+        "sstg_pagerank_f32k": (i32, [vp, u32, ctypes.c_double, vp, u32]),
+        "sstg_bc": (i32, [vp, u32, vp, u32]),
+        "sstg_cc": (i32, [vp, vp, u32]),
+        "sstg_tc": (i32, [vp, P(u64)]),
+        "sstg_bf": (i32, [vp, u32, vp, u32]),
+        "sstg_rmat": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, u64, u64, vp, vp, u32]),
+        "sstg_device_alloc": (i32, [u64, P(vp)]),
+        "sstg_device_free": (i32, [vp]),
+        "sstg_memcpy_h2d": (i32, [vp, vp, u64]),
+        "sstg_memcpy_d2h": (i32, [vp, vp, u64]),
+        "sstg_memcpy_d2d": (i32, [vp, vp, u64]),
+        "sstg_device_synchronize": (i32, []),
+        "sstg_mem_info": (i32, [P(u64), P(u64)]),
+        "sstg_profile_enable": (i32, [i32]),
+        "sstg_profile_reset": (i32, []),
+        "sstg_profile_query": (i32, [ctypes.c_char_p, P(ctypes.c_double), P(u64)]),
+        "sstg_profile_dump": (i32, [ctypes.c_char_p, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def strerror(code):
+    try:
+        return lib().sstg_strerror(code).decode()
+    except Exception:  # pragma: no cover - library missing
+        return "error"
+
+
+def _check(rc, what):
+    if rc == SSTG_OK:
+        return
+    if rc == SSTG_E_NODEVICE:
+        raise DeviceUnavailable(rc, what)
+    if rc == SSTG_E_RANGE:
+        raise IdOutOfRange(rc, what)
+    raise SSTGError(rc, what)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib().sstg_device_count(ctypes.byref(n)), "sstg_device_count")
+    return n.value
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class DeviceArray:
+    """A device allocation owned by Python (plumbing for callers that keep
+    batches or results in HBM)."""
+
+    def __init__(self, count, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().sstg_device_alloc(max(self.nbytes, 1), ctypes.byref(p)),
+               "sstg_device_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        if a.nbytes:
+            _check(lib().sstg_memcpy_h2d(d.ptr, _ptr(a), a.nbytes), "h2d")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.count, dtype=self.dtype)
+        if self.nbytes:
+            _check(lib().sstg_memcpy_d2h(_ptr(out), self.ptr, self.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr is not None and _lib is not None:
+            lib().sstg_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1, device=False):
+    """Edges [first, first+count) of rMat<uint32_t>(n, seed, a, b, c)
+    (rmat_util.h:82-131), generated on the GPU.  device=True returns a pair
+    of DeviceArray."""
+    if device:
+        s, d = DeviceArray(count, np.uint32), DeviceArray(count, np.uint32)
+        _check(lib().sstg_rmat(n, seed, a, b, c, first, count, s.ptr, d.ptr,
+                               SSTG_DEVICE_PTRS), "sstg_rmat")
+        return s, d
+    s = np.empty(count, np.uint32)
+    d = np.empty(count, np.uint32)
+    _check(lib().sstg_rmat(n, seed, a, b, c, first, count, _ptr(s), _ptr(d), 0),
+           "sstg_rmat")
+    return s, d
+
+
+class SparseMatrixV:
+    """Device-resident SparseMatrixV<true> (CSR, binary): one sorted set of
+    column ids per row."""
+
+    def __init__(self, height, width=None):
+        self.n_rows = int(height)
+        self.n_cols = int(height if width is None else width)
+        h = ctypes.c_void_p()
+        _check(lib().sstg_create(self.n_rows, self.n_cols, 0, ctypes.byref(h)),
+               "sstg_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().sstg_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- updates
+    def _batch(self, fn, src, dst, flags, what):
+        if isinstance(src, DeviceArray):
+            n = min(src.count, dst.count)
+            _check(fn(self._h, src.ptr, dst.ptr, n, flags | SSTG_DEVICE_PTRS), what)
+            return
+        if dst is None:  # array of (row, col) pairs
+            e = np.asarray(src)
+            src, dst = e[:, 0], e[:, 1]
+        s, d = _u32(src), _u32(dst)
+        if s.shape != d.shape:
+            raise ValueError("src and dst differ in length")
+        _check(fn(self._h, _ptr(s), _ptr(d), s.size, flags), what)
+
+    def insert_batch(self, src, dst=None, semi_sorted=False):
+        self._batch(lib().sstg_insert_batch, src, dst,
+                    SSTG_SEMI_SORTED if semi_sorted else 0, "insert_batch")
+
+    def remove_batch(self, src, dst=None, semi_sorted=False):
+        self._batch(lib().sstg_remove_batch, src, dst,
+                    SSTG_SEMI_SORTED if semi_sorted else 0, "remove_batch")
+
+    def insert(self, row, col):
+        _check(lib().sstg_insert(self._h, row, col), "insert")
+
+    def remove(self, row, col):
+        _check(lib().sstg_remove(self._h, row, col), "remove")
+
+    # ---- queries
+    def has(self, row, col):
+        out = ctypes.c_int(0)
+        _check(lib().sstg_has(self._h, row, col, ctypes.byref(out)), "has")
+        return bool(out.value)
+
+    def has_batch(self, rows, cols):
+        r, c = _u32(rows), _u32(cols)
+        out = np.empty(r.size, np.uint8)
+        _check(lib().sstg_has_batch(self._h, _ptr(r), _ptr(c), r.size, _ptr(out), 0),
+               "has_batch")
+        return out.astype(bool)
+
+    def num_nodes(self):
+        return self.n_rows
+
+    def M(self):
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_num_edges(self._h, ctypes.byref(out)), "M")
+        return out.value
+
+    num_edges = M
+
+    def degrees(self):
+        out = np.empty(self.n_rows, np.uint32)
+        _check(lib().sstg_degrees(self._h, _ptr(out), 0), "degrees")
+        return out
+
+    def getDegree(self, node):
+        return int(self.degrees()[node])
+
+    def export_csr(self):
+        off = np.empty(self.n_rows + 1, np.uint64)
+        _check(lib().sstg_export_csr(self._h, _ptr(off), None, 0), "export_csr")
+        cols = np.empty(max(int(off[-1]), 1), np.uint32)
+        _check(lib().sstg_export_csr(self._h, _ptr(off), _ptr(cols), 0),
+               "export_csr")
+        return off, cols[: int(off[-1])]
+
+    def get_memory_size(self):
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_memory_size(self._h, ctypes.byref(out)), "memory_size")
+        return out.value
+
+    def touch_all_sum(self):
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_touch_all_sum(self._h, ctypes.byref(out)), "touch_all_sum")
+        return out.value
+
+    def common_neighbors(self, u, v, early_end=False):
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_common_neighbors(self._h, u, v, 1 if early_end else 0,
+                                           ctypes.byref(out)), "common_neighbors")
+        return out.value
+
+    def map_neighbors(self, i, f):
+        """Host-side ordered neighbour scan of row i (SparseMatrix.hpp:218-
+        253): calls f(i, nbr) in ascending order; f returning True stops the
+        scan (early exit, PMA.hpp:420-426)."""
+        off, cols = self.export_csr()
+        for c in cols[off[i]:off[i + 1]].tolist():
+            if f(i, c):
+                break
+
+
+class EdgeMapVertexMap:
+    """The algorithm entry points the reference harness calls
+    (test.hpp:476-618), returning numpy arrays instead of malloc'd memory."""
+
+    @staticmethod
+    def BFS(g, src, levels=False):
+        n = g.n_rows
+        parent = np.empty(n, np.int32)
+        level = np.empty(n, np.uint32)
+        _check(lib().sstg_bfs(g.handle, src, _ptr(parent), _ptr(level), 0), "BFS")
+        return (parent, level) if levels else parent
+
+    @staticmethod
+    def PR_S(g, iters=10, damping=0.85):
+        out = np.empty(g.n_rows, np.float64)
+        _check(lib().sstg_pagerank_f64(g.handle, iters, damping, _ptr(out), 0),
+               "PR_S")
+        return out
+
+    @staticmethod
+    def PR_F32K(g, iters=10, damping=0.85):
+        out = np.empty(g.n_rows, np.float32)
+        _check(lib().sstg_pagerank_f32k(g.handle, iters, damping, _ptr(out), 0),
+               "PR_F32K")
+        return out
+
+    @staticmethod
+    def BC(g, src):
+        out = np.empty(g.n_rows, np.float64)
+        _check(lib().sstg_bc(g.handle, src, _ptr(out), 0), "BC")
+        return out
+
+    @staticmethod
+    def CC(g):
+        out = np.empty(g.n_rows, np.uint32)
+        _check(lib().sstg_cc(g.handle, _ptr(out), 0), "CC")
+        return out
+
+    @staticmethod
+    def TC(g):
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_tc(g.handle, ctypes.byref(out)), "TC")
+        return out.value
+
+    @staticmethod
+    def BF(g, src):
+        out = np.empty(g.n_rows, np.int32)
+        _check(lib().sstg_bf(g.handle, src, _ptr(out), 0), "BF")
+        return out
+
+    @staticmethod
+    def TouchAll(g):
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_touch_all(g.handle, ctypes.byref(out)), "TouchAll")
+        return out.value
+
+
+class Profile:
+    """Kernel timings recorded by the library with HIP events on the stream
+    each kernel runs on."""
+
+    @staticmethod
+    def enable(on=True):
+        lib().sstg_profile_enable(1 if on else 0)
+
+    @staticmethod
+    def reset():
+        lib().sstg_profile_reset()
+
+    @staticmethod
+    def query(name):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_uint64(0)
+        lib().sstg_profile_query(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    @staticmethod
+    def dump():
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib().sstg_profile_dump(buf, len(buf))
+        rows = []
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.rsplit(" ", 2)
+            rows.append((name, int(n), float(ms)))
+        return rows
+
+
+def mem_info():
+    f, t = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(lib().sstg_mem_info(ctypes.byref(f), ctypes.byref(t)), "mem_info")
+    return f.value, t.value
+
+
+def synchronize():
+    _check(lib().sstg_device_synchronize(), "synchronize")

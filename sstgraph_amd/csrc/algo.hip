@@ -1,0 +1,599 @@
+// EdgeMap/VertexMap algorithms over the device-resident tinyset store.
+//
+// The reference calls EdgeMapVertexMap::{BFS, PR_S, BC, CC, TC, BF, TouchAll}
+// (include/SSTGraph/internal/test.hpp:476-618), which scan neighbours through
+// SparseMatrixV::map_neighbors / map_range (SparseMatrix.hpp:174-315).  Here a
+// row's neighbours are pool[off[v] .. off[v] + deg[v]) in ascending order.
+#include "algo.hpp"
+#include "edge_map.hpp"
+#include "primitives.hpp"
+
+namespace sstg {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+
+inline uint64_t blocks(uint64_t n) { return ceil_div(n ? n : 1, kT); }
+
+// ----------------------------------------------------------------- BFS
+__global__ void k_bfs_init(uint32_t n, uint32_t *level, int32_t *parent,
+                           uint32_t src) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v >= n) return;
+  level[v] = (v == src) ? 0u : kInf;
+  if (parent) parent[v] = (v == src) ? (int32_t)src : -1;
+}
+
+// Top-down (sparse push) step of a BFS level as an EdgeMap functor: the
+// first writer of an unvisited neighbour claims it.
+struct BfsPush {
+  uint32_t *level;
+  int32_t *parent;
+  uint32_t next_level;
+  __device__ __forceinline__ bool cond(uint32_t d) const {
+    return level[d] == kInf;
+  }
+  __device__ __forceinline__ bool update_atomic(uint32_t s, uint32_t d) const {
+    if (atomicCAS(&level[d], kInf, next_level) == kInf) {
+      if (parent) parent[d] = (int32_t)s;
+      return true;
+    }
+    return false;
+  }
+};
+
+// Bottom-up (dense pull) step: an unvisited vertex takes the first frontier
+// neighbour it finds.
+struct BfsPull {
+  uint32_t *level;
+  int32_t *parent;
+  uint32_t next_level;
+  __device__ __forceinline__ bool cond(uint32_t d) const {
+    return level[d] == kInf;
+  }
+  __device__ __forceinline__ void update(uint32_t s, uint32_t d) const {
+    level[d] = next_level;
+    if (parent) parent[d] = (int32_t)s;
+  }
+};
+
+__global__ void k_level_flags(const uint32_t *level, uint32_t n, uint32_t L,
+                              uint8_t *flags) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) flags[v] = level[v] == L;
+}
+
+__global__ void k_level_dist(const uint32_t *level, uint32_t n, int32_t *out) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) out[v] = level[v] == kInf ? (int32_t)(0x7FFFFFFF / 2)
+                                       : (int32_t)level[v];
+}
+
+// ------------------------------------------------------------ PageRank
+template <class Real>
+__global__ void k_pr_init(uint32_t n, const uint32_t *deg, Real x0, Real *rank,
+                          Real *contrib) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v >= n) return;
+  rank[v] = x0;
+  uint32_t d = deg[v];
+  contrib[v] = d ? x0 / (Real)d : (Real)0;
+}
+
+// ------------------------------------------------------------------ CC
+__device__ __forceinline__ uint32_t uf_find(uint32_t *p, uint32_t x) {
+  uint32_t y = p[x];
+  while (y != x) {
+    uint32_t z = p[y];
+    if (z != y) p[x] = z;  // path halving; only ever lowers p[x]
+    x = y;
+    y = z;
+  }
+  return x;
+}
+
+__global__ void k_cc_init(DevGraph g, uint32_t *p) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v >= g.n) return;
+  uint32_t d = g.deg[v];
+  uint32_t m = (uint32_t)v;
+  if (d) {
+    uint32_t first = g.pool[g.off[v]];  // rows are sorted: first = min
+    if (first < m) m = first;
+  }
+  p[v] = m;
+}
+
+// Hooks the larger root under the smaller one for every stored edge, flat
+// over edges (8 per thread).  p[x] <= x holds throughout, so each final root
+// is the smallest id of its component.
+__global__ void k_cc_hook(DevGraph g, const uint64_t *prefix, uint64_t m,
+                          uint32_t *p) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * 8;
+  if (e >= m) return;
+  uint64_t end = e + 8 < m ? e + 8 : m;
+  uint64_t v = find_segment(prefix, g.n, e);
+  for (; e < end; e++) {
+    while (e >= prefix[v + 1]) v++;
+    uint32_t u = (uint32_t)v;
+    uint32_t w = g.pool[g.off[v] + (e - prefix[v])];
+    uint32_t ru = uf_find(p, u), rw = uf_find(p, w);
+    while (ru != rw) {
+      uint32_t hi = ru > rw ? ru : rw, lo = ru > rw ? rw : ru;
+      uint32_t old = atomicCAS(&p[hi], hi, lo);
+      if (old == hi) break;
+      ru = uf_find(p, old);
+      rw = uf_find(p, lo);
+    }
+  }
+}
+
+__global__ void k_cc_final(uint32_t n, uint32_t *p, uint32_t *out) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v >= n) return;
+  out[v] = uf_find(p, (uint32_t)v);
+}
+
+// ------------------------------------------------------------------ TC
+// lower[v] = number of neighbours of v below v.
+__global__ void k_lower_degree(DevGraph g, uint32_t *lower) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v >= g.n) return;
+  uint32_t d = g.deg[v];
+  lower[v] = d ? (uint32_t)lower_bound_dev(g.pool + g.off[v], (uint64_t)d,
+                                           (uint32_t)v)
+               : 0;
+}
+
+// |a[0..na) n b[0..nb)| for sorted unique arrays: binary-search the shorter
+// into the longer with a moving lower bound, or a linear merge when the two
+// are of similar length.
+__device__ __forceinline__ uint64_t intersect_count(const uint32_t *a,
+                                                    uint32_t na,
+                                                    const uint32_t *b,
+                                                    uint32_t nb) {
+  if (na > nb) {
+    const uint32_t *t = a;
+    a = b;
+    b = t;
+    uint32_t tn = na;
+    na = nb;
+    nb = tn;
+  }
+  uint64_t c = 0;
+  if (na == 0) return 0;
+  if ((uint64_t)na * 16 < nb) {
+    uint32_t lo = 0;
+    for (uint32_t i = 0; i < na && lo < nb; i++) {
+      uint32_t x = a[i];
+      uint32_t hi = nb;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (b[mid] < x)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < nb && b[lo] == x) c++;
+    }
+  } else {
+    uint32_t i = 0, j = 0;
+    while (i < na && j < nb) {
+      uint32_t x = a[i], y = b[j];
+      c += (x == y);
+      i += (x <= y);
+      j += (y <= x);
+    }
+  }
+  return c;
+}
+
+// One thread per stored edge (u, v): for v < u at position k of N(u), the
+// neighbours of u below v are N(u)[0..k); count those also in
+// N(v)[0..lower[v]) (common_neighbors(u, v, early_end=true), TinySet_small.
+// hpp:357-407).
+__global__ void k_tc_edges(DevGraph g, const uint64_t *prefix, uint64_t m,
+                           const uint32_t *lower, unsigned long long *out) {
+  uint64_t e0 = ((uint64_t)blockIdx.x * kT + threadIdx.x) * 4;
+  uint64_t acc = 0;
+  if (e0 < m) {
+    uint64_t end = e0 + 4 < m ? e0 + 4 : m;
+    uint64_t u = find_segment(prefix, g.n, e0);
+    for (uint64_t e = e0; e < end; e++) {
+      while (e >= prefix[u + 1]) u++;
+      uint32_t k = (uint32_t)(e - prefix[u]);
+      const uint32_t *nu = g.pool + g.off[u];
+      uint32_t v = nu[k];
+      if (v >= (uint32_t)u || k == 0) continue;
+      uint32_t lv = lower[v];
+      if (lv == 0) continue;
+      acc += intersect_count(nu, k, g.pool + g.off[v], lv);
+    }
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0 && acc)
+    atomicAdd(out, (unsigned long long)acc);
+}
+
+__global__ void k_common(DevGraph g, uint32_t u, uint32_t v, int early_end,
+                         unsigned long long *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t ea = early_end ? u : 0xFFFFFFFFFFFFFFFFull;
+  uint64_t eb = early_end ? v : 0xFFFFFFFFFFFFFFFFull;
+  const uint32_t *a = g.pool + g.off[u], *b = g.pool + g.off[v];
+  uint32_t na = g.deg[u], nb = g.deg[v], i = 0, j = 0;
+  unsigned long long c = 0;
+  while (i < na && j < nb) {
+    uint64_t x = a[i], y = b[j];
+    if (x >= ea || y >= eb) break;
+    c += (x == y);
+    i += (x <= y);
+    j += (y <= x);
+  }
+  *out = c;
+}
+
+// ------------------------------------------------------------------ BC
+// sigma(w) = sum of sigma over neighbours one level up; vertex list = the
+// BFS order slice of one level.
+struct SigmaPull {
+  const uint32_t *level;
+  double *sigma;
+  uint32_t L;  // level of the vertices being computed
+  using Acc = double;
+  __device__ __forceinline__ Acc zero() const { return 0.0; }
+  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t u) const {
+    if (level[u] + 1 == L) a += sigma[u];
+  }
+  __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
+    return __shfl_xor(a, o, kWave);
+  }
+  __device__ __forceinline__ void store(uint32_t v, Acc s) const {
+    sigma[v] = s;
+  }
+};
+
+// delta(v) = sum over children w of sigma(v)/sigma(w) * (1 + delta(w)).
+struct DeltaPull {
+  const uint32_t *level;
+  const double *sigma;
+  double *delta;
+  uint32_t L;
+  using Acc = double;
+  __device__ __forceinline__ Acc zero() const { return 0.0; }
+  __device__ __forceinline__ void add(Acc &a, uint32_t v, uint32_t w) const {
+    if (level[w] == L + 1) a += sigma[v] / sigma[w] * (1.0 + delta[w]);
+  }
+  __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
+    return __shfl_xor(a, o, kWave);
+  }
+  __device__ __forceinline__ void store(uint32_t v, Acc s) const {
+    delta[v] = s;
+  }
+};
+
+// sigma push for directed graphs: every vertex v of level L-1 adds sigma(v)
+// to its out-neighbours on level L (integer-valued sums, exact below 2^53).
+__global__ void k_sigma_push(DevGraph g, const uint32_t *list, uint64_t count,
+                             const uint32_t *level, uint32_t L,
+                             double *sigma) {
+  uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  uint64_t i = gid / kWave;
+  if (i >= count) return;
+  uint32_t v = list[i];
+  uint32_t d = g.deg[v];
+  const uint32_t *row = g.pool + g.off[v];
+  double sv = sigma[v];
+  for (uint32_t k = (uint32_t)(gid % kWave); k < d; k += kWave) {
+    uint32_t w = row[k];
+    if (level[w] == L) atomicAdd(&sigma[w], sv);
+  }
+}
+
+// Clears *flag if some stored edge (u, v) has no reverse (v, u).
+__global__ void k_symmetry(DevGraph g, const uint64_t *prefix, uint64_t m,
+                           uint32_t *flag) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * 8;
+  if (e >= m) return;
+  uint64_t end = e + 8 < m ? e + 8 : m;
+  uint64_t u = find_segment(prefix, g.n, e);
+  for (; e < end; e++) {
+    while (e >= prefix[u + 1]) u++;
+    uint32_t v = g.pool[g.off[u] + (e - prefix[u])];
+    bool ok = false;
+    if (v < g.n) {
+      uint32_t dv = g.deg[v];
+      const uint32_t *row = g.pool + g.off[v];
+      uint64_t p = dv ? lower_bound_dev(row, (uint64_t)dv, (uint32_t)u) : 0;
+      ok = p < dv && row[p] == (uint32_t)u;
+    }
+    if (!ok) *flag = 0;
+  }
+}
+
+__global__ void k_bc_init(uint32_t n, double *sigma, double *delta,
+                          uint32_t src) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v >= n) return;
+  sigma[v] = v == src ? 1.0 : 0.0;
+  delta[v] = 0.0;
+}
+
+}  // namespace
+
+// ====================================================================== BFS
+// Direction-optimising BFS (Beamer): top-down push over a compacted queue
+// while the frontier is small, bottom-up pull over a frontier bitmap once the
+// frontier's edges exceed 1/alpha of the unexplored edges, back to top-down
+// when the frontier shrinks below n/beta.  Records the BFS order per level
+// in `order` (level starts in level_start) when requested (BC uses it).
+int graph_is_symmetric(sstg_graph *g, bool *out) {
+  if (g->sym_version == g->version) {
+    *out = g->symmetric;
+    return SSTG_OK;
+  }
+  hipStream_t s = g->stream;
+  uint32_t *flag = (uint32_t *)(g->dev_ctr + 11);
+  uint32_t one = 1;
+  SSTG_HIP(hipMemcpyAsync(flag, &one, sizeof(uint32_t), hipMemcpyHostToDevice,
+                          s));
+  if (g->m) {
+    const uint64_t *prefix = nullptr;
+    SSTG_TRY(graph_degree_prefix(g, &prefix));
+    SSTG_LAUNCH(k_symmetry, ceil_div(ceil_div(g->m, 8), kT), kT, 0, s,
+                g->view(), prefix, g->m, flag);
+  }
+  SSTG_HIP(hipMemcpyAsync(g->host_rb + 20, flag, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  g->symmetric = (uint32_t)g->host_rb[20] != 0;
+  g->sym_version = g->version;
+  *out = g->symmetric;
+  return SSTG_OK;
+}
+
+struct BfsRun {
+  uint32_t levels = 0;
+  uint64_t reached = 0;
+  std::vector<uint64_t> level_start;  // into order, levels + 1 entries
+};
+
+static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
+                    int32_t *parent, uint32_t *order, bool top_down_only,
+                    BfsRun *run) {
+  hipStream_t s = g->stream;
+  Workspace &ws = g->ws;
+  const uint32_t n = g->n_rows;
+  const uint64_t *prefix = nullptr;
+  SSTG_TRY(graph_degree_prefix(g, &prefix));
+  EdgeMapCtx ctx;
+  SSTG_TRY(edge_map_ctx(g, &ctx, kWsAlgo));
+  SSTG_LAUNCH(k_bfs_init, blocks(n), kT, 0, s, n, level, parent, src);
+  uint32_t *queue = order;
+  uint32_t *qbuf = nullptr;
+  if (!queue) {
+    SSTG_TRY(ws.get(kWsAlgo + 10, (uint64_t)n + 1, &qbuf));
+    queue = qbuf;
+  }
+  // queue holds the frontier of the current level at queue + qs
+  SSTG_HIP(hipMemcpyAsync(queue, &src, sizeof(uint32_t), hipMemcpyHostToDevice,
+                          s));
+  uint32_t src_deg = 0;
+  SSTG_HIP(hipMemcpyAsync(&src_deg, g->deg + src, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  uint64_t qs = 0, nf = 1, mf = src_deg;
+  uint64_t explored = src_deg;  // edges of visited vertices
+  bool bottom_up = false;
+  const double alpha = 15.0, beta = 18.0;
+  if (run) {  // level 0 is queue[0, 1); each step appends the end of the next
+    run->level_start.clear();
+    run->level_start.push_back(0);
+    run->level_start.push_back(1);
+  }
+  // a bottom-up step pulls over out-neighbours, which equals pulling over
+  // in-neighbours only when every edge has its reverse
+  bool symmetric = false;
+  if (!top_down_only) SSTG_TRY(graph_is_symmetric(g, &symmetric));
+  top_down_only = top_down_only || !symmetric;
+  uint32_t L = 0;
+  uint8_t *flags = nullptr;
+  while (nf > 0) {
+    uint64_t unexplored = g->m > explored ? g->m - explored : 0;
+    bool want_bu = !top_down_only &&
+                   (bottom_up ? (double)nf >= (double)n / beta
+                              : (double)mf > (double)unexplored / alpha);
+    EdgeMapResult res;
+    if (want_bu) {
+      if (!bottom_up) SSTG_TRY(frontier_to_bitmap(ctx, queue + qs, nf, s));
+      BfsPull f{level, parent, L + 1};
+      SSTG_TRY(edge_map_dense(g, ctx, f, &res));
+      bottom_up = true;
+      // the next top-down step (if any) needs the queue of level L+1
+      bool next_bu = (double)res.count >= (double)n / beta;
+      if (run || !next_bu) {
+        SSTG_TRY(ws.get(kWsAlgo + 11, n, &flags));
+        SSTG_LAUNCH(k_level_flags, blocks(n), kT, 0, s, (const uint32_t *)level,
+                    n, L + 1, flags);
+        uint64_t *tmp64 = nullptr;
+        SSTG_TRY(ws.get(kWsAlgo + 12, n, &tmp64));
+        SSTG_TRY(select_flagged_u64(flags, nullptr, n, tmp64,
+                                    g->dev_ctr + 7, ws, kWsPrim, s));
+        SSTG_TRY(narrow_u64_to_u32(tmp64, res.count, queue + qs + nf, s));
+      }
+    } else {
+      BfsPush f{level, parent, L + 1};
+      SSTG_TRY(edge_map_sparse(g, ctx, queue + qs, nf, mf, f,
+                               queue + qs + nf, &res));
+      bottom_up = false;
+    }
+    qs += nf;
+    nf = res.count;
+    mf = res.degree_sum;
+    explored += mf;
+    L++;
+    if (run) run->level_start.push_back(qs + nf);
+  }
+  if (run) {
+    run->levels = L;
+    run->reached = qs;
+  }
+  return SSTG_OK;
+}
+
+int algo_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level) {
+  uint32_t *lv = level;
+  if (!lv) SSTG_TRY(g->ws.get(kWsAlgo + 13, g->n_rows, &lv));
+  SSTG_TRY(bfs_core(g, src, lv, parent, nullptr, false, nullptr));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  return SSTG_OK;
+}
+
+int algo_bf(sstg_graph *g, uint32_t src, int32_t *out) {
+  uint32_t *lv = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 13, g->n_rows, &lv));
+  SSTG_TRY(bfs_core(g, src, lv, nullptr, nullptr, false, nullptr));
+  SSTG_LAUNCH(k_level_dist, blocks(g->n_rows), kT, 0, g->stream,
+              (const uint32_t *)lv, g->n_rows, out);
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  return SSTG_OK;
+}
+
+// ================================================================ PageRank
+template <class Real, bool kKahan>
+static int pagerank_impl(sstg_graph *g, uint32_t iters, double damping,
+                         Real *out) {
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  if (n == 0) return SSTG_OK;
+  EdgeMapCtx ctx;
+  SSTG_TRY(edge_map_ctx(g, &ctx, kWsAlgo));
+  DegreeBins bins;
+  SSTG_TRY(degree_bins(g, &bins));
+  Real *contrib = nullptr, *contrib2 = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 20, n, &contrib));
+  SSTG_TRY(g->ws.get(kWsAlgo + 21, n, &contrib2));
+  const Real x0 = (Real)(1.0 / (double)n);
+  SSTG_LAUNCH(k_pr_init<Real>, blocks(n), kT, 0, s, n, (const uint32_t *)g->deg,
+              x0, out, contrib);
+  const Real base = (Real)((1.0 - damping) / (double)n);
+  const Real damp = (Real)damping;
+  for (uint32_t it = 0; it < iters; it++) {
+    PrPull<Real, kKahan> f{contrib, contrib2, out, g->deg, damp, base};
+    SSTG_TRY(vertex_pull_all(g, bins, f));
+    Real *t = contrib;
+    contrib = contrib2;
+    contrib2 = t;
+  }
+  SSTG_HIP(hipStreamSynchronize(s));
+  return SSTG_OK;
+}
+
+int algo_pagerank(sstg_graph *g, uint32_t iters, double damping,
+                  double *out) {
+  return pagerank_impl<double, false>(g, iters, damping, out);
+}
+
+int algo_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,
+                       float *out) {
+  return pagerank_impl<float, true>(g, iters, damping, out);
+}
+
+// ====================================================================== CC
+int algo_cc(sstg_graph *g, uint32_t *out) {
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  if (n == 0) return SSTG_OK;
+  const uint64_t *prefix = nullptr;
+  SSTG_TRY(graph_degree_prefix(g, &prefix));
+  uint32_t *p = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 22, n, &p));
+  SSTG_LAUNCH(k_cc_init, blocks(n), kT, 0, s, g->view(), p);
+  if (g->m)
+    SSTG_LAUNCH(k_cc_hook, ceil_div(ceil_div(g->m, 8), kT), kT, 0, s,
+                g->view(), prefix, g->m, p);
+  SSTG_LAUNCH(k_cc_final, blocks(n), kT, 0, s, n, p, out);
+  SSTG_HIP(hipStreamSynchronize(s));
+  return SSTG_OK;
+}
+
+// ====================================================================== TC
+int algo_tc(sstg_graph *g, uint64_t *out) {
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  *out = 0;
+  if (n == 0 || g->m == 0) return SSTG_OK;
+  const uint64_t *prefix = nullptr;
+  SSTG_TRY(graph_degree_prefix(g, &prefix));
+  uint32_t *lower = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 23, n, &lower));
+  SSTG_LAUNCH(k_lower_degree, blocks(n), kT, 0, s, g->view(), lower);
+  unsigned long long *acc = (unsigned long long *)(g->dev_ctr + 8);
+  SSTG_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), s));
+  SSTG_LAUNCH(k_tc_edges, ceil_div(ceil_div(g->m, 4), kT), kT, 0, s, g->view(),
+              prefix, g->m, (const uint32_t *)lower, acc);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, acc, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  *out = g->host_rb[0];
+  return SSTG_OK;
+}
+
+int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
+                          int early_end, uint64_t *out) {
+  hipStream_t s = g->stream;
+  unsigned long long *acc = (unsigned long long *)(g->dev_ctr + 8);
+  SSTG_LAUNCH(k_common, 1, 64, 0, s, g->view(), u, v, early_end, acc);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, acc, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  *out = g->host_rb[0];
+  return SSTG_OK;
+}
+
+// ====================================================================== BC
+int algo_bc(sstg_graph *g, uint32_t src, double *out) {
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  uint32_t *level = nullptr, *order = nullptr;
+  double *sigma = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 13, n, &level));
+  SSTG_TRY(g->ws.get(kWsAlgo + 14, (uint64_t)n + 1, &order));
+  SSTG_TRY(g->ws.get(kWsAlgo + 15, n, &sigma));
+  BfsRun run;
+  SSTG_TRY(bfs_core(g, src, level, nullptr, order, false, &run));
+  SSTG_LAUNCH(k_bc_init, blocks(n), kT, 0, s, n, sigma, out, src);
+  // forward: sigma level by level (level 0 is the source, sigma = 1); pulled
+  // from the parents on symmetric graphs, pushed along out-edges otherwise
+  bool symmetric = false;
+  SSTG_TRY(graph_is_symmetric(g, &symmetric));
+  for (uint32_t L = 1; L < run.levels; L++) {
+    if (symmetric) {
+      uint64_t b = run.level_start[L], e = run.level_start[L + 1];
+      if (e <= b) continue;
+      SigmaPull f{level, sigma, L};
+      SSTG_TRY(vertex_pull_list(g, order + b, e - b, f));
+    } else {
+      uint64_t b = run.level_start[L - 1], e = run.level_start[L];
+      if (e <= b) continue;
+      SSTG_LAUNCH(k_sigma_push, ceil_div((e - b) * kWave, kT), kT, 0, s,
+                  g->view(), (const uint32_t *)(order + b), e - b,
+                  (const uint32_t *)level, L, sigma);
+    }
+  }
+  // backward: dependencies from the deepest level up to the source
+  for (int64_t L = (int64_t)run.levels - 2; L >= 0; L--) {
+    uint64_t b = run.level_start[L], e = run.level_start[L + 1];
+    if (e <= b) continue;
+    DeltaPull f{level, sigma, out, (uint32_t)L};
+    SSTG_TRY(vertex_pull_list(g, order + b, e - b, f));
+  }
+  SSTG_HIP(hipStreamSynchronize(s));
+  return SSTG_OK;
+}
+
+}  // namespace sstg

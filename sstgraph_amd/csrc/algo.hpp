@@ -1,0 +1,20 @@
+// EdgeMap/VertexMap algorithms on the device-resident store (algo.hip).
+// All outputs are device arrays of n_rows entries.
+#pragma once
+
+#include "graph.hpp"
+
+namespace sstg {
+
+int algo_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level);
+int algo_pagerank(sstg_graph *g, uint32_t iters, double damping, double *out);
+int algo_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,
+                       float *out);
+int algo_bc(sstg_graph *g, uint32_t src, double *out);
+int algo_cc(sstg_graph *g, uint32_t *out);
+int algo_tc(sstg_graph *g, uint64_t *out);
+int algo_bf(sstg_graph *g, uint32_t src, int32_t *out);
+int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
+                          int early_end, uint64_t *out);
+
+}  // namespace sstg

@@ -1,0 +1,526 @@
+// C ABI of sstgraph_amd (include/sstg.h): argument checking, host/device
+// staging of arrays, and the runtime pieces shared by every call (workspace,
+// kernel profiler, error notes).
+#include <cstdlib>
+
+#include "algo.hpp"
+#include "graph.hpp"
+
+namespace sstg {
+
+static thread_local char g_last_error[256] = {0};
+
+void note_hip_error(hipError_t e, const char *what, const char *file,
+                    int line) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)",
+                hipGetErrorString(e), what, file, line);
+  if (std::getenv("SSTG_VERBOSE"))
+    std::fprintf(stderr, "sstgraph_amd: %s\n", g_last_error);
+}
+
+// ------------------------------------------------------------ profiler
+Profiler &profiler() {
+  static Profiler p;
+  return p;
+}
+
+hipEvent_t Profiler::get_event() {
+  if (!free_events.empty()) {
+    hipEvent_t e = free_events.back();
+    free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void Profiler::begin(const char *, hipStream_t s, hipEvent_t *a) {
+  std::lock_guard<std::mutex> lk(mu);
+  *a = get_event();
+  if (*a) hipEventRecord(*a, s);
+}
+
+void Profiler::end(const char *name, hipStream_t s, hipEvent_t a) {
+  std::lock_guard<std::mutex> lk(mu);
+  if (!a) return;
+  hipEvent_t b = get_event();
+  if (!b) return;
+  hipEventRecord(b, s);
+  pending.push_back(Rec{name, a, b});
+  if (pending.size() > 4096) {
+    // keep the event pool bounded
+    for (auto &r : pending) {
+      hipEventSynchronize(r.b);
+      float ms = 0;
+      hipEventElapsedTime(&ms, r.a, r.b);
+      bool found = false;
+      for (auto &x : acc)
+        if (x.name == r.name) {
+          x.ms += ms;
+          x.n++;
+          found = true;
+          break;
+        }
+      if (!found) acc.push_back(Acc{r.name, (double)ms, 1});
+      free_events.push_back(r.a);
+      free_events.push_back(r.b);
+    }
+    pending.clear();
+  }
+}
+
+void Profiler::flush() {
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto &r : pending) {
+    hipEventSynchronize(r.b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    bool found = false;
+    for (auto &x : acc)
+      if (x.name == r.name) {
+        x.ms += ms;
+        x.n++;
+        found = true;
+        break;
+      }
+    if (!found) acc.push_back(Acc{r.name, (double)ms, 1});
+    free_events.push_back(r.a);
+    free_events.push_back(r.b);
+  }
+  pending.clear();
+}
+
+void Profiler::reset() {
+  flush();
+  std::lock_guard<std::mutex> lk(mu);
+  acc.clear();
+}
+
+// ----------------------------------------------------------- workspace
+int Workspace::get(int slot, size_t bytes, void **out) {
+  if (slot < 0) return SSTG_E_INVAL;
+  if ((size_t)slot >= slots.size()) slots.resize(slot + 1);
+  Slot &s = slots[slot];
+  if (s.bytes < bytes) {
+    if (s.p) {
+      if (stream) hipStreamSynchronize(stream);
+      hipFree(s.p);
+      total -= s.bytes;
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    want = (want + 4095) & ~(size_t)4095;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      want = (bytes + 4095) & ~(size_t)4095;
+      e = hipMalloc(&p, want);
+      if (e != hipSuccess) {
+        note_hip_error(e, "workspace hipMalloc", __FILE__, __LINE__);
+        return SSTG_E_NOMEM;
+      }
+    }
+    s.p = p;
+    s.bytes = want;
+    total += want;
+  }
+  *out = s.p;
+  return SSTG_OK;
+}
+
+void Workspace::release_all() {
+  if (stream) hipStreamSynchronize(stream);
+  for (auto &s : slots)
+    if (s.p) hipFree(s.p);
+  slots.clear();
+  total = 0;
+}
+
+static bool device_ok(int *dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (hipGetDevice(dev) != hipSuccess) return false;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, *dev) != hipSuccess) return false;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+// Output staging: runs fn(device_ptr) and copies the result to host memory
+// unless the caller passed device memory.
+template <class T, class F>
+static int with_output(sstg_graph *g, T *out, uint64_t count, uint32_t flags,
+                       int slot, F fn) {
+  if (!out) return fn((T *)nullptr);
+  if (flags & SSTG_DEVICE_PTRS) return fn(out);
+  T *d = nullptr;
+  SSTG_TRY(g->ws.get(slot, count, &d));
+  SSTG_TRY(fn(d));
+  SSTG_HIP(hipMemcpyAsync(out, d, count * sizeof(T), hipMemcpyDeviceToHost,
+                          g->stream));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  return SSTG_OK;
+}
+
+}  // namespace sstg
+
+using namespace sstg;
+
+extern "C" {
+
+const char *sstg_strerror(int status) {
+  switch (status) {
+    case SSTG_OK:
+      return "ok";
+    case SSTG_E_INVAL:
+      return "invalid argument";
+    case SSTG_E_RANGE:
+      return "vertex id out of range";
+    case SSTG_E_NOMEM:
+      return "out of device memory";
+    case SSTG_E_DEVICE:
+      return g_last_error[0] ? g_last_error : "HIP runtime error";
+    case SSTG_E_NODEVICE:
+      return "no gfx950 device";
+    case SSTG_E_UNSUPPORTED:
+      return "unsupported";
+    default:
+      return "unknown status";
+  }
+}
+
+int sstg_abi_version(void) { return SSTG_ABI_VERSION; }
+
+int sstg_device_count(int *out) {
+  if (!out) return SSTG_E_INVAL;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *out = n;
+  return SSTG_OK;
+}
+
+int sstg_create(uint32_t n_rows, uint32_t n_cols, uint32_t flags,
+                sstg_graph **out) {
+  (void)flags;
+  if (!out) return SSTG_E_INVAL;
+  *out = nullptr;
+  int dev = 0;
+  if (!device_ok(&dev)) return SSTG_E_NODEVICE;
+  sstg_graph *g = new (std::nothrow) sstg_graph();
+  if (!g) return SSTG_E_NOMEM;
+  g->device = dev;
+  g->n_rows = n_rows;
+  g->n_cols = n_cols;
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    delete g;
+    return SSTG_E_DEVICE;
+  }
+  g->ws.stream = g->stream;
+  int rc = graph_alloc(g);
+  if (rc != SSTG_OK) {
+    graph_free(g);
+    hipStreamDestroy(g->stream);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return SSTG_OK;
+}
+
+int sstg_destroy(sstg_graph *g) {
+  if (!g) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  graph_free(g);
+  hipStreamDestroy(g->stream);
+  delete g;
+  return SSTG_OK;
+}
+
+int sstg_insert_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                      uint64_t n, uint32_t flags) {
+  if (!g || (n && (!src || !dst))) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_update(g, src, dst, n, true,
+                      !(flags & SSTG_KEEP_SELF_LOOPS),
+                      (flags & SSTG_DEVICE_PTRS) != 0);
+}
+
+int sstg_remove_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                      uint64_t n, uint32_t flags) {
+  if (!g || (n && (!src || !dst))) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_update(g, src, dst, n, false, false,
+                      (flags & SSTG_DEVICE_PTRS) != 0);
+}
+
+int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col) {
+  if (!g) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_update(g, &row, &col, 1, true, false, false);
+}
+
+int sstg_remove(sstg_graph *g, uint32_t row, uint32_t col) {
+  if (!g) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_update(g, &row, &col, 1, false, false, false);
+}
+
+int sstg_has(sstg_graph *g, uint32_t row, uint32_t col, int *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (row >= g->n_rows || col >= g->n_cols) return SSTG_E_RANGE;
+  hipSetDevice(g->device);
+  uint8_t h = 0;
+  SSTG_TRY(graph_has_batch(g, &row, &col, 1, &h, false));
+  *out = h;
+  return SSTG_OK;
+}
+
+int sstg_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
+                   uint64_t n, uint8_t *out, uint32_t flags) {
+  if (!g || (n && (!rows || !cols || !out))) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_has_batch(g, rows, cols, n, out,
+                         (flags & SSTG_DEVICE_PTRS) != 0);
+}
+
+int sstg_num_nodes(sstg_graph *g, uint32_t *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  *out = g->n_rows;
+  return SSTG_OK;
+}
+
+int sstg_num_edges(sstg_graph *g, uint64_t *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  *out = g->m;
+  return SSTG_OK;
+}
+
+int sstg_degrees(sstg_graph *g, uint32_t *out, uint32_t flags) {
+  if (!g || !out) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  SSTG_HIP(hipMemcpyAsync(out, g->deg, (uint64_t)g->n_rows * sizeof(uint32_t),
+                          (flags & SSTG_DEVICE_PTRS) ? hipMemcpyDeviceToDevice
+                                                     : hipMemcpyDeviceToHost,
+                          g->stream));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  return SSTG_OK;
+}
+
+int sstg_export_csr(sstg_graph *g, uint64_t *offsets, uint32_t *cols,
+                    uint32_t flags) {
+  if (!g || !offsets) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_export(g, offsets, cols, (flags & SSTG_DEVICE_PTRS) != 0);
+}
+
+int sstg_memory_size(sstg_graph *g, uint64_t *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  uint64_t n = g->n_rows;
+  *out = sizeof(sstg_graph) + n * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) +
+         g->pool_cap * sizeof(uint32_t);
+  return SSTG_OK;
+}
+
+int sstg_touch_all_sum(sstg_graph *g, uint64_t *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_touch_sum(g, out);
+}
+
+int sstg_touch_all(sstg_graph *g, uint64_t *out) {
+  return sstg_touch_all_sum(g, out);
+}
+
+int sstg_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
+                          int early_end, uint64_t *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (u >= g->n_rows || v >= g->n_rows) return SSTG_E_RANGE;
+  hipSetDevice(g->device);
+  return algo_common_neighbors(g, u, v, early_end, out);
+}
+
+int sstg_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level,
+             uint32_t flags) {
+  if (!g) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  if (src >= g->n_rows) return SSTG_E_RANGE;
+  hipSetDevice(g->device);
+  return with_output<int32_t>(
+      g, parent, g->n_rows, flags, kWsOut, [&](int32_t *dp) {
+        return with_output<uint32_t>(
+            g, level, g->n_rows, flags, kWsOut2,
+            [&](uint32_t *dl) { return algo_bfs(g, src, dp, dl); });
+      });
+}
+
+int sstg_pagerank_f64(sstg_graph *g, uint32_t iters, double damping,
+                      double *out, uint32_t flags) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  hipSetDevice(g->device);
+  return with_output<double>(g, out, g->n_rows, flags, kWsOut, [&](double *d) {
+    return algo_pagerank(g, iters, damping, d);
+  });
+}
+
+int sstg_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,
+                       float *out, uint32_t flags) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  hipSetDevice(g->device);
+  return with_output<float>(g, out, g->n_rows, flags, kWsOut, [&](float *d) {
+    return algo_pagerank_f32k(g, iters, damping, d);
+  });
+}
+
+int sstg_bc(sstg_graph *g, uint32_t src, double *out, uint32_t flags) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  if (src >= g->n_rows) return SSTG_E_RANGE;
+  hipSetDevice(g->device);
+  return with_output<double>(g, out, g->n_rows, flags, kWsOut,
+                             [&](double *d) { return algo_bc(g, src, d); });
+}
+
+int sstg_cc(sstg_graph *g, uint32_t *out, uint32_t flags) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  hipSetDevice(g->device);
+  return with_output<uint32_t>(g, out, g->n_rows, flags, kWsOut,
+                               [&](uint32_t *d) { return algo_cc(g, d); });
+}
+
+int sstg_tc(sstg_graph *g, uint64_t *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  hipSetDevice(g->device);
+  return algo_tc(g, out);
+}
+
+int sstg_bf(sstg_graph *g, uint32_t src, int32_t *out, uint32_t flags) {
+  if (!g || !out) return SSTG_E_INVAL;
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;  // graphs are square
+  if (src >= g->n_rows) return SSTG_E_RANGE;
+  hipSetDevice(g->device);
+  return with_output<int32_t>(g, out, g->n_rows, flags, kWsOut,
+                              [&](int32_t *d) { return algo_bf(g, src, d); });
+}
+
+int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
+              uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
+              uint32_t flags) {
+  if (count && (!src || !dst)) return SSTG_E_INVAL;
+  int dev = 0;
+  if (!device_ok(&dev)) return SSTG_E_NODEVICE;
+  hipStream_t s = nullptr;
+  if (flags & SSTG_DEVICE_PTRS) {
+    SSTG_TRY(rmat_generate(n, seed, a, b, c, first, count, src, dst, s));
+    SSTG_HIP(hipStreamSynchronize(s));
+    return SSTG_OK;
+  }
+  uint32_t *ds = nullptr, *dd = nullptr;
+  SSTG_HIP(hipMalloc(&ds, (count ? count : 1) * sizeof(uint32_t)));
+  SSTG_HIP(hipMalloc(&dd, (count ? count : 1) * sizeof(uint32_t)));
+  int rc = rmat_generate(n, seed, a, b, c, first, count, ds, dd, s);
+  if (rc == SSTG_OK && count) {
+    hipMemcpy(src, ds, count * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    hipMemcpy(dst, dd, count * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  }
+  hipFree(ds);
+  hipFree(dd);
+  return rc;
+}
+
+int sstg_device_alloc(uint64_t bytes, void **out) {
+  if (!out) return SSTG_E_INVAL;
+  SSTG_HIP(hipMalloc(out, bytes ? bytes : 1));
+  return SSTG_OK;
+}
+
+int sstg_device_free(void *p) {
+  SSTG_HIP(hipFree(p));
+  return SSTG_OK;
+}
+
+int sstg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
+  SSTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SSTG_OK;
+}
+
+int sstg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
+  SSTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SSTG_OK;
+}
+
+int sstg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
+  SSTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return SSTG_OK;
+}
+
+int sstg_device_synchronize(void) {
+  SSTG_HIP(hipDeviceSynchronize());
+  return SSTG_OK;
+}
+
+int sstg_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
+  size_t f = 0, t = 0;
+  SSTG_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return SSTG_OK;
+}
+
+int sstg_profile_enable(int on) {
+  profiler().on = on != 0;
+  return SSTG_OK;
+}
+
+int sstg_profile_reset(void) {
+  profiler().reset();
+  return SSTG_OK;
+}
+
+int sstg_profile_query(const char *name, double *total_ms,
+                       uint64_t *launches) {
+  if (!name) return SSTG_E_INVAL;
+  Profiler &p = profiler();
+  p.flush();
+  std::lock_guard<std::mutex> lk(p.mu);
+  double ms = 0;
+  uint64_t n = 0;
+  for (auto &a : p.acc)
+    if (a.name.find(name) != std::string::npos) {
+      ms += a.ms;
+      n += a.n;
+    }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = n;
+  return SSTG_OK;
+}
+
+int sstg_profile_dump(char *buf, uint64_t buf_bytes) {
+  if (!buf || buf_bytes == 0) return SSTG_E_INVAL;
+  Profiler &p = profiler();
+  p.flush();
+  std::lock_guard<std::mutex> lk(p.mu);
+  std::string s;
+  for (auto &a : p.acc) {
+    char line[256];
+    std::snprintf(line, sizeof(line), "%s %llu %.6f\n", a.name.c_str(),
+                  (unsigned long long)a.n, a.ms);
+    s += line;
+  }
+  std::snprintf(buf, buf_bytes, "%s", s.c_str());
+  return SSTG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Shared definitions of the sstgraph_amd HIP library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sstg.h"
+
+namespace sstg {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors
+struct Status {
+  int code = SSTG_OK;
+};
+
+#define SSTG_HIP(expr)                                                        \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ::sstg::note_hip_error(e_, #expr, __FILE__, __LINE__);                  \
+      return e_ == hipErrorOutOfMemory ? SSTG_E_NOMEM : SSTG_E_DEVICE;        \
+    }                                                                         \
+  } while (0)
+
+#define SSTG_TRY(expr)                                                        \
+  do {                                                                        \
+    int rc_ = (expr);                                                         \
+    if (rc_ != SSTG_OK) return rc_;                                           \
+  } while (0)
+
+void note_hip_error(hipError_t e, const char *what, const char *file, int line);
+
+// ------------------------------------------------------------- profiler
+// Brackets kernel launches with HIP events on the launch stream when enabled.
+struct Profiler {
+  struct Rec {
+    const char *name;
+    hipEvent_t a, b;
+  };
+  struct Acc {
+    std::string name;
+    double ms = 0;
+    uint64_t n = 0;
+  };
+  bool on = false;
+  std::mutex mu;
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> free_events;
+  std::vector<Acc> acc;
+  hipEvent_t get_event();
+  void begin(const char *name, hipStream_t s, hipEvent_t *a);
+  void end(const char *name, hipStream_t s, hipEvent_t a);
+  void flush();
+  void reset();
+};
+Profiler &profiler();
+
+// Launch helper: LAUNCH(name, kernel, grid, block, shmem, stream, args...)
+#define SSTG_LAUNCH(kernel, grid, block, shmem, stream, ...)                  \
+  do {                                                                        \
+    ::sstg::Profiler &p_ = ::sstg::profiler();                                \
+    hipEvent_t ev_ = nullptr;                                                 \
+    if (p_.on) p_.begin(#kernel, stream, &ev_);                               \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream,        \
+                       __VA_ARGS__);                                          \
+    if (p_.on) p_.end(#kernel, stream, ev_);                                  \
+  } while (0)
+
+// ---------------------------------------------------------- workspace
+// Growable device scratch slots, reused across calls on one graph.  All use
+// is ordered on the graph's stream; a slot is only regrown after the stream
+// has drained.
+struct Workspace {
+  struct Slot {
+    void *p = nullptr;
+    size_t bytes = 0;
+  };
+  std::vector<Slot> slots;
+  hipStream_t stream = nullptr;
+  size_t total = 0;
+  int get(int slot, size_t bytes, void **out);
+  template <class T> int get(int slot, size_t count, T **out) {
+    void *p = nullptr;
+    int rc = get(slot, count * sizeof(T) + 256, &p);
+    *out = static_cast<T *>(p);
+    return rc;
+  }
+  void release_all();
+};
+
+// ------------------------------------------------------------- helpers
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+inline uint32_t bit_width_u32(uint32_t x) {
+  return x ? 32u - (uint32_t)__builtin_clz(x) : 0u;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// popcount(mask & lanes-below-me)
+__device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <class T>
+__device__ __forceinline__ uint64_t lower_bound_dev(const T *a, uint64_t len,
+                                                    T x) {
+  uint64_t lo = 0, hi = len;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// index of the segment containing flat position x, given exclusive offsets
+// off[0..k) (off[0] == 0, non-decreasing): largest s with off[s] <= x.
+__device__ __forceinline__ uint64_t find_segment(const uint64_t *off,
+                                                 uint64_t k, uint64_t x) {
+  uint64_t lo = 0, hi = k;
+  while (hi - lo > 1) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// wave-wide inclusive sum (wave64)
+template <class T> __device__ __forceinline__ T wave_inclusive_sum(T v) {
+  const uint32_t l = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    T t = __shfl_up(v, o, kWave);
+    if (l >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// ------------------------------------------------------- device graph
+// The tinyset store of one graph in HBM.  Row v holds deg[v] sorted unique
+// column ids at pool[off[v] .. off[v] + deg[v]); the slot has room for cap[v]
+// ids (a power of two, at least 4).  A row that outgrows its slot moves to a
+// fresh slot at the pool tail; the old slot becomes garbage until the next
+// rebuild compacts the pool.
+struct DevGraph {
+  uint32_t n;       // rows
+  uint64_t *off;    // [n]
+  uint32_t *deg;    // [n]
+  uint32_t *cap;    // [n]
+  uint32_t *pool;   // [pool_cap]
+};
+
+}  // namespace sstg

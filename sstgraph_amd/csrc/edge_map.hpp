@@ -1,0 +1,411 @@
+// EdgeMap / VertexMap on the device-resident store (Ligra-style interface of
+// the reference's EdgeMapVertexMap layer, whose callers are at
+// include/SSTGraph/internal/test.hpp:476-618 and whose graph-side contract is
+// SparseMatrixV::map_neighbors / map_range, SparseMatrix.hpp:174-315).
+//
+// Functor contracts (device methods):
+//   sparse push  : bool cond(d); bool update_atomic(s, d)  -> d joins the
+//                  output frontier when update_atomic returns true
+//   dense pull   : bool cond(d); void update(s, d)         -> d scans its
+//                  neighbours until one is in the input frontier bitmap
+//   vertex pull  : Acc zero(); Acc term(v, u) / add(acc, v, u);
+//                  Acc merge(a, b); void store(v, acc)      -> reduction over
+//                  all neighbours of v
+#pragma once
+
+#include "graph.hpp"
+#include "primitives.hpp"
+
+namespace sstg {
+
+struct EdgeMapResult {
+  uint64_t count = 0;       // vertices in the output frontier
+  uint64_t degree_sum = 0;  // sum of their degrees
+};
+
+struct EdgeMapCtx {
+  uint32_t *bm_cur = nullptr;   // input frontier bitmap
+  uint32_t *bm_next = nullptr;  // output frontier bitmap
+  uint64_t words = 0;
+  uint64_t *fpre = nullptr;     // frontier degree prefix (sparse)
+  uint64_t fcap = 0;
+  int slot = 0;
+  unsigned long long *ctr = nullptr;  // [0] count, [1] degree sum
+};
+
+namespace em {
+
+constexpr int kT = 256;
+constexpr int kChunk = 8;
+
+__global__ void k_clear_u32(uint32_t *p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_queue_to_bitmap(const uint32_t *q, uint64_t nq,
+                                  uint32_t *bm) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < nq) {
+    uint32_t v = q[i];
+    atomicOr(&bm[v >> 5], 1u << (v & 31));
+  }
+}
+
+__global__ void k_frontier_degrees(DevGraph g, const uint32_t *q, uint64_t nq,
+                                   uint64_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < nq) out[i] = g.deg[q[i]];
+  if (i == nq) out[nq] = 0;
+}
+
+__global__ void k_narrow(const uint64_t *in, uint64_t n, uint32_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)in[i];
+}
+
+// Wave-aggregated append of d to out; also accumulates count and degree sum.
+__device__ __forceinline__ void append(bool take, uint32_t d, uint32_t dd,
+                                       uint32_t *out,
+                                       unsigned long long *ctr) {
+  uint64_t mask = __ballot(take);
+  if (!mask) return;
+  uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+  uint32_t cnt = (uint32_t)__popcll(mask);
+  unsigned long long dsum = take ? dd : 0;
+  // sum degrees of the taken lanes over the active lanes
+  for (int o = kWave / 2; o > 0; o >>= 1)
+    dsum += __shfl_xor(dsum, o, kWave);
+  unsigned long long base = 0;
+  if (lane_id() == leader) {
+    base = atomicAdd(&ctr[0], (unsigned long long)cnt);
+    atomicAdd(&ctr[1], dsum);
+  }
+  base = __shfl(base, leader, kWave);
+  if (take && out) out[base + mask_rank(mask)] = d;
+}
+
+template <class F>
+__global__ void k_sparse_push(DevGraph g, const uint32_t *front,
+                              const uint64_t *fpre, uint64_t nf, uint64_t mf,
+                              F f, uint32_t *next, unsigned long long *ctr) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  uint64_t end = e + kChunk < mf ? e + kChunk : mf;
+  uint64_t i = e < mf ? find_segment(fpre, nf, e) : 0;
+  // every lane runs kChunk rounds so the wave stays converged at append()
+  for (int r = 0; r < kChunk; r++, e++) {
+    bool take = false;
+    uint32_t d = 0, dd = 0;
+    if (e < end) {
+      while (e >= fpre[i + 1]) i++;
+      uint32_t s = front[i];
+      d = g.pool[g.off[s] + (e - fpre[i])];
+      if (f.cond(d) && f.update_atomic(s, d)) {
+        take = true;
+        dd = g.deg[d];
+      }
+    }
+    append(take, d, dd, next, ctr);
+  }
+}
+
+// Dense pull with G lanes per vertex (G divides 64).  A vertex whose cond
+// holds scans its neighbours G at a time and stops at the first chunk that
+// holds a frontier member; the lowest such lane is the source.
+template <class F, int G>
+__global__ void k_dense_pull(DevGraph g, F f, const uint32_t *bm_cur,
+                             uint32_t *bm_next, unsigned long long *ctr) {
+  const uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  const uint32_t lig = (uint32_t)(gid % G);
+  const uint64_t v = gid / G;
+  const uint32_t glane0 = lane_id() & ~(uint32_t)(G - 1);
+  const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << glane0);
+  bool found = false;
+  uint32_t dv = 0;
+  if (v < g.n && f.cond((uint32_t)v)) {
+    dv = g.deg[v];
+    const uint32_t *row = g.pool + g.off[v];
+    for (uint32_t base = 0; base < dv; base += G) {
+      uint32_t k = base + lig;
+      bool hit = false;
+      uint32_t u = 0;
+      if (k < dv) {
+        u = row[k];
+        hit = (bm_cur[u >> 5] >> (u & 31)) & 1u;
+      }
+      uint64_t m = __ballot(hit) & gmask;
+      if (m) {
+        uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+        uint32_t su = __shfl(u, leader, kWave);
+        if (lig == 0) {
+          f.update(su, (uint32_t)v);
+          atomicOr(&bm_next[v >> 5], 1u << (v & 31));
+          found = true;
+        }
+        break;
+      }
+    }
+  }
+  bool take = found && lig == 0;
+  append(take, (uint32_t)v, dv, (uint32_t *)nullptr, ctr);
+}
+
+// ------------------------------------------------------------ vertex pull
+// Reduction over all neighbours of each vertex with G lanes per vertex
+// (G <= 64 via wave shuffles; G == 256 uses the whole block).  The lane
+// stride and the shuffle tree are fixed, so results do not depend on timing.
+template <class F, int G>
+__global__ void __launch_bounds__(kT)
+    k_vertex_pull(DevGraph g, const uint32_t *list, uint64_t count, F f) {
+  using Acc = typename F::Acc;
+  const uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  const uint64_t grp = (G == 256) ? blockIdx.x : gid / G;
+  const uint32_t lig = (G == 256) ? threadIdx.x : (uint32_t)(gid % G);
+  if (grp >= count) return;  // whole groups leave together
+  const uint32_t v = list ? list[grp] : (uint32_t)grp;
+  const uint32_t d = g.deg[v];
+  const uint32_t *row = g.pool + g.off[v];
+  Acc acc = f.zero();
+  for (uint32_t k = lig; k < d; k += G) f.add(acc, v, row[k]);
+  if (G > 1) {
+    const int w = (G < kWave) ? G : kWave;
+    for (int o = w / 2; o > 0; o >>= 1) acc = f.merge(acc, F::shfl_xor(acc, o));
+  }
+  if (G == 256) {
+    __shared__ Acc part[kT / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Acc a = part[0];
+      for (int i = 1; i < kT / kWave; i++) a = f.merge(a, part[i]);
+      f.store(v, a);
+    }
+    return;
+  }
+  if (lig == 0) f.store(v, acc);
+}
+
+}  // namespace em
+
+inline int edge_map_ctx(sstg_graph *g, EdgeMapCtx *ctx, int slot) {
+  ctx->words = ((uint64_t)g->n_rows + 31) / 32 + 1;
+  ctx->slot = slot;
+  SSTG_TRY(g->ws.get(slot + 0, ctx->words, &ctx->bm_cur));
+  SSTG_TRY(g->ws.get(slot + 1, ctx->words, &ctx->bm_next));
+  ctx->ctr = (unsigned long long *)(g->dev_ctr + 9);
+  return SSTG_OK;
+}
+
+inline int frontier_to_bitmap(EdgeMapCtx &ctx, const uint32_t *q, uint64_t nq,
+                              hipStream_t s) {
+  SSTG_LAUNCH(em::k_clear_u32, ceil_div(ctx.words, em::kT), em::kT, 0, s,
+              ctx.bm_cur, ctx.words);
+  if (nq)
+    SSTG_LAUNCH(em::k_queue_to_bitmap, ceil_div(nq, em::kT), em::kT, 0, s, q,
+                nq, ctx.bm_cur);
+  return SSTG_OK;
+}
+
+inline int narrow_u64_to_u32(const uint64_t *in, uint64_t n, uint32_t *out,
+                             hipStream_t s) {
+  if (n)
+    SSTG_LAUNCH(em::k_narrow, ceil_div(n, em::kT), em::kT, 0, s, in, n, out);
+  return SSTG_OK;
+}
+
+inline int read_result(sstg_graph *g, EdgeMapCtx &ctx, EdgeMapResult *res) {
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctx.ctr, 2 * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, g->stream));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  res->count = g->host_rb[0];
+  res->degree_sum = g->host_rb[1];
+  return SSTG_OK;
+}
+
+// Sparse push over the frontier queue front[0..nf) whose degree sum is mf;
+// the output frontier is appended at next.
+template <class F>
+int edge_map_sparse(sstg_graph *g, EdgeMapCtx &ctx, const uint32_t *front,
+                    uint64_t nf, uint64_t mf, F f, uint32_t *next,
+                    EdgeMapResult *res) {
+  hipStream_t s = g->stream;
+  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
+  if (nf && mf) {
+    SSTG_TRY(g->ws.get(ctx.slot + 2, nf + 1, &ctx.fpre));
+    SSTG_LAUNCH(em::k_frontier_degrees, ceil_div(nf + 1, em::kT), em::kT, 0, s,
+                g->view(), front, nf, ctx.fpre);
+    SSTG_TRY(scan_exclusive_u64(ctx.fpre, ctx.fpre, nf + 1, nullptr, g->ws,
+                                kWsPrim2, s));
+    SSTG_LAUNCH(em::k_sparse_push<F>, ceil_div(ceil_div(mf, em::kChunk), em::kT),
+                em::kT, 0, s, g->view(), front, (const uint64_t *)ctx.fpre, nf,
+                mf, f, next, ctx.ctr);
+  }
+  return read_result(g, ctx, res);
+}
+
+// Dense pull over all vertices against ctx.bm_cur; the output frontier is
+// ctx.bm_next, after which the two bitmaps swap roles.
+template <class F>
+int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
+  hipStream_t s = g->stream;
+  constexpr int G = 8;
+  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
+  SSTG_LAUNCH(em::k_clear_u32, ceil_div(ctx.words, em::kT), em::kT, 0, s,
+              ctx.bm_next, ctx.words);
+  uint64_t threads = (uint64_t)g->n_rows * G;
+  SSTG_LAUNCH((em::k_dense_pull<F, G>), ceil_div(threads, em::kT), em::kT, 0, s,
+              g->view(), f, (const uint32_t *)ctx.bm_cur, ctx.bm_next, ctx.ctr);
+  uint32_t *t = ctx.bm_cur;
+  ctx.bm_cur = ctx.bm_next;
+  ctx.bm_next = t;
+  return read_result(g, ctx, res);
+}
+
+// ----------------------------------------------------------- degree bins
+// Vertices split by degree for the vertex-pull kernels (cached per graph
+// version): bin 0 deg <= 8 (one lane), bin 1 <= 64 (8 lanes), bin 2 <= 2048
+// (a wave), bin 3 larger (a block).
+struct DegreeBins {
+  uint32_t *list[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t count[4] = {0, 0, 0, 0};
+};
+
+namespace em {
+__global__ void k_bin_flags(const uint32_t *deg, uint32_t n, uint32_t lo,
+                            uint32_t hi, uint8_t *flags) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) {
+    uint32_t d = deg[v];
+    flags[v] = d >= lo && d <= hi;
+  }
+}
+}  // namespace em
+
+inline int degree_bins(sstg_graph *g, DegreeBins *out) {
+  static const uint32_t lo[4] = {0, 9, 65, 2049};
+  static const uint32_t hi[4] = {8, 64, 2048, 0xFFFFFFFFu};
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  for (int b = 0; b < 4; b++)
+    SSTG_TRY(g->ws.get(kWsAlgo + 26 + b, (uint64_t)n + 1, &out->list[b]));
+  if (g->bins_version == g->version) {
+    for (int b = 0; b < 4; b++) out->count[b] = g->bins_count[b];
+    return SSTG_OK;
+  }
+  uint8_t *flags = nullptr;
+  uint64_t *tmp = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 24, n, &flags));
+  SSTG_TRY(g->ws.get(kWsAlgo + 25, n, &tmp));
+  for (int b = 0; b < 4; b++) {
+    SSTG_LAUNCH(em::k_bin_flags, ceil_div(n, em::kT), em::kT, 0, s,
+                (const uint32_t *)g->deg, n, lo[b], hi[b], flags);
+    SSTG_TRY(select_flagged_u64(flags, nullptr, n, tmp, g->dev_ctr + 7, g->ws,
+                                kWsPrim, s));
+    SSTG_HIP(hipMemcpyAsync(g->host_rb + 16, g->dev_ctr + 7, sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    SSTG_HIP(hipStreamSynchronize(s));
+    out->count[b] = g->host_rb[16];
+    SSTG_TRY(narrow_u64_to_u32(tmp, out->count[b], out->list[b], s));
+  }
+  for (int b = 0; b < 4; b++) g->bins_count[b] = out->count[b];
+  g->bins_version = g->version;
+  return SSTG_OK;
+}
+
+template <class F>
+int vertex_pull_all(sstg_graph *g, const DegreeBins &bins, F f) {
+  hipStream_t s = g->stream;
+  DevGraph v = g->view();
+  if (bins.count[0])
+    SSTG_LAUNCH((em::k_vertex_pull<F, 1>), ceil_div(bins.count[0], em::kT),
+                em::kT, 0, s, v, (const uint32_t *)bins.list[0], bins.count[0],
+                f);
+  if (bins.count[1])
+    SSTG_LAUNCH((em::k_vertex_pull<F, 8>), ceil_div(bins.count[1] * 8, em::kT),
+                em::kT, 0, s, v, (const uint32_t *)bins.list[1], bins.count[1],
+                f);
+  if (bins.count[2])
+    SSTG_LAUNCH((em::k_vertex_pull<F, 64>),
+                ceil_div(bins.count[2] * 64, em::kT), em::kT, 0, s, v,
+                (const uint32_t *)bins.list[2], bins.count[2], f);
+  if (bins.count[3])
+    SSTG_LAUNCH((em::k_vertex_pull<F, 256>), bins.count[3], em::kT, 0, s, v,
+                (const uint32_t *)bins.list[3], bins.count[3], f);
+  return SSTG_OK;
+}
+
+// Vertex pull over an explicit vertex list with one wave per vertex.
+template <class F>
+int vertex_pull_list(sstg_graph *g, const uint32_t *list, uint64_t count, F f) {
+  if (!count) return SSTG_OK;
+  SSTG_LAUNCH((em::k_vertex_pull<F, 64>), ceil_div(count * 64, em::kT), em::kT,
+              0, g->stream, g->view(), list, count, f);
+  return SSTG_OK;
+}
+
+// ------------------------------------------------------------ functors
+// PageRank pull: rank(v) = damping * sum_u contrib(u) + base and
+// contrib'(v) = rank(v) / deg(v).  With kKahan the per-lane sums carry a
+// compensation term and lanes are combined with Neumaier's two-sum.
+template <class Real, bool kKahan> struct PrPull;
+
+template <class Real> struct PrPull<Real, false> {
+  const Real *contrib;
+  Real *contrib_next;
+  Real *rank;
+  const uint32_t *deg;
+  Real damp, base;
+  using Acc = Real;
+  __device__ __forceinline__ Acc zero() const { return (Real)0; }
+  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t u) const {
+    a += contrib[u];
+  }
+  __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
+    return __shfl_xor(a, o, kWave);
+  }
+  __device__ __forceinline__ void store(uint32_t v, Acc a) const {
+    Real r = damp * a + base;
+    rank[v] = r;
+    uint32_t d = deg[v];
+    contrib_next[v] = d ? r / (Real)d : (Real)0;
+  }
+};
+
+struct KahanF {
+  float s, c;
+};
+
+template <> struct PrPull<float, true> {
+  const float *contrib;
+  float *contrib_next;
+  float *rank;
+  const uint32_t *deg;
+  float damp, base;
+  using Acc = KahanF;
+  __device__ __forceinline__ Acc zero() const { return KahanF{0.f, 0.f}; }
+  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t u) const {
+    float y = contrib[u] - a.c;
+    float t = a.s + y;
+    a.c = (t - a.s) - y;
+    a.s = t;
+  }
+  __device__ __forceinline__ Acc merge(Acc a, Acc b) const {
+    // Neumaier two-sum of the partial sums; compensations add up
+    float t = a.s + b.s;
+    float e = (fabsf(a.s) >= fabsf(b.s)) ? (a.s - t) + b.s : (b.s - t) + a.s;
+    return KahanF{t, a.c + b.c - e};
+  }
+  __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
+    return KahanF{__shfl_xor(a.s, o, kWave), __shfl_xor(a.c, o, kWave)};
+  }
+  __device__ __forceinline__ void store(uint32_t v, Acc a) const {
+    float sum = a.s - a.c;
+    float r = damp * sum + base;
+    rank[v] = r;
+    uint32_t d = deg[v];
+    contrib_next[v] = d ? r / (float)d : 0.f;
+  }
+};
+
+}  // namespace sstg

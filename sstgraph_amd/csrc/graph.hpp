@@ -1,0 +1,89 @@
+// Host-side handle of one device-resident graph.
+#pragma once
+
+#include "common.hpp"
+
+struct sstg_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t n_rows = 0, n_cols = 0;
+  uint64_t m = 0;             // live edges
+  uint64_t version = 0;       // bumped by every mutation
+  // tinyset store (see sstg::DevGraph)
+  uint64_t *off = nullptr;
+  uint32_t *deg = nullptr;
+  uint32_t *cap = nullptr;
+  uint32_t *pool = nullptr;
+  uint64_t pool_cap = 0;      // elements
+  uint64_t pool_tail = 0;     // first free element
+  uint64_t garbage = 0;       // elements in abandoned slots
+  uint64_t rebuilds = 0;
+  // per-call scratch
+  sstg::Workspace ws;
+  // small pinned host block for read-backs
+  uint64_t *host_rb = nullptr;
+  // device-side counters block (see update.hip)
+  uint64_t *dev_ctr = nullptr;
+  // cached derived data: exclusive prefix of deg (n_rows + 1 entries)
+  uint64_t *deg_prefix = nullptr;
+  uint64_t deg_prefix_version = ~0ull;
+  // cached degree bins of the vertex-pull kernels (lists live in workspace)
+  uint64_t bins_version = ~0ull;
+  uint64_t bins_count[4] = {0, 0, 0, 0};
+  // cached: does every stored edge have its reverse?
+  uint64_t sym_version = ~0ull;
+  bool symmetric = false;
+
+  sstg::DevGraph view() const {
+    return sstg::DevGraph{n_rows, off, deg, cap, pool};
+  }
+};
+
+namespace sstg {
+
+// update.hip
+int graph_alloc(sstg_graph *g);
+void graph_free(sstg_graph *g);
+int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                 uint64_t n, bool insert, bool skip_self, bool device_ptrs);
+int graph_rebuild(sstg_graph *g, uint64_t extra_elems);
+int graph_degree_prefix(sstg_graph *g, const uint64_t **out);
+int graph_export(sstg_graph *g, uint64_t *offsets, uint32_t *cols,
+                 bool device_ptrs);
+int graph_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
+                    uint64_t n, uint8_t *out, bool device_ptrs);
+int graph_touch_sum(sstg_graph *g, uint64_t *out);
+// algo.hip
+int graph_is_symmetric(sstg_graph *g, bool *out);
+
+// rmat.hip
+int rmat_generate(uint32_t n, uint32_t seed, double a, double b, double c,
+                  uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
+                  hipStream_t s);
+
+// Workspace slot map of one graph (each user owns a range of slots).
+enum : int {
+  kWsKeysA = 0,
+  kWsKeysB = 1,
+  kWsFlags = 2,
+  kWsBatch = 3,
+  kWsSegIdx = 4,
+  kWsSegV = 5,
+  kWsSegP0 = 6,
+  kWsSegStage = 7,
+  kWsSegAlloc = 8,
+  kWsSegWork = 9,
+  kWsSegOldDeg = 10,
+  kWsStage = 11,
+  kWsInSrc = 12,
+  kWsInDst = 13,
+  kWsMisc = 14,
+  kWsOut = 15,
+  kWsOut2 = 80,
+  kWsAlgo = 16,      // 16..47 algorithms
+  kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
+  kWsPrim2 = 64,     // 64..79 second primitive range
+  kWsSlots = 96,
+};
+
+}  // namespace sstg

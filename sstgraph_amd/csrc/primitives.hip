@@ -1,0 +1,321 @@
+// Scans, stream compaction and LSD radix sort for gfx950 (wave64).
+#include "primitives.hpp"
+#include "graph.hpp"
+
+namespace sstg {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+
+// Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
+// Returns the exclusive prefix; *total gets the block sum.
+template <class T>
+__device__ __forceinline__ T block_exclusive(T v, T *wtot, T *total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  T incl = wave_inclusive_sum(v);
+  if (lane == kWave - 1) wtot[w] = incl;
+  __syncthreads();
+  T woff = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanThreads / kWave; i++) {
+    T x = wtot[i];
+    if (i < w) woff += x;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return woff + incl - v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kScanThreads)
+    k_tile_reduce(const T *in, uint64_t n, T *partials) {
+  __shared__ T wtot[kScanThreads / kWave];
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  T acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; j++) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T t = 0;
+    for (int i = 0; i < kScanThreads / kWave; i++) t += wtot[i];
+    partials[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of each tile plus its offset; the block owning the last
+// tile also writes the grand total.
+template <class T>
+__global__ void __launch_bounds__(kScanThreads)
+    k_tile_scan(const T *in, T *out, uint64_t n, const T *offsets,
+                T *d_total) {
+  __shared__ T wtot[kScanThreads / kWave];
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  T carry = offsets ? offsets[blockIdx.x] : (T)0;
+  T vals[kScanItems];
+#pragma unroll
+  for (int j = 0; j < kScanItems; j++) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    vals[j] = (i < n) ? in[i] : (T)0;
+  }
+#pragma unroll
+  for (int j = 0; j < kScanItems; j++) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    T tot;
+    T ex = block_exclusive(vals[j], wtot, &tot);
+    if (i < n) out[i] = carry + ex;
+    carry += tot;
+  }
+  if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    *d_total = carry;
+}
+
+template <class T>
+int scan_impl(const T *in, T *out, uint64_t n, T *d_total, Workspace &ws,
+              int slot, hipStream_t s) {
+  if (n == 0) {
+    if (d_total) SSTG_HIP(hipMemsetAsync(d_total, 0, sizeof(T), s));
+    return SSTG_OK;
+  }
+  uint64_t tiles = ceil_div(n, kScanTile);
+  if (tiles == 1) {
+    SSTG_LAUNCH(k_tile_scan<T>, 1, kScanThreads, 0, s, in, out, n,
+                (const T *)nullptr, d_total);
+    return SSTG_OK;
+  }
+  if (slot >= kWsSlots + 16) return SSTG_E_INVAL;  // recursion depth guard
+  T *partials = nullptr;
+  SSTG_TRY(ws.get(slot, tiles, &partials));
+  SSTG_LAUNCH(k_tile_reduce<T>, tiles, kScanThreads, 0, s, in, n, partials);
+  SSTG_TRY(scan_impl<T>(partials, partials, tiles, (T *)nullptr, ws, slot + 1,
+                        s));
+  SSTG_LAUNCH(k_tile_scan<T>, tiles, kScanThreads, 0, s, in, out, n,
+              (const T *)partials, d_total);
+  return SSTG_OK;
+}
+
+// ------------------------------------------------------------- select
+__global__ void __launch_bounds__(kScanThreads)
+    k_count_flags(const uint8_t *flags, uint64_t n, uint64_t *counts) {
+  __shared__ uint64_t wtot[kScanThreads / kWave];
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; j++) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    if (i < n) acc += flags[i] ? 1 : 0;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int i = 0; i < kScanThreads / kWave; i++) t += wtot[i];
+    counts[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    k_select(const uint8_t *flags, const uint64_t *values, uint64_t n,
+             const uint64_t *offsets, uint64_t *out) {
+  __shared__ uint64_t wtot[kScanThreads / kWave];
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t carry = offsets[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanItems; j++) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    uint64_t f = (i < n && flags[i]) ? 1 : 0;
+    uint64_t tot;
+    uint64_t ex = block_exclusive(f, wtot, &tot);
+    if (f) out[carry + ex] = values ? values[i] : i;
+    carry += tot;
+  }
+}
+
+// --------------------------------------------------------- radix sort
+constexpr int kRsThreads = 256;
+constexpr int kRsWaves = kRsThreads / kWave;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;  // 4096 keys
+constexpr int kRsBins = 256;
+
+// Digit histogram of each tile; hist is digit-major: hist[d * tiles + t].
+__global__ void __launch_bounds__(kRsThreads)
+    k_rs_hist(const uint64_t *keys, uint64_t n, int shift, uint32_t *hist,
+              uint32_t tiles) {
+  __shared__ uint32_t h[kRsWaves][kRsBins];
+  const int w = threadIdx.x / kWave;
+  for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
+    (&h[0][0])[i] = 0;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+#pragma unroll 4
+  for (int j = 0; j < kRsItems; j++) {
+    uint64_t i = base + (uint64_t)j * kRsThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRsBins; d += kRsThreads) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int ww = 0; ww < kRsWaves; ww++) c += h[ww][d];
+    hist[(uint64_t)d * tiles + blockIdx.x] = c;
+  }
+}
+
+// Stable scatter of one tile.  Keys are read wave-striped (wave w, item j,
+// lane l -> tile offset w*64*ITEMS + j*64 + l), which is also their order for
+// stability.  Lanes with equal digits are found with 8 ballots; ranks are
+// counted per wave in LDS, turned into tile-local positions, the tile is
+// sorted in LDS and then written out digit run by digit run (coalesced).
+__global__ void __launch_bounds__(kRsThreads)
+    k_rs_scatter(const uint64_t *in, uint64_t *out, uint64_t n, int shift,
+                 const uint32_t *offs, uint32_t tiles) {
+  __shared__ uint64_t stage[kRsTile];
+  __shared__ uint32_t wcnt[kRsWaves][kRsBins];
+  __shared__ uint32_t tstart[kRsBins];  // tile-local start of each digit
+  __shared__ uint32_t gstart[kRsBins];  // global start of each digit
+  __shared__ uint32_t wtot[kRsWaves];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
+    (&wcnt[0][0])[i] = 0;
+  for (int d = threadIdx.x; d < kRsBins; d += kRsThreads)
+    gstart[d] = offs[(uint64_t)d * tiles + blockIdx.x];
+  __syncthreads();
+
+  const uint64_t base = (uint64_t)blockIdx.x * kRsTile +
+                        (uint64_t)w * (kWave * kRsItems);
+  uint64_t key[kRsItems];
+  uint32_t rank[kRsItems];
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    uint64_t i = base + (uint64_t)j * kWave + lane;
+    bool valid = i < n;
+    key[j] = valid ? in[i] : 0;
+    uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    uint32_t before = wcnt[w][d];
+    uint32_t r = mask_rank(peers);
+    rank[j] = before + r;
+    // the lowest lane of each peer group advances the wave's digit count
+    if (valid && r == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // per digit: exclusive offsets across waves, tile total
+  uint32_t dtot = 0;
+  if (threadIdx.x < kRsBins) {
+    const int d = threadIdx.x;
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < kRsWaves; ww++) {
+      uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = run;
+      run += c;
+    }
+    dtot = run;
+  }
+  // exclusive scan of digit totals over 256 digits (one per thread)
+  {
+    uint32_t incl = wave_inclusive_sum(dtot);
+    if (lane == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+#pragma unroll
+    for (int ww = 0; ww < kRsWaves; ww++)
+      if (ww < w) woff += wtot[ww];
+    if (threadIdx.x < kRsBins) tstart[threadIdx.x] = woff + incl - dtot;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    uint64_t i = base + (uint64_t)j * kWave + lane;
+    if (i < n) {
+      uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+      stage[tstart[d] + wcnt[w][d] + rank[j]] = key[j];
+    }
+  }
+  __syncthreads();
+  const uint64_t tbase = (uint64_t)blockIdx.x * kRsTile;
+  const uint32_t valid_n =
+      (uint32_t)((n - tbase) < (uint64_t)kRsTile ? (n - tbase) : kRsTile);
+  for (uint32_t i = threadIdx.x; i < valid_n; i += kRsThreads) {
+    uint64_t k = stage[i];
+    uint32_t d = (uint32_t)(k >> shift) & 255u;
+    out[(uint64_t)gstart[d] + (i - tstart[d])] = k;
+  }
+}
+
+}  // namespace
+
+int scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n,
+                       uint64_t *d_total, Workspace &ws, int ws_base,
+                       hipStream_t s) {
+  return scan_impl<uint64_t>(in, out, n, d_total, ws, ws_base, s);
+}
+
+int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
+                       uint32_t *d_total, Workspace &ws, int ws_base,
+                       hipStream_t s) {
+  return scan_impl<uint32_t>(in, out, n, d_total, ws, ws_base, s);
+}
+
+int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
+                       uint64_t n, uint64_t *out, uint64_t *d_count,
+                       Workspace &ws, int ws_base, hipStream_t s) {
+  if (n == 0) {
+    SSTG_HIP(hipMemsetAsync(d_count, 0, sizeof(uint64_t), s));
+    return SSTG_OK;
+  }
+  uint64_t tiles = ceil_div(n, kScanTile);
+  uint64_t *counts = nullptr;
+  SSTG_TRY(ws.get(ws_base, tiles, &counts));
+  SSTG_LAUNCH(k_count_flags, tiles, kScanThreads, 0, s, flags, n, counts);
+  SSTG_TRY(scan_exclusive_u64(counts, counts, tiles, d_count, ws, ws_base + 1,
+                              s));
+  SSTG_LAUNCH(k_select, tiles, kScanThreads, 0, s, flags, values, n,
+              (const uint64_t *)counts, out);
+  return SSTG_OK;
+}
+
+int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
+                   const int *shifts, int npass, Workspace &ws, int ws_base,
+                   hipStream_t s, uint64_t **result) {
+  *result = keys;
+  if (n <= 1 || npass == 0) return SSTG_OK;
+  if (n >= (1ull << 32)) return SSTG_E_INVAL;
+  uint32_t tiles = (uint32_t)ceil_div(n, kRsTile);
+  uint32_t *hist = nullptr;
+  SSTG_TRY(ws.get(ws_base, (uint64_t)tiles * kRsBins, &hist));
+  uint64_t *src = keys, *dst = tmp;
+  for (int p = 0; p < npass; p++) {
+    SSTG_LAUNCH(k_rs_hist, tiles, kRsThreads, 0, s, (const uint64_t *)src, n,
+                shifts[p], hist, tiles);
+    SSTG_TRY(scan_exclusive_u32(hist, hist, (uint64_t)tiles * kRsBins, nullptr,
+                                ws, ws_base + 1, s));
+    SSTG_LAUNCH(k_rs_scatter, tiles, kRsThreads, 0, s, (const uint64_t *)src,
+                dst, n, shifts[p], (const uint32_t *)hist, tiles);
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+  }
+  *result = src;
+  return SSTG_OK;
+}
+
+}  // namespace sstg

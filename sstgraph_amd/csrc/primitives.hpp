@@ -1,0 +1,36 @@
+// Device-wide primitives used by the update pipeline and the traversals:
+// exclusive scans, stream compaction and an LSD radix sort of 64-bit keys.
+#pragma once
+
+#include "common.hpp"
+
+namespace sstg {
+
+// out[i] = sum_{j<i} in[i]; in == out allowed.  If d_total is not null the
+// grand total is written there (device memory).
+int scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n,
+                       uint64_t *d_total, Workspace &ws, int ws_base,
+                       hipStream_t s);
+int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
+                       uint32_t *d_total, Workspace &ws, int ws_base,
+                       hipStream_t s);
+
+// Stream compaction driven by a byte flag per element.  Writes the kept
+// elements (values[i], or i itself when values is null) in order to out and
+// their number to *d_count (device).
+int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
+                       uint64_t n, uint64_t *out, uint64_t *d_count,
+                       Workspace &ws, int ws_base, hipStream_t s);
+
+// Stable LSD radix sort of n keys on 8-bit digits starting at the given bit
+// shifts (applied in order).  keys and tmp alternate as source and
+// destination; *result points at the buffer holding the sorted keys.
+// Requires n < 2^32.
+int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
+                   const int *shifts, int npass, Workspace &ws, int ws_base,
+                   hipStream_t s, uint64_t **result);
+
+// Workspace slots used by the primitives, starting at ws_base.
+constexpr int kPrimSlots = 8;
+
+}  // namespace sstg

@@ -1,0 +1,671 @@
+// Batch edge insert / delete on the device-resident tinyset store.
+//
+// Reference semantics: SparseMatrixV<true>::insert_batch (SparseMatrix.hpp:
+// 556-643) makes each row old U {y : (x,y) in batch, x != y}; remove_batch
+// (SparseMatrix.hpp:645-718) makes it old \ batch.  Duplicate inserts and
+// absent removes are no-ops (PMA.hpp:1139-1154, 1226-1228).
+//
+// Pipeline for one batch of n (src, dst) pairs, all on the graph's stream:
+//   1. pack      keys = src << 32 | dst, range check
+//   2. sort      LSD radix sort over the row and column bits
+//   3. mark      keep first copy of each key; drop x==y (insert); binary
+//                search the row: insert keeps absent keys, remove keeps
+//                present ones
+//   4. select    compact the kept keys -> B (sorted, unique, effective)
+//   5. heads     row boundaries in B -> one segment per touched row
+//   6. segprep   per row: new degree, in place or relocate, first moved
+//                position p0, staged suffix length, output work length
+//   7. scans     stage / alloc / work offsets (+ totals read back once)
+//   8. stage     copy each in-place row's suffix [p0, deg) to scratch
+//   9. merge     insert: merge-path merge of (suffix or whole old row) with
+//                the row's batch run into the row slot (or its new slot);
+//                remove: compact the staged suffix minus the batch run
+//  10. finalize  deg / off / cap of the touched rows
+// Steps 8 and 9 run over a flat index space (one thread per 8 outputs), so a
+// hub row with a million neighbours and a thousand one-edge rows are balanced
+// the same way.
+#include "graph.hpp"
+#include "primitives.hpp"
+
+namespace sstg {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kChunk = 8;  // outputs per thread in the flat kernels
+
+// Counter block (graph.dev_ctr):
+enum {
+  kCtrErr = 0,
+  kCtrB = 1,        // effective batch elements
+  kCtrT = 2,        // touched rows
+  kCtrStage = 3,    // staged elements
+  kCtrAlloc = 4,    // elements allocated at the pool tail
+  kCtrWork = 5,     // flat work items of the merge step
+  kCtrGarbage = 6,  // elements in slots abandoned by relocation
+  kCtrScratch = 7,
+  kCtrCount = 16,
+};
+
+__device__ __forceinline__ uint64_t grow_cap(uint64_t d, uint32_t n_cols) {
+  uint64_t c = 4;
+  while (c < d) c <<= 1;
+  uint64_t lim = n_cols < 4 ? 4 : n_cols;
+  if (c > lim && d <= lim) c = lim;
+  return c;
+}
+
+__global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n,
+                       uint32_t n_rows, uint32_t n_cols, uint64_t *keys,
+                       uint64_t *ctr) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s = src[i], d = dst[i];
+  if (s >= n_rows || d >= n_cols) {
+    ctr[kCtrErr] = 1;
+    s = 0;
+    d = 0;
+  }
+  keys[i] = ((uint64_t)s << 32) | d;
+}
+
+__global__ void k_mark(const uint64_t *keys, uint64_t n, DevGraph g,
+                       int insert, int skip_self, uint8_t *flags) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  bool keep = (i == 0) || (keys[i - 1] != k);
+  uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)k;
+  if (insert && skip_self && s == d) keep = false;
+  if (keep) {
+    uint32_t dg = g.deg[s];
+    bool exists = false;
+    if (dg) {
+      const uint32_t *row = g.pool + g.off[s];
+      uint64_t p = lower_bound_dev(row, (uint64_t)dg, d);
+      exists = p < dg && row[p] == d;
+    }
+    keep = insert ? !exists : exists;
+  }
+  flags[i] = keep ? 1 : 0;
+}
+
+__global__ void k_heads(const uint64_t *B, const uint64_t *ctr, uint64_t nmax,
+                        uint8_t *flags) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nmax) return;
+  uint64_t b = ctr[kCtrB];
+  flags[i] = (i < b) && (i == 0 || (B[i] >> 32) != (B[i - 1] >> 32));
+}
+
+// One thread per touched row (s < t); rows s in [t, nmax] get zero lengths so
+// the scans over nmax + 1 entries give exact offsets and totals.
+__global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
+                          const uint64_t *seg_idx, DevGraph g, int insert,
+                          uint32_t n_cols, uint64_t nmax, uint32_t *seg_v,
+                          uint32_t *seg_p0, uint32_t *seg_olddeg,
+                          uint64_t *seg_stage, uint64_t *seg_alloc,
+                          uint64_t *seg_work, uint64_t *garbage) {
+  uint64_t s = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (s > nmax) return;
+  uint64_t t = ctr[kCtrT], b = ctr[kCtrB];
+  if (s >= t) {
+    seg_stage[s] = 0;
+    seg_alloc[s] = 0;
+    seg_work[s] = 0;
+    return;
+  }
+  uint64_t bs = seg_idx[s], be = (s + 1 < t) ? seg_idx[s + 1] : b;
+  uint64_t blen = be - bs;
+  uint32_t v = (uint32_t)(B[bs] >> 32);
+  uint32_t first = (uint32_t)B[bs];
+  uint32_t od = g.deg[v], cp = g.cap[v];
+  uint64_t p0 = 0, stage = 0, alloc = 0, work = 0;
+  if (insert) {
+    uint64_t nd = (uint64_t)od + blen;
+    if (nd > cp) {  // relocate: whole row merged into a fresh slot
+      alloc = grow_cap(nd, n_cols);
+      work = nd;
+      if (cp) atomicAdd((unsigned long long *)garbage, (unsigned long long)cp);
+    } else {
+      p0 = od ? lower_bound_dev(g.pool + g.off[v], (uint64_t)od, first) : 0;
+      stage = od - p0;
+      work = stage + blen;
+    }
+  } else {
+    p0 = lower_bound_dev(g.pool + g.off[v], (uint64_t)od, first);
+    stage = od - p0;
+    work = stage;
+  }
+  seg_v[s] = v;
+  seg_p0[s] = (uint32_t)p0;
+  seg_olddeg[s] = od;
+  seg_stage[s] = stage;
+  seg_alloc[s] = alloc;
+  seg_work[s] = work;
+}
+
+struct SegArgs {
+  const uint64_t *B;
+  const uint64_t *seg_idx;
+  const uint32_t *seg_v;
+  const uint32_t *seg_p0;
+  const uint32_t *seg_olddeg;
+  const uint64_t *stage_off;  // exclusive, t + 1 valid entries
+  const uint64_t *alloc_off;
+  const uint64_t *work_off;
+  uint64_t t, b;
+  uint64_t pool_tail;
+  uint32_t *scratch;
+};
+
+__global__ void k_stage(SegArgs a, DevGraph g, uint64_t total) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  if (e >= total) return;
+  uint64_t s = find_segment(a.stage_off, a.t, e);
+  uint64_t end = e + kChunk < total ? e + kChunk : total;
+  for (; e < end; e++) {
+    while (e >= a.stage_off[s + 1]) s++;
+    uint32_t v = a.seg_v[s];
+    a.scratch[e] = g.pool[g.off[v] + a.seg_p0[s] + (e - a.stage_off[s])];
+  }
+}
+
+struct MergeSeg {
+  const uint32_t *A;
+  uint64_t alen;
+  const uint64_t *Bk;
+  uint64_t blen;
+  uint32_t *dst;
+  uint64_t begin, end;  // flat work range
+};
+
+__device__ __forceinline__ MergeSeg load_seg(const SegArgs &a, const DevGraph &g,
+                                             uint64_t s, bool insert) {
+  MergeSeg m;
+  uint64_t bs = a.seg_idx[s], be = (s + 1 < a.t) ? a.seg_idx[s + 1] : a.b;
+  m.Bk = a.B + bs;
+  m.blen = be - bs;
+  uint32_t v = a.seg_v[s];
+  bool reloc = insert && (a.alloc_off[s + 1] > a.alloc_off[s]);
+  if (reloc) {
+    m.A = g.pool + g.off[v];
+    m.alen = a.seg_olddeg[s];
+    m.dst = g.pool + a.pool_tail + a.alloc_off[s];
+  } else {
+    m.A = a.scratch + a.stage_off[s];
+    m.alen = a.stage_off[s + 1] - a.stage_off[s];
+    m.dst = g.pool + g.off[v] + a.seg_p0[s];
+  }
+  m.begin = a.work_off[s];
+  m.end = a.work_off[s + 1];
+  return m;
+}
+
+__global__ void k_merge_insert(SegArgs a, DevGraph g, uint64_t total) {
+  uint64_t o = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  if (o >= total) return;
+  uint64_t end = o + kChunk < total ? o + kChunk : total;
+  uint64_t s = find_segment(a.work_off, a.t, o);
+  MergeSeg m = load_seg(a, g, s, true);
+  // merge-path split of output index j between A and the row's batch run
+  uint64_t j = o - m.begin;
+  uint64_t lo = j > m.blen ? j - m.blen : 0, hi = j < m.alen ? j : m.alen;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (m.A[mid] < (uint32_t)m.Bk[j - 1 - mid])
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  uint64_t i = lo, bi = j - lo;
+  for (; o < end; o++) {
+    while (o >= m.end) {
+      s++;
+      m = load_seg(a, g, s, true);
+      j = 0;
+      i = 0;
+      bi = 0;
+    }
+    uint32_t x;
+    if (bi >= m.blen || (i < m.alen && m.A[i] < (uint32_t)m.Bk[bi]))
+      x = m.A[i++];
+    else
+      x = (uint32_t)m.Bk[bi++];
+    m.dst[j++] = x;
+  }
+}
+
+__global__ void k_compact_remove(SegArgs a, DevGraph g, uint64_t total) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  if (e >= total) return;
+  uint64_t end = e + kChunk < total ? e + kChunk : total;
+  uint64_t s = find_segment(a.work_off, a.t, e);
+  MergeSeg m = load_seg(a, g, s, false);
+  uint64_t i = e - m.begin;
+  // batch elements below A[i] were removed before position i
+  uint64_t rb = 0;
+  {
+    uint64_t lo = 0, hi = m.blen;
+    uint32_t x = m.A[i];
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if ((uint32_t)m.Bk[mid] < x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    rb = lo;
+  }
+  for (; e < end; e++) {
+    while (e >= m.end) {
+      s++;
+      m = load_seg(a, g, s, false);
+      i = 0;
+      rb = 0;
+    }
+    uint32_t x = m.A[i];
+    if (rb < m.blen && (uint32_t)m.Bk[rb] == x)
+      rb++;
+    else
+      m.dst[i - rb] = x;
+    i++;
+  }
+}
+
+__global__ void k_finalize(SegArgs a, DevGraph g, int insert) {
+  uint64_t s = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= a.t) return;
+  uint64_t bs = a.seg_idx[s], be = (s + 1 < a.t) ? a.seg_idx[s + 1] : a.b;
+  uint32_t blen = (uint32_t)(be - bs);
+  uint32_t v = a.seg_v[s];
+  uint32_t od = a.seg_olddeg[s];
+  g.deg[v] = insert ? od + blen : od - blen;
+  if (insert && a.alloc_off[s + 1] > a.alloc_off[s]) {
+    g.off[v] = a.pool_tail + a.alloc_off[s];
+    g.cap[v] = (uint32_t)(a.alloc_off[s + 1] - a.alloc_off[s]);
+  }
+}
+
+// ----------------------------------------------------------- rebuild etc.
+__global__ void k_deg_to_u64(const uint32_t *deg, uint64_t n, uint64_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) out[i] = deg[i];
+  if (i == n) out[n] = 0;
+}
+
+__global__ void k_newcap(const uint32_t *deg, const uint32_t *cap, uint64_t n,
+                         uint32_t n_cols, int recompute, uint32_t *new_cap,
+                         uint64_t *cap64) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    cap64[n] = 0;
+    return;
+  }
+  uint32_t c = cap[i];
+  if (recompute) c = deg[i] ? (uint32_t)grow_cap(deg[i], n_cols) : 0;
+  new_cap[i] = c;
+  cap64[i] = c;
+}
+
+// Copies every row (deg[v] ids) to dst_base(v) + k, flat over all edges.
+// dst_off == nullptr: dense output at the row's degree prefix.
+__global__ void k_gather_rows(DevGraph g, const uint64_t *prefix, uint64_t m,
+                              uint32_t *out, const uint64_t *dst_off) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  if (e >= m) return;
+  uint64_t end = e + kChunk < m ? e + kChunk : m;
+  uint64_t v = find_segment(prefix, g.n, e);
+  for (; e < end; e++) {
+    while (e >= prefix[v + 1]) v++;
+    uint64_t k = e - prefix[v];
+    uint64_t d = dst_off ? dst_off[v] + k : e;
+    out[d] = g.pool[g.off[v] + k];
+  }
+}
+
+__global__ void k_has(DevGraph g, const uint32_t *rows, const uint32_t *cols,
+                      uint64_t n, uint32_t n_cols, uint8_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  uint32_t r = rows[i], c = cols[i];
+  uint8_t h = 0;
+  if (r < g.n && c < n_cols) {
+    uint32_t dg = g.deg[r];
+    if (dg) {
+      const uint32_t *row = g.pool + g.off[r];
+      uint64_t p = lower_bound_dev(row, (uint64_t)dg, c);
+      h = p < dg && row[p] == c;
+    }
+  }
+  out[i] = h;
+}
+
+__global__ void k_touch_sum(DevGraph g, const uint64_t *prefix, uint64_t m,
+                            unsigned long long *out) {
+  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  uint64_t acc = 0;
+  if (e < m) {
+    uint64_t end = e + kChunk < m ? e + kChunk : m;
+    uint64_t v = find_segment(prefix, g.n, e);
+    for (; e < end; e++) {
+      while (e >= prefix[v + 1]) v++;
+      acc += g.pool[g.off[v] + (e - prefix[v])];
+    }
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & (kWave - 1)) == 0 && acc)
+    atomicAdd(out, (unsigned long long)acc);
+}
+
+inline uint64_t blocks(uint64_t n) { return ceil_div(n ? n : 1, kT); }
+inline uint64_t chunk_blocks(uint64_t n) {
+  return ceil_div(ceil_div(n ? n : 1, kChunk), kT);
+}
+
+}  // namespace
+
+int graph_alloc(sstg_graph *g) {
+  uint64_t n = g->n_rows ? g->n_rows : 1;
+  SSTG_HIP(hipMalloc(&g->off, n * sizeof(uint64_t)));
+  SSTG_HIP(hipMalloc(&g->deg, n * sizeof(uint32_t)));
+  SSTG_HIP(hipMalloc(&g->cap, n * sizeof(uint32_t)));
+  SSTG_HIP(hipMemsetAsync(g->off, 0, n * sizeof(uint64_t), g->stream));
+  SSTG_HIP(hipMemsetAsync(g->deg, 0, n * sizeof(uint32_t), g->stream));
+  SSTG_HIP(hipMemsetAsync(g->cap, 0, n * sizeof(uint32_t), g->stream));
+  SSTG_HIP(hipMalloc(&g->dev_ctr, kCtrCount * sizeof(uint64_t)));
+  SSTG_HIP(hipHostMalloc(&g->host_rb, 64 * sizeof(uint64_t),
+                         hipHostMallocDefault));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  return SSTG_OK;
+}
+
+void graph_free(sstg_graph *g) {
+  if (g->stream) hipStreamSynchronize(g->stream);
+  hipFree(g->off);
+  hipFree(g->deg);
+  hipFree(g->cap);
+  hipFree(g->pool);
+  hipFree(g->dev_ctr);
+  hipFree(g->deg_prefix);
+  if (g->host_rb) hipHostFree(g->host_rb);
+  g->ws.release_all();
+  g->off = nullptr;
+  g->deg = g->cap = g->pool = nullptr;
+  g->dev_ctr = g->deg_prefix = g->host_rb = nullptr;
+}
+
+// Moves every row into a fresh pool laid out in row order (the root-level
+// rebalance of the packed-memory store).  recompute: reset each slot to
+// grow_cap(deg) (reclaims slack left by deletes); otherwise slot sizes are
+// kept.  extra: elements to leave free at the tail.
+static int rebuild_impl(sstg_graph *g, uint64_t extra, bool recompute) {
+  hipStream_t s = g->stream;
+  const uint64_t n = g->n_rows;
+  uint32_t *new_cap = nullptr;
+  uint64_t *new_off = nullptr, *prefix = nullptr;
+  SSTG_HIP(hipMalloc(&new_cap, (n + 1) * sizeof(uint32_t)));
+  SSTG_HIP(hipMalloc(&new_off, (n + 1) * sizeof(uint64_t)));
+  SSTG_LAUNCH(k_newcap, blocks(n + 1), kT, 0, s, (const uint32_t *)g->deg,
+              (const uint32_t *)g->cap, n, g->n_cols, recompute ? 1 : 0,
+              new_cap, new_off);
+  SSTG_TRY(scan_exclusive_u64(new_off, new_off, n + 1, g->dev_ctr + kCtrScratch,
+                              g->ws, kWsPrim, s));
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, g->dev_ctr + kCtrScratch,
+                          sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  uint64_t total = g->host_rb[0];
+  uint64_t want = total + extra;
+  uint64_t new_cap_elems = want + want / 4 + (1u << 20);
+  uint32_t *new_pool = nullptr;
+  hipError_t e = hipMalloc(&new_pool, new_cap_elems * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    new_cap_elems = want + 1024;
+    SSTG_HIP(hipMalloc(&new_pool, new_cap_elems * sizeof(uint32_t)));
+  }
+  if (g->m) {
+    SSTG_HIP(hipMalloc(&prefix, (n + 1) * sizeof(uint64_t)));
+    SSTG_LAUNCH(k_deg_to_u64, blocks(n + 1), kT, 0, s, (const uint32_t *)g->deg,
+                n, prefix);
+    SSTG_TRY(scan_exclusive_u64(prefix, prefix, n + 1, nullptr, g->ws,
+                                kWsPrim, s));
+    SSTG_LAUNCH(k_gather_rows, chunk_blocks(g->m), kT, 0, s, g->view(),
+                (const uint64_t *)prefix, g->m, new_pool,
+                (const uint64_t *)new_off);
+  }
+  SSTG_HIP(hipStreamSynchronize(s));
+  hipFree(prefix);
+  hipFree(g->pool);
+  g->pool = new_pool;
+  g->pool_cap = new_cap_elems;
+  g->pool_tail = total;
+  g->garbage = 0;
+  SSTG_HIP(hipMemcpyAsync(g->off, new_off, n * sizeof(uint64_t),
+                          hipMemcpyDeviceToDevice, s));
+  SSTG_HIP(hipMemcpyAsync(g->cap, new_cap, n * sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  hipFree(new_cap);
+  hipFree(new_off);
+  g->rebuilds++;
+  return SSTG_OK;
+}
+
+int graph_rebuild(sstg_graph *g, uint64_t extra_elems) {
+  return rebuild_impl(g, extra_elems, true);
+}
+
+static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                        uint64_t n, bool insert, bool skip_self) {
+  hipStream_t s = g->stream;
+  Workspace &ws = g->ws;
+  uint64_t *ctr = g->dev_ctr;
+  // density rule at the root: once abandoned slots exceed half of the used
+  // pool, compact before taking more space
+  if (insert && g->garbage > 0 && g->garbage * 2 > g->pool_tail)
+    SSTG_TRY(rebuild_impl(g, 0, true));
+
+  uint64_t *keys = nullptr, *tmp = nullptr, *B = nullptr, *seg_idx = nullptr;
+  uint8_t *flags = nullptr;
+  SSTG_TRY(ws.get(kWsKeysA, n, &keys));
+  SSTG_TRY(ws.get(kWsKeysB, n, &tmp));
+  SSTG_TRY(ws.get(kWsFlags, n + 1, &flags));
+  SSTG_HIP(hipMemsetAsync(ctr, 0, kCtrCount * sizeof(uint64_t), s));
+  SSTG_LAUNCH(k_pack, blocks(n), kT, 0, s, src, dst, n, g->n_rows, g->n_cols,
+              keys, ctr);
+  int shifts[16], np = 0;
+  uint32_t cb = bit_width_u32(g->n_cols ? g->n_cols - 1 : 0);
+  uint32_t rb = bit_width_u32(g->n_rows ? g->n_rows - 1 : 0);
+  for (uint32_t b = 0; b < cb; b += 8) shifts[np++] = (int)b;
+  for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
+  uint64_t *sorted = nullptr;
+  SSTG_TRY(radix_sort_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  SSTG_LAUNCH(k_mark, blocks(n), kT, 0, s, (const uint64_t *)sorted, n,
+              g->view(), insert ? 1 : 0, skip_self ? 1 : 0, flags);
+  // B reuses the buffer that does not hold the sorted keys
+  B = (sorted == keys) ? tmp : keys;
+  SSTG_TRY(select_flagged_u64(flags, sorted, n, B, ctr + kCtrB, ws, kWsPrim,
+                              s));
+  // at most one segment per row: segment arrays hold nseg + 1 entries
+  const uint64_t nseg = n < (uint64_t)g->n_rows ? n : (uint64_t)g->n_rows;
+  SSTG_TRY(ws.get(kWsSegIdx, nseg + 1, &seg_idx));
+  SSTG_LAUNCH(k_heads, blocks(n), kT, 0, s, (const uint64_t *)B,
+              (const uint64_t *)ctr, n, flags);
+  SSTG_TRY(select_flagged_u64(flags, nullptr, n, seg_idx, ctr + kCtrT, ws,
+                              kWsPrim, s));
+  uint32_t *seg_v = nullptr, *seg_p0 = nullptr, *seg_od = nullptr;
+  uint64_t *seg_stage = nullptr, *seg_alloc = nullptr, *seg_work = nullptr;
+  SSTG_TRY(ws.get(kWsSegV, nseg + 1, &seg_v));
+  SSTG_TRY(ws.get(kWsSegP0, nseg + 1, &seg_p0));
+  SSTG_TRY(ws.get(kWsSegOldDeg, nseg + 1, &seg_od));
+  SSTG_TRY(ws.get(kWsSegStage, nseg + 1, &seg_stage));
+  SSTG_TRY(ws.get(kWsSegAlloc, nseg + 1, &seg_alloc));
+  SSTG_TRY(ws.get(kWsSegWork, nseg + 1, &seg_work));
+  SSTG_LAUNCH(k_segprep, blocks(nseg + 1), kT, 0, s, (const uint64_t *)B,
+              (const uint64_t *)ctr, (const uint64_t *)seg_idx, g->view(),
+              insert ? 1 : 0, g->n_cols, nseg, seg_v, seg_p0, seg_od,
+              seg_stage, seg_alloc, seg_work, ctr + kCtrGarbage);
+  SSTG_TRY(scan_exclusive_u64(seg_stage, seg_stage, nseg + 1, ctr + kCtrStage,
+                              ws, kWsPrim, s));
+  SSTG_TRY(scan_exclusive_u64(seg_alloc, seg_alloc, nseg + 1, ctr + kCtrAlloc,
+                              ws, kWsPrim, s));
+  SSTG_TRY(scan_exclusive_u64(seg_work, seg_work, nseg + 1, ctr + kCtrWork, ws,
+                              kWsPrim, s));
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, kCtrCount * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  const uint64_t *h = g->host_rb;
+  if (h[kCtrErr]) return SSTG_E_RANGE;
+  const uint64_t b = h[kCtrB], t = h[kCtrT];
+  const uint64_t stage_tot = h[kCtrStage], alloc_tot = h[kCtrAlloc],
+                 work_tot = h[kCtrWork], garbage_add = h[kCtrGarbage];
+  if (b == 0) return SSTG_OK;
+  if (g->pool_tail + alloc_tot > g->pool_cap) {
+    // slot sizes are kept, so the decisions made above stay valid
+    SSTG_TRY(rebuild_impl(g, alloc_tot, false));
+  }
+  uint32_t *scratch = nullptr;
+  SSTG_TRY(ws.get(kWsStage, stage_tot + 1, &scratch));
+  SegArgs a;
+  a.B = B;
+  a.seg_idx = seg_idx;
+  a.seg_v = seg_v;
+  a.seg_p0 = seg_p0;
+  a.seg_olddeg = seg_od;
+  a.stage_off = seg_stage;
+  a.alloc_off = seg_alloc;
+  a.work_off = seg_work;
+  a.t = t;
+  a.b = b;
+  a.pool_tail = g->pool_tail;
+  a.scratch = scratch;
+  if (stage_tot)
+    SSTG_LAUNCH(k_stage, chunk_blocks(stage_tot), kT, 0, s, a, g->view(),
+                stage_tot);
+  if (work_tot) {
+    if (insert)
+      SSTG_LAUNCH(k_merge_insert, chunk_blocks(work_tot), kT, 0, s, a,
+                  g->view(), work_tot);
+    else
+      SSTG_LAUNCH(k_compact_remove, chunk_blocks(work_tot), kT, 0, s, a,
+                  g->view(), work_tot);
+  }
+  SSTG_LAUNCH(k_finalize, blocks(t), kT, 0, s, a, g->view(), insert ? 1 : 0);
+  SSTG_HIP(hipStreamSynchronize(s));
+  g->pool_tail += alloc_tot;
+  g->garbage += garbage_add;
+  g->m = insert ? g->m + b : g->m - b;
+  g->version++;
+  return SSTG_OK;
+}
+
+int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                 uint64_t n, bool insert, bool skip_self, bool device_ptrs) {
+  if (n == 0) return SSTG_OK;
+  // batches above 2^31 pairs are applied in pieces; union and difference are
+  // unaffected by the split
+  const uint64_t kMaxChunk = 1ull << 31;
+  for (uint64_t base = 0; base < n; base += kMaxChunk) {
+    uint64_t len = n - base < kMaxChunk ? n - base : kMaxChunk;
+    const uint32_t *s = src + base, *d = dst + base;
+    if (!device_ptrs) {
+      uint32_t *ds = nullptr, *dd = nullptr;
+      SSTG_TRY(g->ws.get(kWsInSrc, len, &ds));
+      SSTG_TRY(g->ws.get(kWsInDst, len, &dd));
+      SSTG_HIP(hipMemcpyAsync(ds, s, len * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, g->stream));
+      SSTG_HIP(hipMemcpyAsync(dd, d, len * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, g->stream));
+      s = ds;
+      d = dd;
+    }
+    SSTG_TRY(update_chunk(g, s, d, len, insert, skip_self));
+  }
+  return SSTG_OK;
+}
+
+int graph_degree_prefix(sstg_graph *g, const uint64_t **out) {
+  if (!g->deg_prefix) {
+    SSTG_HIP(hipMalloc(&g->deg_prefix, ((uint64_t)g->n_rows + 1) *
+                                           sizeof(uint64_t)));
+    g->deg_prefix_version = ~0ull;
+  }
+  if (g->deg_prefix_version != g->version) {
+    uint64_t n = g->n_rows;
+    SSTG_LAUNCH(k_deg_to_u64, blocks(n + 1), kT, 0, g->stream,
+                (const uint32_t *)g->deg, n, g->deg_prefix);
+    SSTG_TRY(scan_exclusive_u64(g->deg_prefix, g->deg_prefix, n + 1, nullptr,
+                                g->ws, kWsPrim, g->stream));
+    g->deg_prefix_version = g->version;
+  }
+  *out = g->deg_prefix;
+  return SSTG_OK;
+}
+
+int graph_export(sstg_graph *g, uint64_t *offsets, uint32_t *cols,
+                 bool device_ptrs) {
+  const uint64_t *prefix = nullptr;
+  SSTG_TRY(graph_degree_prefix(g, &prefix));
+  hipStream_t s = g->stream;
+  uint64_t n = g->n_rows;
+  if (offsets) {
+    SSTG_HIP(hipMemcpyAsync(offsets, prefix, (n + 1) * sizeof(uint64_t),
+                            device_ptrs ? hipMemcpyDeviceToDevice
+                                        : hipMemcpyDeviceToHost,
+                            s));
+  }
+  if (cols && g->m) {
+    uint32_t *dcols = cols;
+    if (!device_ptrs) SSTG_TRY(g->ws.get(kWsMisc, g->m, &dcols));
+    SSTG_LAUNCH(k_gather_rows, chunk_blocks(g->m), kT, 0, s, g->view(), prefix,
+                g->m, dcols, (const uint64_t *)nullptr);
+    if (!device_ptrs)
+      SSTG_HIP(hipMemcpyAsync(cols, dcols, g->m * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+  }
+  SSTG_HIP(hipStreamSynchronize(s));
+  return SSTG_OK;
+}
+
+int graph_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
+                    uint64_t n, uint8_t *out, bool device_ptrs) {
+  if (n == 0) return SSTG_OK;
+  hipStream_t s = g->stream;
+  const uint32_t *r = rows, *c = cols;
+  uint8_t *o = out;
+  if (!device_ptrs) {
+    uint32_t *dr = nullptr, *dc = nullptr;
+    SSTG_TRY(g->ws.get(kWsInSrc, n, &dr));
+    SSTG_TRY(g->ws.get(kWsInDst, n, &dc));
+    SSTG_TRY(g->ws.get(kWsFlags, n, &o));
+    SSTG_HIP(hipMemcpyAsync(dr, rows, n * 4, hipMemcpyHostToDevice, s));
+    SSTG_HIP(hipMemcpyAsync(dc, cols, n * 4, hipMemcpyHostToDevice, s));
+    r = dr;
+    c = dc;
+  }
+  SSTG_LAUNCH(k_has, blocks(n), kT, 0, s, g->view(), r, c, n, g->n_cols, o);
+  if (!device_ptrs)
+    SSTG_HIP(hipMemcpyAsync(out, o, n, hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  return SSTG_OK;
+}
+
+int graph_touch_sum(sstg_graph *g, uint64_t *out) {
+  const uint64_t *prefix = nullptr;
+  SSTG_TRY(graph_degree_prefix(g, &prefix));
+  hipStream_t s = g->stream;
+  unsigned long long *acc = (unsigned long long *)(g->dev_ctr + kCtrScratch);
+  SSTG_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), s));
+  if (g->m)
+    SSTG_LAUNCH(k_touch_sum, chunk_blocks(g->m), kT, 0, s, g->view(), prefix,
+                g->m, acc);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, acc, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  *out = g->host_rb[0];
+  return SSTG_OK;
+}
+
+}  // namespace sstg

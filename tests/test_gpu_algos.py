@@ -1,0 +1,161 @@
+"""BFS, PageRank, BC, CC, TC and BF on the device vs the CPU oracle.
+BFS levels, the CC labels and the triangle count are bit-exact; FP64 PR and
+BC agree within 1e-9 relative (atol 1e-12 for exact zeros)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import sstgraph_amd as S
+from oracle import oracle as O
+from tests.gpu_common import need_gpu, rel_close, say
+from tests.graphs import GOLDEN, csr_from_edges, random_graph, read_mtx
+
+pytestmark = pytest.mark.gpu
+EMV = S.EdgeMapVertexMap
+RTOL = 1e-9
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    need_gpu()
+
+
+def _device_graph(n, off, cols):
+    g = S.SparseMatrixV(n)
+    rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
+    g.insert_batch(rows, cols)
+    return g
+
+
+@pytest.fixture(scope="module")
+def c1():
+    need_gpu()
+    n, s, d = O.rmat_graph_edges(16, 16, 0)
+    og = O.Graph(n)
+    og.insert_batch(s, d)
+    off, cols = og.csr()
+    g = S.SparseMatrixV(n)
+    g.insert_batch(s, d)
+    return n, g, off, cols
+
+
+def _check_bfs(g, off, cols, src):
+    parent, level = EMV.BFS(g, src, levels=True)
+    olev, _ = O.bfs(off, cols, src)
+    assert np.array_equal(level, olev), f"BFS levels from {src}"
+    reached = olev != S.UNREACHED
+    assert np.array_equal(parent >= 0, reached)
+    assert parent[src] == src
+    v = np.nonzero(reached)[0]
+    v = v[v != src]
+    p = parent[v]
+    assert np.all(level[p] + 1 == level[v])
+    # the tree edge runs parent -> v
+    for x, px in list(zip(v.tolist(), p.tolist()))[:: max(1, len(v) // 2000)]:
+        assert x in cols[off[px]:off[px + 1]]
+
+
+def test_c1_bfs_levels_match(c1):
+    n, g, off, cols = c1
+    gold = json.load(open(os.path.join(GOLDEN, "graph_goldens.json")))
+    gold = gold["rmat_s16_ef16_seed0"]
+    for src in [0, 1, 12345, n - 1]:
+        say("C1 bfs", src)
+        _check_bfs(g, off, cols, src)
+    import hashlib
+    _, level = EMV.BFS(g, 0, levels=True)
+    assert hashlib.sha256(level.tobytes()).hexdigest() == gold["sha_levels"]
+
+
+def test_c1_cc_tc_match(c1):
+    n, g, off, cols = c1
+    gold = json.load(open(os.path.join(GOLDEN, "graph_goldens.json")))
+    gold = gold["rmat_s16_ef16_seed0"]
+    say("C1 cc")
+    lab = EMV.CC(g)
+    assert np.array_equal(lab, O.cc(off, cols))
+    say("C1 tc")
+    assert EMV.TC(g) == O.tc(off, cols) == gold["triangles"]
+
+
+def test_c1_pagerank_within_1e9(c1):
+    n, g, off, cols = c1
+    for iters in [1, 10, 20]:
+        say("C1 pr", iters)
+        got = EMV.PR_S(g, iters, 0.85)
+        want = O.pagerank(off, cols, iters, 0.85)
+        ok, worst = rel_close(got, want, RTOL)
+        assert ok, f"PR iters={iters} worst rel err {worst}"
+
+
+def test_c1_bc_within_1e9(c1):
+    n, g, off, cols = c1
+    for src in [0, 777]:
+        say("C1 bc", src)
+        got = EMV.BC(g, src)
+        want = O.bc(off, cols, src)
+        ok, worst = rel_close(got, want, RTOL, atol_floor=1e-12)
+        assert ok, f"BC src={src} worst rel err {worst}"
+
+
+def test_c1_bf_and_touchall(c1):
+    n, g, off, cols = c1
+    dist = EMV.BF(g, 0)
+    assert np.array_equal(dist, O.bf(off, cols, 0))
+    assert EMV.TouchAll(g) == int(cols.astype(np.uint64).sum())
+
+
+def test_common_neighbors(c1):
+    n, g, off, cols = c1
+    rng = np.random.default_rng(3)
+    for u, v in rng.integers(0, n, (40, 2)).tolist() + [(n - 1, n - 2), (5, 5)]:
+        for ee in (False, True):
+            assert g.common_neighbors(u, v, ee) == \
+                O.common_neighbors(off, cols, u, v, ee)
+
+
+def test_examples_closed_forms():
+    for name, tc, hist, bc0 in [("johnson8-2-4.mtx", 420, [1, 15, 12], 27.0),
+                                ("star.mtx", 1, [1, 7, 12], 19.0)]:
+        n, s, d = read_mtx(os.path.join(GOLDEN, name))
+        g = S.SparseMatrixV(n)
+        g.insert_batch(s, d)
+        off, cols = csr_from_edges(n, s, d)
+        assert EMV.TC(g) == tc
+        _, lev = EMV.BFS(g, 0, levels=True)
+        assert np.bincount(lev).tolist() == hist
+        b = EMV.BC(g, 0)
+        assert b[0] == pytest.approx(bc0, rel=1e-12)
+        ok, _ = rel_close(b, O.bc(off, cols, 0), RTOL, 1e-12)
+        assert ok
+    n, s, d = read_mtx(os.path.join(GOLDEN, "johnson8-2-4.mtx"))
+    g = S.SparseMatrixV(n)
+    g.insert_batch(s, d)
+    np.testing.assert_allclose(EMV.PR_S(g, 10), 1 / 28, rtol=1e-13)
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_random_graphs_all_algorithms(symmetric):
+    for seed, n, m in [(1, 300, 900), (2, 5000, 60000), (3, 2, 1), (4, 1, 0)]:
+        say("random", seed, n, m, symmetric)
+        off, cols = random_graph(n, m, seed, symmetric=symmetric)
+        g = _device_graph(n, off, cols)
+        _check_bfs(g, off, cols, 0)
+        assert np.array_equal(EMV.CC(g), O.cc(off, cols))
+        assert EMV.TC(g) == O.tc(off, cols)
+        ok, w = rel_close(EMV.PR_S(g, 7, 0.85), O.pagerank(off, cols, 7, 0.85), RTOL)
+        assert ok, w
+        ok, w = rel_close(EMV.BC(g, 0), O.bc(off, cols, 0), RTOL, 1e-12)
+        assert ok, w
+
+
+def test_pagerank_f32_kahan_error_vs_f64(c1):
+    n, g, off, cols = c1
+    want = O.pagerank(off, cols, 10, 0.85)
+    got = EMV.PR_F32K(g, 10, 0.85).astype(np.float64)
+    rel = np.abs(got - want) / want
+    say("PR f32+Kahan max rel err", rel.max())
+    # FP32 ranks carry ~2^-24 relative rounding per operation
+    assert rel.max() < 1e-5

@@ -1,0 +1,163 @@
+"""Batch insert / delete on the device store vs the CPU oracle: contents must
+be bit-exact after every batch.  Calls go through the C ABI (libsstg.so)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import sstgraph_amd as S
+from oracle import oracle as O
+from tests.gpu_common import assert_same_csr, need_gpu, say
+from tests.graphs import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _gpu():
+    need_gpu()
+
+
+def test_device_rmat_matches_reference_vectors():
+    cases = json.load(open(os.path.join(GOLDEN, "rmat_vectors.json")))
+    for c in cases:
+        say("rmat", c["n"], c["seed"], c["first"])
+        s, d = S.rmat(c["n"], c["seed"], c["count"], first=c["first"],
+                      a=c["a"], b=c["b"], c=c["c"])
+        assert s.tolist() == c["src"] and d.tolist() == c["dst"]
+    # a long run against the oracle restatement (pinned above)
+    s, d = S.rmat(1 << 20, 12, 1 << 20)
+    os_, od = O.rmat(1 << 20, 12, 1 << 20)
+    assert np.array_equal(s, os_) and np.array_equal(d, od)
+
+
+@pytest.mark.parametrize("n,sizes", [(1000, [1, 7, 100, 101, 1000, 20000]),
+                                     (70000, [5000, 300000, 2])])
+def test_insert_batches_match_oracle(n, sizes):
+    rng = np.random.default_rng(n)
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    for k in sizes:
+        say("insert", n, k)
+        s = rng.integers(0, n, k).astype(np.uint32)
+        d = rng.integers(0, n, k).astype(np.uint32)
+        g.insert_batch(s, d)
+        og.insert_batch(s, d)
+        assert_same_csr(g, og, f"insert n={n} k={k}")
+
+
+def test_remove_batches_match_oracle():
+    rng = np.random.default_rng(7)
+    n = 5000
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    s = rng.integers(0, n, 200000).astype(np.uint32)
+    d = rng.integers(0, n, 200000).astype(np.uint32)
+    g.insert_batch(s, d)
+    og.insert_batch(s, d)
+    for step, k in enumerate([1, 50, 3000, 60000, 150000]):
+        say("remove", k)
+        # half existing edges (some repeated), half random (mostly absent)
+        pick = rng.integers(0, len(s), k // 2 + 1)
+        rs = np.concatenate([s[pick], rng.integers(0, n, k // 2)]).astype(np.uint32)
+        rd = np.concatenate([d[pick], rng.integers(0, n, k // 2)]).astype(np.uint32)
+        g.remove_batch(rs, rd)
+        og.remove_batch(rs, rd)
+        assert_same_csr(g, og, f"remove step {step}")
+        # re-insert part so rows regrow into their slots
+        g.insert_batch(rs[: k // 3], rd[: k // 3])
+        og.insert_batch(rs[: k // 3], rd[: k // 3])
+        assert_same_csr(g, og, f"reinsert step {step}")
+
+
+def test_self_loop_and_duplicate_rules():
+    g, og = S.SparseMatrixV(50), O.Graph(50)
+    s = np.array([1, 1, 2, 2, 2, 3, 3], np.uint32)
+    d = np.array([1, 4, 2, 5, 5, 4, 4], np.uint32)
+    g.insert_batch(s, d)
+    og.insert_batch(s, d)
+    assert_same_csr(g, og, "batch")
+    assert not g.has(1, 1) and not g.has(2, 2) and g.has(2, 5)
+    g.insert(7, 7)  # single insert keeps the self-loop (SparseMatrix.hpp:536-544)
+    og.insert(7, 7)
+    assert g.has(7, 7)
+    assert_same_csr(g, og, "single insert")
+    g.remove_batch(np.array([7, 9], np.uint32), np.array([7, 9], np.uint32))
+    og.remove_batch(np.array([7, 9], np.uint32), np.array([7, 9], np.uint32))
+    assert not g.has(7, 7)
+    assert_same_csr(g, og, "remove self-loop")
+
+
+def test_out_of_range_ids_leave_graph_unchanged():
+    g = S.SparseMatrixV(100)
+    g.insert_batch(np.array([1, 2], np.uint32), np.array([3, 4], np.uint32))
+    with pytest.raises(S.IdOutOfRange):
+        g.insert_batch(np.array([1, 100], np.uint32), np.array([5, 6], np.uint32))
+    with pytest.raises(S.IdOutOfRange):
+        g.remove_batch(np.array([1, 2], np.uint32), np.array([3, 100], np.uint32))
+    assert g.M() == 2 and g.has(1, 3) and g.has(2, 4)
+    g.insert_batch(np.array([], np.uint32), np.array([], np.uint32))
+    assert g.M() == 2
+
+
+def test_rectangular_and_zero_key():
+    # keys with (i & 0xFFFF) == 0, including 0, stay present
+    # (Tinyset_test.cpp:198-240)
+    maxv = 1 << 20
+    g = S.SparseMatrixV(3, maxv)
+    keys = np.arange(0, maxv, 0x10000, dtype=np.uint32)
+    other = np.random.default_rng(4).integers(0, maxv, 50000).astype(np.uint32)
+    allk = np.concatenate([keys, other])
+    g.insert_batch(np.ones(len(allk), np.uint32), allk)
+    assert g.has_batch(np.ones(len(keys), np.uint32), keys).all()
+    off, cols = g.export_csr()
+    assert cols[off[1]:off[2]].tolist() == sorted(set(allk.tolist()))
+
+
+def test_hub_row_grows_through_relocations():
+    rng = np.random.default_rng(9)
+    n = 1 << 18
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    for k in [10, 1000, 100000, 400000, 3]:
+        say("hub", k)
+        s = np.full(k, 5, np.uint32)
+        s[: k // 10] = rng.integers(0, n, k // 10)
+        d = rng.integers(0, n, k).astype(np.uint32)
+        g.insert_batch(s, d)
+        og.insert_batch(s, d)
+        assert_same_csr(g, og, f"hub k={k}")
+    assert g.touch_all_sum() == og.touch_all_sum()
+    assert np.array_equal(g.degrees(), og.degrees())
+
+
+def _c1():
+    n, s, d = O.rmat_graph_edges(16, 16, 0)
+    return n, s, d
+
+
+def test_c1_build_then_update_batches_bit_exact():
+    """Scale-16 RMAT (seed 0) built by batch insert, then insert+remove
+    batches of 1e3, 1e4, 1e5 RMAT edges (seeds 100..) with 20% existing
+    edges; contents compared after every batch."""
+    n, s, d = _c1()
+    gold = json.load(open(os.path.join(GOLDEN, "graph_goldens.json")))
+    gold = gold["rmat_s16_ef16_seed0"]
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    g.insert_batch(s, d)
+    og.insert_batch(s, d)
+    assert g.M() == gold["m_directed"]
+    assert_same_csr(g, og, "C1 build")
+    off, cols = og.csr()
+    for k, bsz in enumerate([1000, 10000, 100000]):
+        say("C1 batch", bsz)
+        fs, fd = O.rmat(n, 100 + k, int(bsz * 0.8))
+        pick = (np.array([O.hash64(100 + k + 7919 * j) for j in range(bsz - len(fs))],
+                         dtype=np.uint64) % np.uint64(len(cols))).astype(np.int64)
+        rows = np.searchsorted(off, pick, side="right") - 1
+        bs = np.concatenate([fs, rows.astype(np.uint32)])
+        bd = np.concatenate([fd, cols[pick]])
+        g.insert_batch(bs, bd)
+        og.insert_batch(bs, bd)
+        assert_same_csr(g, og, f"C1 insert {bsz}")
+        g.remove_batch(bs, bd)
+        og.remove_batch(bs, bd)
+        assert_same_csr(g, og, f"C1 remove {bsz}")
