@@ -1,0 +1,287 @@
+"""Benchmark of the SSTGraph hot path on one MI355X.
+
+Workload (dynamic updates on RMAT scale 24): base graph rMat(n=2^24, seed 2,
+a/b/c = 0.5/0.1/0.1) with 16n generated edges, symmetrised, built by batch
+insert into the device tinyset store; then update batches of 1e3 .. 1e7
+edges (80% fresh rMat edges with seeds 100.., 20% edges already present).
+A step is the reference's update loop body (test.hpp:668-677): insert_batch
+of the batch, then remove_batch of the same batch.  The timed batch is the
+largest one (1e7 edges); the other sizes are reported under "extra".
+
+value = batch edges / mean insert_batch time (the reference's throughput
+definition, test.hpp:681-687: every batch edge counts), inputs resident in
+HBM.  BFS (source 0) and 20 PageRank iterations on the same graph are
+reported as GTEPS under "extra".
+
+    python bench.py [--steps K] [--warmup W] [--scale 24] [--batch 1e7]
+                    [--no-cpu] [--gpus 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hash64_np(u):
+    """rmat_util.h:29-39 on uint64 numpy arrays (wrapping arithmetic)."""
+    u = u.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        v = u * np.uint64(3935559000370003845) + np.uint64(2691343689449507681)
+        v ^= v >> np.uint64(21)
+        v ^= v << np.uint64(37)
+        v ^= v >> np.uint64(4)
+        v *= np.uint64(4768777513237032717)
+        v ^= v << np.uint64(20)
+        v ^= v >> np.uint64(41)
+        v ^= v << np.uint64(5)
+    return v
+
+
+def device_symmetric_rmat(S, n, seed, count):
+    """2*count pairs on the device: rMat edges (u,v) followed by (v,u)."""
+    L = S.lib()
+    src = S.DeviceArray(2 * count, np.uint32)
+    dst = S.DeviceArray(2 * count, np.uint32)
+    S._check(L.sstg_rmat(n, seed, 0.5, 0.1, 0.1, 0, count, src.ptr, dst.ptr,
+                         S.SSTG_DEVICE_PTRS), "rmat")
+    half = ctypes.c_void_p(src.ptr.value + 4 * count)
+    halfd = ctypes.c_void_p(dst.ptr.value + 4 * count)
+    S._check(L.sstg_memcpy_d2d(half, dst.ptr, 4 * count), "d2d")
+    S._check(L.sstg_memcpy_d2d(halfd, src.ptr, 4 * count), "d2d")
+    return src, dst
+
+
+def make_batch(S, n, off, cols, size, seed):
+    """80% fresh rMat(n, seed) edges + 20% edges present in the base graph
+    (edge index hash64(seed, j) mod m), as device arrays."""
+    fresh = int(size * 0.8)
+    fs, fd = S.rmat(n, seed, fresh)
+    j = np.arange(size - fresh, dtype=np.uint64) + np.uint64(seed) * np.uint64(1 << 32)
+    pick = (hash64_np(j) % np.uint64(len(cols))).astype(np.int64)
+    rows = (np.searchsorted(off, pick, side="right") - 1).astype(np.uint32)
+    bs = np.concatenate([fs, rows])
+    bd = np.concatenate([fd, cols[pick]])
+    return S.DeviceArray.from_host(bs), S.DeviceArray.from_host(bd), bs, bd
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(off, cols, n, bs, bd, budget_s=30.0):
+    """The oracle's batch insert (per-row sorted merge, OpenMP) timed on
+    this host on the same base graph and the same batch."""
+    from oracle import oracle as O  # CPU baseline leg only
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    threads = O.threads()
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 and time.perf_counter() - t_start < budget_s:
+        ti = timed(lambda: og.insert_batch(bs, bd))
+        og.remove_batch(bs, bd)
+        times.append(ti)
+    med = float(np.median(times))
+    return dict(value=len(bs) / med, unit="edges/s", cores=threads,
+                kind="port",
+                sample=f"oracle insert_batch of one {len(bs)}-edge batch into "
+                       f"the scale-24 base graph, median of {len(times)} runs")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--batch", type=float, default=1e7)
+    ap.add_argument("--pr-iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-algos", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import sstgraph_amd as S
+    L = S.lib()
+    n = 1 << args.scale
+    count = args.edge_factor * n
+    B = int(args.batch)
+
+    # ---------------------------------------------------------- base graph
+    log(f"generating rMat scale {args.scale} seed 2: {2 * count} directed pairs")
+    src, dst = device_symmetric_rmat(S, n, 2, count)
+    g = S.SparseMatrixV(n)
+    S.synchronize()
+    t_build = timed(lambda: g.insert_batch(src, dst))
+    m = g.M()
+    log(f"built: m={m} in {t_build:.3f}s ({2 * count / t_build:.3e} pairs/s)")
+    src.free()
+    dst.free()
+    off, cols = g.export_csr()
+    extra = {"build_seconds": t_build, "build_pairs_per_s": 2 * count / t_build,
+             "m_directed": m, "n": n, "rmat_probabilities": "hashDouble in [0,2): "
+             "effective 0.25/0.05/0.05/0.65 (reference generator)"}
+
+    # ------------------------------------------------ BFS / PR on the graph
+    # Outputs stay in HBM (DeviceArray); the reference generator leaves vertex
+    # 0 isolated at this scale, so BFS is also run from the largest hub.
+    if not args.no_algos:
+        EMV = S.EdgeMapVertexMap
+        deg = np.diff(off).astype(np.int64)
+        hub = int(np.argmax(deg))
+        d_par = S.DeviceArray(n, np.int32)
+        d_lev = S.DeviceArray(n, np.uint32)
+        for name, src_v in (("bfs_src0", 0), ("bfs_hub", hub)):
+            EMV.BFS(g, src_v, out=(d_par, d_lev))
+            S.synchronize()
+            tb = [timed(lambda: EMV.BFS(g, src_v, out=(d_par, d_lev)))
+                  for _ in range(5)]
+            level = d_lev.to_host()
+            reached = level != S.UNREACHED
+            m_r = int(deg[reached].sum())
+            t = float(np.median(tb))
+            extra[name] = dict(src=src_v, ms=1e3 * t, reached=int(reached.sum()),
+                               m_reached=m_r, gteps=m_r / t / 1e9,
+                               levels=int(level[reached].max()) + 1,
+                               bytes_model=4 * m_r + 12 * n,
+                               gbs_model=(4 * m_r + 12 * n) / t / 1e9)
+            log(f"{name} (src {src_v}): {1e3 * t:.3f} ms, reached {int(reached.sum())}, "
+                f"{m_r / t / 1e9:.2f} GTEPS")
+        d_pr = S.DeviceArray(n, np.float64)
+        EMV.PR_S(g, args.pr_iters, out=d_pr)
+        S.synchronize()
+        tp = [timed(lambda: EMV.PR_S(g, args.pr_iters, out=d_pr)) for _ in range(3)]
+        t = float(np.median(tp))
+        per_it = t / args.pr_iters
+        extra["pagerank"] = dict(iters=args.pr_iters, ms=1e3 * t,
+                                 gteps=m * args.pr_iters / t / 1e9,
+                                 ms_per_iter=1e3 * per_it,
+                                 bytes_model_per_iter=4 * m + 24 * n,
+                                 gbs_model=(4 * m + 24 * n) / per_it / 1e9)
+        log(f"PR x{args.pr_iters}: {1e3 * t:.2f} ms ({m * args.pr_iters / t / 1e9:.2f} GTEPS, "
+            f"{(4 * m + 24 * n) / per_it / 1e9:.0f} GB/s model)")
+        for d in (d_par, d_lev, d_pr):
+            d.free()
+
+    # ------------------------------------------------------- update batches
+    sizes = [1000, 10000, 100000, 1000000, B]
+    per_size = {}
+    headline = None
+    for k, size in enumerate(sizes):
+        ds, dd, hs, hd = make_batch(S, n, off, cols, size, 100 + k)
+        steps = args.steps if size == B else max(3, min(args.steps, 20))
+        for _ in range(args.warmup):
+            g.insert_batch(ds, dd)
+            g.remove_batch(ds, dd)
+        S.synchronize()
+        ti, tr = [], []
+        for _ in range(steps):
+            ti.append(timed(lambda: g.insert_batch(ds, dd)))
+            tr.append(timed(lambda: g.remove_batch(ds, dd)))
+        S.synchronize()
+        # insert-then-remove of the same batch also deletes the batch edges
+        # that were present before (test.hpp:668-677), so after the first
+        # step the edge count is stable at m minus those edges
+        m_after = g.M()
+        g.insert_batch(ds, dd)
+        g.remove_batch(ds, dd)
+        assert g.M() == m_after, "update steps must leave the graph stable"
+        rec = dict(steps=steps, insert_ms=1e3 * float(np.mean(ti)),
+                   remove_ms=1e3 * float(np.mean(tr)),
+                   inserts_per_s=size / float(np.mean(ti)),
+                   removes_per_s=size / float(np.mean(tr)))
+        rec["m_after"] = m_after
+        per_size[str(size)] = rec
+        log(f"batch {size}: insert {rec['insert_ms']:.3f} ms "
+            f"({rec['inserts_per_s']:.3e}/s), remove {rec['remove_ms']:.3f} ms")
+        if size == B:
+            headline = (ds, dd, hs, hd, ti, tr)
+        else:
+            ds.free()
+            dd.free()
+    extra["batches"] = per_size
+
+    # ------------------------------- kernel timings of the headline batch
+    ds, dd, hs, hd, ti, tr = headline
+    S.Profile.reset()
+    S.Profile.enable(True)
+    prof_steps = max(3, min(args.steps, 5))
+    for _ in range(prof_steps):
+        g.insert_batch(ds, dd)
+        g.remove_batch(ds, dd)
+    S.synchronize()
+    S.Profile.enable(False)
+    kernels = sorted(S.Profile.dump(), key=lambda r: -r[2])
+    extra["kernels_per_step_ms"] = {name: ms / prof_steps for name, _, ms in kernels[:12]}
+    # dominant kernel of the insert: the row merge
+    merge_ms, merge_n = S.Profile.query("k_merge_insert")
+    # algorithmic bytes of the merge, per step (8(d) row-merge model):
+    # read each touched row's old suffix and its batch run, write the new row
+    deg = np.diff(off).astype(np.int64)
+    keys = np.unique(hs.astype(np.int64) * n + hd.astype(np.int64))
+    ks, kd = keys // n, keys % n
+    new = keys[(ks != kd)]
+    new_rows, new_counts = np.unique(new // n, return_counts=True)
+    d_old = deg[new_rows].sum()
+    merge_bytes = 4 * (2 * int(d_old) + int(new_counts.sum())) + 8 * int(new_counts.sum())
+    ms_per = merge_ms / max(merge_n, 1)
+    achieved = merge_bytes / (ms_per * 1e-3) / 1e9 if ms_per > 0 else 0.0
+    roofline = dict(kernel="k_merge_insert", bound="hbm", achieved=achieved,
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, algorithmic_bytes=merge_bytes,
+                    avg_launch_ms=ms_per)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(off, cols, n, hs, hd)
+        log(f"cpu baseline {cpu['value']:.3e} edges/s on {cpu['cores']} threads")
+
+    ins = float(np.mean(ti))
+    out = {
+        "metric": "edges inserted/s + BFS/PR GTEPS on scale-24 RMAT; p99 query latency served",
+        "value": B / ins,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * (ins + float(np.mean(tr))),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (rMat generator of the reference, seeded)",
+        "config": {"workload": f"dynamic updates: rMat scale {args.scale} base graph "
+                               f"(seed 2, edge factor {args.edge_factor}, symmetrised), "
+                               f"insert+remove batches of {B} edges (seed 104, 20% existing)",
+                   "scale": args.scale, "batch": B, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

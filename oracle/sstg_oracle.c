@@ -534,16 +534,15 @@ void orc_bc(uint32_t n, const uint64_t *off, const uint32_t *cols,
       }
     }
   }
-  /* sigma by pulling from parents, in BFS order */
+  /* sigma pushed along out-edges in BFS order (parents of w are its
+   * in-neighbours one level up) */
   sigma[src] = 1.0;
-  for (uint32_t i = 1; i < qt; i++) {
-    uint32_t w = q[i];
-    double s = 0.0;
-    for (uint64_t e = off[w]; e < off[w + 1]; e++) {
-      uint32_t v = cols[e];
-      if (lv[v] != UINT32_MAX && lv[v] + 1 == lv[w]) s += sigma[v];
+  for (uint32_t i = 0; i < qt; i++) {
+    uint32_t v = q[i];
+    for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+      uint32_t w = cols[e];
+      if (lv[w] == lv[v] + 1) sigma[w] += sigma[v];
     }
-    sigma[w] = s;
   }
   /* dependencies by pulling from children, in reverse BFS order */
   for (int64_t i = (int64_t)qt - 1; i >= 0; i--) {

@@ -329,15 +329,28 @@ class EdgeMapVertexMap:
     (test.hpp:476-618), returning numpy arrays instead of malloc'd memory."""
 
     @staticmethod
-    def BFS(g, src, levels=False):
+    def BFS(g, src, levels=False, out=None):
+        """Parent array (and levels).  out=(parent, level) DeviceArrays keeps
+        the results in HBM (either may be None)."""
         n = g.n_rows
+        if out is not None:
+            p, l = out
+            _check(lib().sstg_bfs(g.handle, src, p.ptr if p is not None else None,
+                                  l.ptr if l is not None else None,
+                                  SSTG_DEVICE_PTRS), "BFS")
+            return out
         parent = np.empty(n, np.int32)
         level = np.empty(n, np.uint32)
         _check(lib().sstg_bfs(g.handle, src, _ptr(parent), _ptr(level), 0), "BFS")
         return (parent, level) if levels else parent
 
     @staticmethod
-    def PR_S(g, iters=10, damping=0.85):
+    def PR_S(g, iters=10, damping=0.85, out=None):
+        """PageRank in FP64; out=DeviceArray keeps the ranks in HBM."""
+        if out is not None:
+            _check(lib().sstg_pagerank_f64(g.handle, iters, damping, out.ptr,
+                                           SSTG_DEVICE_PTRS), "PR_S")
+            return out
         out = np.empty(g.n_rows, np.float64)
         _check(lib().sstg_pagerank_f64(g.handle, iters, damping, _ptr(out), 0),
                "PR_S")

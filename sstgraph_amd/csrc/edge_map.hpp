@@ -109,45 +109,86 @@ __global__ void k_sparse_push(DevGraph g, const uint32_t *front,
   }
 }
 
-// Dense pull with G lanes per vertex (G divides 64).  A vertex whose cond
-// holds scans its neighbours G at a time and stops at the first chunk that
-// holds a frontier member; the lowest such lane is the source.
+// Dense pull over the vertex list list[0..count) with G lanes per vertex
+// (G divides 64).  A vertex whose cond holds scans its neighbours G at a
+// time and stops at the first chunk that holds a frontier member; the lowest
+// such lane is the source.
 template <class F, int G>
-__global__ void k_dense_pull(DevGraph g, F f, const uint32_t *bm_cur,
-                             uint32_t *bm_next, unsigned long long *ctr) {
+__global__ void __launch_bounds__(kT)
+    k_dense_pull(DevGraph g, const uint32_t *list, uint64_t count, F f,
+                 const uint32_t *bm_cur, uint32_t *bm_next,
+                 unsigned long long *ctr) {
   const uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
   const uint32_t lig = (uint32_t)(gid % G);
-  const uint64_t v = gid / G;
+  const uint64_t i = gid / G;
   const uint32_t glane0 = lane_id() & ~(uint32_t)(G - 1);
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << glane0);
   bool found = false;
   uint32_t dv = 0;
-  if (v < g.n && f.cond((uint32_t)v)) {
-    dv = g.deg[v];
-    const uint32_t *row = g.pool + g.off[v];
-    for (uint32_t base = 0; base < dv; base += G) {
-      uint32_t k = base + lig;
-      bool hit = false;
-      uint32_t u = 0;
-      if (k < dv) {
-        u = row[k];
-        hit = (bm_cur[u >> 5] >> (u & 31)) & 1u;
-      }
-      uint64_t m = __ballot(hit) & gmask;
-      if (m) {
-        uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-        uint32_t su = __shfl(u, leader, kWave);
-        if (lig == 0) {
-          f.update(su, (uint32_t)v);
-          atomicOr(&bm_next[v >> 5], 1u << (v & 31));
-          found = true;
+  uint32_t v = 0;
+  if (i < count) {
+    v = list[i];
+    if (f.cond(v)) {
+      dv = g.deg[v];
+      const uint32_t *row = g.pool + g.off[v];
+      for (uint32_t base = 0; base < dv; base += G) {
+        uint32_t k = base + lig;
+        bool hit = false;
+        uint32_t u = 0;
+        if (k < dv) {
+          u = row[k];
+          hit = (bm_cur[u >> 5] >> (u & 31)) & 1u;
         }
-        break;
+        uint64_t m = __ballot(hit) & gmask;
+        if (m) {
+          uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+          uint32_t su = __shfl(u, leader, kWave);
+          if (lig == 0) {
+            f.update(su, v);
+            atomicOr(&bm_next[v >> 5], 1u << (v & 31));
+            found = true;
+          }
+          break;
+        }
       }
     }
   }
   bool take = found && lig == 0;
-  append(take, (uint32_t)v, dv, (uint32_t *)nullptr, ctr);
+  append(take, v, dv, (uint32_t *)nullptr, ctr);
+}
+
+// Dense pull with a whole block per (high-degree) vertex: the block scans
+// 256 neighbours per round and stops after the first round with a hit.
+template <class F>
+__global__ void __launch_bounds__(kT)
+    k_dense_pull_block(DevGraph g, const uint32_t *list, uint64_t count, F f,
+                       const uint32_t *bm_cur, uint32_t *bm_next,
+                       unsigned long long *ctr) {
+  __shared__ uint32_t first_hit;
+  const uint32_t v = list[blockIdx.x];
+  if (!f.cond(v)) return;  // uniform across the block
+  const uint32_t dv = g.deg[v];
+  const uint32_t *row = g.pool + g.off[v];
+  if (threadIdx.x == 0) first_hit = 0xFFFFFFFFu;
+  __syncthreads();
+  for (uint32_t base = 0; base < dv; base += kT) {
+    uint32_t k = base + threadIdx.x;
+    if (k < dv) {
+      uint32_t u = row[k];
+      if ((bm_cur[u >> 5] >> (u & 31)) & 1u) atomicMin(&first_hit, k);
+    }
+    __syncthreads();
+    uint32_t h = first_hit;
+    if (h != 0xFFFFFFFFu) {
+      if (threadIdx.x == 0) {
+        f.update(row[h], v);
+        atomicOr(&bm_next[v >> 5], 1u << (v & 31));
+        atomicAdd(&ctr[0], 1ull);
+        atomicAdd(&ctr[1], (unsigned long long)dv);
+      }
+      return;
+    }
+  }
 }
 
 // ------------------------------------------------------------ vertex pull
@@ -243,24 +284,6 @@ int edge_map_sparse(sstg_graph *g, EdgeMapCtx &ctx, const uint32_t *front,
   return read_result(g, ctx, res);
 }
 
-// Dense pull over all vertices against ctx.bm_cur; the output frontier is
-// ctx.bm_next, after which the two bitmaps swap roles.
-template <class F>
-int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
-  hipStream_t s = g->stream;
-  constexpr int G = 8;
-  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
-  SSTG_LAUNCH(em::k_clear_u32, ceil_div(ctx.words, em::kT), em::kT, 0, s,
-              ctx.bm_next, ctx.words);
-  uint64_t threads = (uint64_t)g->n_rows * G;
-  SSTG_LAUNCH((em::k_dense_pull<F, G>), ceil_div(threads, em::kT), em::kT, 0, s,
-              g->view(), f, (const uint32_t *)ctx.bm_cur, ctx.bm_next, ctx.ctr);
-  uint32_t *t = ctx.bm_cur;
-  ctx.bm_cur = ctx.bm_next;
-  ctx.bm_next = t;
-  return read_result(g, ctx, res);
-}
-
 // ----------------------------------------------------------- degree bins
 // Vertices split by degree for the vertex-pull kernels (cached per graph
 // version): bin 0 deg <= 8 (one lane), bin 1 <= 64 (8 lanes), bin 2 <= 2048
@@ -310,6 +333,42 @@ inline int degree_bins(sstg_graph *g, DegreeBins *out) {
   for (int b = 0; b < 4; b++) g->bins_count[b] = out->count[b];
   g->bins_version = g->version;
   return SSTG_OK;
+}
+
+// Dense pull over all vertices against ctx.bm_cur, split by degree bin
+// (1, 8, 64 lanes or a block per vertex); the output frontier is
+// ctx.bm_next, after which the two bitmaps swap roles.
+template <class F>
+int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
+  hipStream_t s = g->stream;
+  DegreeBins bins;
+  SSTG_TRY(degree_bins(g, &bins));
+  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
+  SSTG_LAUNCH(em::k_clear_u32, ceil_div(ctx.words, em::kT), em::kT, 0, s,
+              ctx.bm_next, ctx.words);
+  DevGraph v = g->view();
+  const uint32_t *cur = ctx.bm_cur;
+  if (bins.count[0])
+    SSTG_LAUNCH((em::k_dense_pull<F, 1>), ceil_div(bins.count[0], em::kT),
+                em::kT, 0, s, v, (const uint32_t *)bins.list[0], bins.count[0],
+                f, cur, ctx.bm_next, ctx.ctr);
+  if (bins.count[1])
+    SSTG_LAUNCH((em::k_dense_pull<F, 8>), ceil_div(bins.count[1] * 8, em::kT),
+                em::kT, 0, s, v, (const uint32_t *)bins.list[1], bins.count[1],
+                f, cur, ctx.bm_next, ctx.ctr);
+  if (bins.count[2])
+    SSTG_LAUNCH((em::k_dense_pull<F, 64>),
+                ceil_div(bins.count[2] * 64, em::kT), em::kT, 0, s, v,
+                (const uint32_t *)bins.list[2], bins.count[2], f, cur,
+                ctx.bm_next, ctx.ctr);
+  if (bins.count[3])
+    SSTG_LAUNCH(em::k_dense_pull_block<F>, bins.count[3], em::kT, 0, s, v,
+                (const uint32_t *)bins.list[3], bins.count[3], f, cur,
+                ctx.bm_next, ctx.ctr);
+  uint32_t *t = ctx.bm_cur;
+  ctx.bm_cur = ctx.bm_next;
+  ctx.bm_next = t;
+  return read_result(g, ctx, res);
 }
 
 template <class F>

@@ -287,6 +287,26 @@ __global__ void k_finalize(SegArgs a, DevGraph g, int insert) {
   }
 }
 
+// After a batch on a symmetric graph: the graph stays symmetric iff every
+// effective inserted edge (u, v) now has (v, u) and every removed one has
+// lost it.  Clears *flag otherwise.
+__global__ void k_sym_batch(const uint64_t *B, uint64_t b, DevGraph g,
+                            int insert, uint32_t *flag) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= b) return;
+  uint32_t u = (uint32_t)(B[i] >> 32), v = (uint32_t)B[i];
+  bool has = false;
+  if (v < g.n) {
+    uint32_t dv = g.deg[v];
+    if (dv) {
+      const uint32_t *row = g.pool + g.off[v];
+      uint64_t p = lower_bound_dev(row, (uint64_t)dv, u);
+      has = p < dv && row[p] == u;
+    }
+  }
+  if (has != (insert != 0)) *flag = 0;
+}
+
 // ----------------------------------------------------------- rebuild etc.
 __global__ void k_deg_to_u64(const uint32_t *deg, uint64_t n, uint64_t *out) {
   uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
@@ -553,7 +573,25 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                   g->view(), work_tot);
   }
   SSTG_LAUNCH(k_finalize, blocks(t), kT, 0, s, a, g->view(), insert ? 1 : 0);
+  // track symmetry through small batches; a batch that is large next to the
+  // graph leaves it unknown (checked in full when a traversal needs it)
+  const bool was_symmetric = g->sym_version == g->version && g->symmetric &&
+                             b * 8 <= g->m;
+  uint32_t *symflag = (uint32_t *)(ctr + kCtrScratch);
+  if (was_symmetric) {
+    SSTG_HIP(hipMemsetAsync(symflag, 0xFF, sizeof(uint32_t), s));
+    SSTG_LAUNCH(k_sym_batch, blocks(b), kT, 0, s, (const uint64_t *)B, b,
+                g->view(), insert ? 1 : 0, symflag);
+    SSTG_HIP(hipMemcpyAsync(g->host_rb + 32, symflag, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, s));
+  }
   SSTG_HIP(hipStreamSynchronize(s));
+  if (was_symmetric) {
+    g->symmetric = (uint32_t)g->host_rb[32] != 0;
+    g->sym_version = g->version + 1;
+  } else {
+    g->sym_version = ~0ull;
+  }
   g->pool_tail += alloc_tot;
   g->garbage += garbage_add;
   g->m = insert ? g->m + b : g->m - b;

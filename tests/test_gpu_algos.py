@@ -159,3 +159,38 @@ def test_pagerank_f32_kahan_error_vs_f64(c1):
     say("PR f32+Kahan max rel err", rel.max())
     # FP32 ranks carry ~2^-24 relative rounding per operation
     assert rel.max() < 1e-5
+
+
+def test_symmetry_tracking_through_updates():
+    """Bottom-up steps are only valid on symmetric graphs; the library tracks
+    symmetry through small batches.  BFS/BC must stay exact as the graph goes
+    from symmetric to directed and back."""
+    rng = np.random.default_rng(17)
+    n = 20000
+    off, cols = random_graph(n, 300000, 19, symmetric=True)
+    g = _device_graph(n, off, cols)
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+
+    def check(what):
+        say("symmetry", what)
+        o_off, o_cols = og.csr()
+        _check_bfs(g, o_off, o_cols, 1)
+        ok, w = rel_close(EMV.BC(g, 1), O.bc(o_off, o_cols, 1), RTOL, 1e-12)
+        assert ok, (what, w)
+
+    check("symmetric base")
+    s = rng.integers(0, n, 500).astype(np.uint32)
+    d = rng.integers(0, n, 500).astype(np.uint32)
+    g.insert_batch(np.concatenate([s, d]), np.concatenate([d, s]))
+    og.insert_batch(np.concatenate([s, d]), np.concatenate([d, s]))
+    check("symmetric batch")
+    g.insert_batch(s[:50], d[:50] ^ 1)   # one direction only
+    og.insert_batch(s[:50], d[:50] ^ 1)
+    check("directed batch")
+    g.insert_batch(d[:50] ^ 1, s[:50])   # the reverses
+    og.insert_batch(d[:50] ^ 1, s[:50])
+    check("reverses added")
+    g.remove_batch(np.concatenate([s[:100], d[:100]]), np.concatenate([d[:100], s[:100]]))
+    og.remove_batch(np.concatenate([s[:100], d[:100]]), np.concatenate([d[:100], s[:100]]))
+    check("symmetric removal")

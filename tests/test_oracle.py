@@ -279,10 +279,11 @@ def _brute_dependency(n, adj, s):
     return out
 
 
-@pytest.mark.parametrize("seed", [21, 22, 23])
-def test_bc_matches_brute_force_path_counting(seed):
+@pytest.mark.parametrize("seed,symmetric", [(21, True), (22, True), (23, True),
+                                            (24, False), (25, False)])
+def test_bc_matches_brute_force_path_counting(seed, symmetric):
     n = 40
-    off, cols = random_graph(n, 70, seed)
+    off, cols = random_graph(n, 70 if symmetric else 160, seed, symmetric=symmetric)
     adj = [cols[off[v]:off[v + 1]].tolist() for v in range(n)]
     for s in [0, 7]:
         got = O.bc(off, cols, s)
