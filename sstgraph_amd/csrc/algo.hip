@@ -4,6 +4,8 @@
 // (include/SSTGraph/internal/test.hpp:476-618), which scan neighbours through
 // SparseMatrixV::map_neighbors / map_range (SparseMatrix.hpp:174-315).  Here a
 // row's neighbours are pool[off[v] .. off[v] + deg[v]) in ascending order.
+#include <cstdlib>
+
 #include "algo.hpp"
 #include "edge_map.hpp"
 #include "primitives.hpp"
@@ -56,6 +58,9 @@ struct BfsPull {
   __device__ __forceinline__ void update(uint32_t s, uint32_t d) const {
     level[d] = next_level;
     if (parent) parent[d] = (int32_t)s;
+  }
+  __device__ __forceinline__ bool in_output(uint32_t v) const {
+    return level[v] == next_level;
   }
 };
 
@@ -402,6 +407,7 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   top_down_only = top_down_only || !symmetric;
   uint32_t L = 0;
   uint8_t *flags = nullptr;
+  const bool verbose = std::getenv("SSTG_VERBOSE") != nullptr;
   while (nf > 0) {
     uint64_t unexplored = g->m > explored ? g->m - explored : 0;
     bool want_bu = !top_down_only &&
@@ -431,6 +437,10 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                                queue + qs + nf, &res));
       bottom_up = false;
     }
+    if (verbose)
+      std::fprintf(stderr, "bfs level %u %s nf=%llu mf=%llu -> next %llu\n", L,
+                   want_bu ? "pull" : "push", (unsigned long long)nf,
+                   (unsigned long long)mf, (unsigned long long)res.count);
     qs += nf;
     nf = res.count;
     mf = res.degree_sum;

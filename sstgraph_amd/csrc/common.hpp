@@ -139,6 +139,17 @@ __device__ __forceinline__ uint64_t find_segment(const uint64_t *off,
   return lo;
 }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with a
+// private L2.  xcd_block() renumbers block b of nb so that each XCD runs a
+// contiguous range of logical blocks (XCD x gets the x-th slice), which keeps
+// neighbouring vertex ranges -- and the gathers they share -- in one L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / kXcds, r = nb % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // wave-wide inclusive sum (wave64)
 template <class T> __device__ __forceinline__ T wave_inclusive_sum(T v) {
   const uint32_t l = lane_id();

@@ -85,28 +85,97 @@ __device__ __forceinline__ void append(bool take, uint32_t d, uint32_t dd,
   if (take && out) out[base + mask_rank(mask)] = d;
 }
 
+// Block-wide exclusive scan of one u32 per thread (kT threads); returns the
+// thread's offset and the block total in *total.
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total) {
+  __shared__ uint32_t wt[kT / kWave];
+  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
+  uint32_t incl = wave_inclusive_sum(v);
+  if (lane == kWave - 1) wt[w] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kT / kWave; i++) {
+    uint32_t x = wt[i];
+    if ((uint32_t)i < w) off += x;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+// Adds a per-thread count and degree sum to ctr with one atomic pair per
+// block.
+__device__ __forceinline__ void block_count(uint32_t cnt, uint64_t dsum,
+                                            unsigned long long *ctr) {
+  __shared__ unsigned long long bc[2];
+  if (threadIdx.x == 0) {
+    bc[0] = 0;
+    bc[1] = 0;
+  }
+  __syncthreads();
+  uint64_t c = wave_sum((uint64_t)cnt), d = wave_sum(dsum);
+  if (lane_id() == 0 && (c | d)) {
+    atomicAdd(&bc[0], (unsigned long long)c);
+    atomicAdd(&bc[1], (unsigned long long)d);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && (bc[0] | bc[1])) {
+    atomicAdd(&ctr[0], bc[0]);
+    atomicAdd(&ctr[1], bc[1]);
+  }
+}
+
 template <class F>
-__global__ void k_sparse_push(DevGraph g, const uint32_t *front,
-                              const uint64_t *fpre, uint64_t nf, uint64_t mf,
-                              F f, uint32_t *next, unsigned long long *ctr) {
+__global__ void __launch_bounds__(kT)
+    k_sparse_push(DevGraph g, const uint32_t *front, const uint64_t *fpre,
+                  uint64_t nf, uint64_t mf, F f, uint32_t *next,
+                  unsigned long long *ctr) {
+  // each thread expands kChunk consecutive frontier edges, keeps the vertices
+  // it claims in registers, and the block appends them with one atomic
+  __shared__ uint32_t got_lds[kT * kChunk];
+  uint32_t *got = got_lds + threadIdx.x * kChunk;
   uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
   uint64_t end = e + kChunk < mf ? e + kChunk : mf;
-  uint64_t i = e < mf ? find_segment(fpre, nf, e) : 0;
-  // every lane runs kChunk rounds so the wave stays converged at append()
-  for (int r = 0; r < kChunk; r++, e++) {
-    bool take = false;
-    uint32_t d = 0, dd = 0;
-    if (e < end) {
+  uint32_t cnt = 0;
+  uint64_t dsum = 0;
+  if (e < end) {
+    uint64_t i = find_segment(fpre, nf, e);
+    for (; e < end; e++) {
       while (e >= fpre[i + 1]) i++;
       uint32_t s = front[i];
-      d = g.pool[g.off[s] + (e - fpre[i])];
+      uint32_t d = g.pool[g.off[s] + (e - fpre[i])];
       if (f.cond(d) && f.update_atomic(s, d)) {
-        take = true;
-        dd = g.deg[d];
+        got[cnt++] = d;
+        dsum += g.deg[d];
       }
     }
-    append(take, d, dd, next, ctr);
   }
+  uint32_t total;
+  uint32_t off = block_scan_u32(cnt, &total);
+  __shared__ unsigned long long base;
+  if (threadIdx.x == 0 && total) {
+    base = atomicAdd(&ctr[0], (unsigned long long)total);
+  }
+  __syncthreads();
+  if (total) {
+    for (uint32_t k = 0; k < cnt; k++) next[base + off + k] = got[k];
+  }
+  block_count(0, dsum, ctr);
+}
+
+// Output frontier bitmap of a dense step: bit v = f.in_output(v); one lane
+// per vertex, a ballot per 64 vertices, no atomics.
+template <class F>
+__global__ void k_bitmap_from(uint32_t n, F f, uint32_t *bm, uint64_t words) {
+  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  bool in = v < n && f.in_output((uint32_t)v);
+  uint64_t m = __ballot(in);
+  uint32_t lane = lane_id();
+  uint64_t w = v >> 5;
+  if ((lane & 31) == 0 && w < words)
+    bm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
 }
 
 // Dense pull over the vertex list list[0..count) with G lanes per vertex
@@ -118,7 +187,8 @@ __global__ void __launch_bounds__(kT)
     k_dense_pull(DevGraph g, const uint32_t *list, uint64_t count, F f,
                  const uint32_t *bm_cur, uint32_t *bm_next,
                  unsigned long long *ctr) {
-  const uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  const uint64_t gid =
+      (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kT + threadIdx.x;
   const uint32_t lig = (uint32_t)(gid % G);
   const uint64_t i = gid / G;
   const uint32_t glane0 = lane_id() & ~(uint32_t)(G - 1);
@@ -145,7 +215,6 @@ __global__ void __launch_bounds__(kT)
           uint32_t su = __shfl(u, leader, kWave);
           if (lig == 0) {
             f.update(su, v);
-            atomicOr(&bm_next[v >> 5], 1u << (v & 31));
             found = true;
           }
           break;
@@ -154,7 +223,7 @@ __global__ void __launch_bounds__(kT)
     }
   }
   bool take = found && lig == 0;
-  append(take, v, dv, (uint32_t *)nullptr, ctr);
+  block_count(take ? 1u : 0u, take ? dv : 0u, ctr);
 }
 
 // Dense pull with a whole block per (high-degree) vertex: the block scans
@@ -165,7 +234,7 @@ __global__ void __launch_bounds__(kT)
                        const uint32_t *bm_cur, uint32_t *bm_next,
                        unsigned long long *ctr) {
   __shared__ uint32_t first_hit;
-  const uint32_t v = list[blockIdx.x];
+  const uint32_t v = list[xcd_block(blockIdx.x, gridDim.x)];
   if (!f.cond(v)) return;  // uniform across the block
   const uint32_t dv = g.deg[v];
   const uint32_t *row = g.pool + g.off[v];
@@ -182,7 +251,6 @@ __global__ void __launch_bounds__(kT)
     if (h != 0xFFFFFFFFu) {
       if (threadIdx.x == 0) {
         f.update(row[h], v);
-        atomicOr(&bm_next[v >> 5], 1u << (v & 31));
         atomicAdd(&ctr[0], 1ull);
         atomicAdd(&ctr[1], (unsigned long long)dv);
       }
@@ -199,15 +267,26 @@ template <class F, int G>
 __global__ void __launch_bounds__(kT)
     k_vertex_pull(DevGraph g, const uint32_t *list, uint64_t count, F f) {
   using Acc = typename F::Acc;
-  const uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  const uint64_t grp = (G == 256) ? blockIdx.x : gid / G;
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t gid = (uint64_t)blk * kT + threadIdx.x;
+  const uint64_t grp = (G == 256) ? blk : gid / G;
   const uint32_t lig = (G == 256) ? threadIdx.x : (uint32_t)(gid % G);
   if (grp >= count) return;  // whole groups leave together
   const uint32_t v = list ? list[grp] : (uint32_t)grp;
   const uint32_t d = g.deg[v];
   const uint32_t *row = g.pool + g.off[v];
   Acc acc = f.zero();
-  for (uint32_t k = lig; k < d; k += G) f.add(acc, v, row[k]);
+  uint32_t k = lig;
+  // four neighbour ids in flight before their gathers
+  for (; k + 3 * G < d; k += 4 * G) {
+    uint32_t u0 = row[k], u1 = row[k + G], u2 = row[k + 2 * G],
+             u3 = row[k + 3 * G];
+    f.add(acc, v, u0);
+    f.add(acc, v, u1);
+    f.add(acc, v, u2);
+    f.add(acc, v, u3);
+  }
+  for (; k < d; k += G) f.add(acc, v, row[k]);
   if (G > 1) {
     const int w = (G < kWave) ? G : kWave;
     for (int o = w / 2; o > 0; o >>= 1) acc = f.merge(acc, F::shfl_xor(acc, o));
@@ -344,8 +423,6 @@ int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
   DegreeBins bins;
   SSTG_TRY(degree_bins(g, &bins));
   SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
-  SSTG_LAUNCH(em::k_clear_u32, ceil_div(ctx.words, em::kT), em::kT, 0, s,
-              ctx.bm_next, ctx.words);
   DevGraph v = g->view();
   const uint32_t *cur = ctx.bm_cur;
   if (bins.count[0])
@@ -365,6 +442,8 @@ int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
     SSTG_LAUNCH(em::k_dense_pull_block<F>, bins.count[3], em::kT, 0, s, v,
                 (const uint32_t *)bins.list[3], bins.count[3], f, cur,
                 ctx.bm_next, ctx.ctr);
+  SSTG_LAUNCH(em::k_bitmap_from<F>, ceil_div(ctx.words * 32, em::kT), em::kT,
+              0, s, g->n_rows, f, ctx.bm_next, ctx.words);
   uint32_t *t = ctx.bm_cur;
   ctx.bm_cur = ctx.bm_next;
   ctx.bm_next = t;
