@@ -223,34 +223,54 @@ def main():
     extra["batches"] = per_size
 
     # ------------------------------- kernel timings of the headline batch
+    # HIP events around each kernel launch on the graph's stream; the
+    # insert launches are profiled on their own.
     ds, dd, hs, hd, ti, tr = headline
     S.Profile.reset()
-    S.Profile.enable(True)
     prof_steps = max(3, min(args.steps, 5))
+    stats = None
     for _ in range(prof_steps):
+        S.Profile.enable(True)
         g.insert_batch(ds, dd)
+        S.synchronize()
+        S.Profile.enable(False)
+        stats = g.last_update_stats()
         g.remove_batch(ds, dd)
     S.synchronize()
-    S.Profile.enable(False)
     kernels = sorted(S.Profile.dump(), key=lambda r: -r[2])
-    extra["kernels_per_step_ms"] = {name: ms / prof_steps for name, _, ms in kernels[:12]}
-    # dominant kernel of the insert: the row merge
-    merge_ms, merge_n = S.Profile.query("k_merge_insert")
-    # algorithmic bytes of the merge, per step (8(d) row-merge model):
-    # read each touched row's old suffix and its batch run, write the new row
+    extra["insert_kernels_ms"] = {name: ms / prof_steps for name, _, ms in kernels[:12]}
+    extra["insert_stats"] = stats
+    dom_name, dom_n, dom_ms = kernels[0]
+    ms_per = dom_ms / max(dom_n, 1)
+    # algorithmic bytes of the dominant kernel per launch
+    if "k_merge_tile" in dom_name:
+        # read each tile row's moved suffix (or whole old row) and its batch
+        # run, write the merged row: 4 B in + 4 B out per merge input
+        alg_bytes = 8 * stats["tile_inputs"]
+    elif "k_merge_insert" in dom_name:
+        alg_bytes = 8 * stats["merged"] + 4 * stats["effective"]
+    else:
+        alg_bytes = 16 * B  # radix pass: read + write 8-byte keys
+    achieved = alg_bytes / (ms_per * 1e-3) / 1e9 if ms_per > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_merge_tile_scale24_batch1e7.traffic.json")
+    if os.path.exists(tfile) and args.scale == 24 and B == 10_000_000:
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    roofline = dict(kernel=dom_name, bound="hbm", achieved=achieved,
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=traffic, algorithmic_bytes=alg_bytes,
+                    avg_launch_ms=ms_per)
+    # whole-operation row-merge model (8 B per batch pair + 4 B per old and
+    # new element of each touched row + 16 B per touched row)
     deg = np.diff(off).astype(np.int64)
     keys = np.unique(hs.astype(np.int64) * n + hd.astype(np.int64))
-    ks, kd = keys // n, keys % n
-    new = keys[(ks != kd)]
-    new_rows, new_counts = np.unique(new // n, return_counts=True)
-    d_old = deg[new_rows].sum()
-    merge_bytes = 4 * (2 * int(d_old) + int(new_counts.sum())) + 8 * int(new_counts.sum())
-    ms_per = merge_ms / max(merge_n, 1)
-    achieved = merge_bytes / (ms_per * 1e-3) / 1e9 if ms_per > 0 else 0.0
-    roofline = dict(kernel="k_merge_insert", bound="hbm", achieved=achieved,
-                    peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, algorithmic_bytes=merge_bytes,
-                    avg_launch_ms=ms_per)
+    ks = keys // n
+    keep = ks != keys % n
+    trow, tcount = np.unique(ks[keep], return_counts=True)
+    d_old = int(deg[trow].sum())
+    op_bytes = 8 * B + 4 * (2 * d_old + int(tcount.sum())) + 16 * len(trow)
+    extra["insert_op_model_bytes"] = op_bytes
+    extra["insert_op_model_gbs"] = op_bytes / float(np.mean(ti)) / 1e9
 
     cpu = None
     if not args.no_cpu:

@@ -76,6 +76,24 @@ int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col);
 /* SparseMatrixV::remove(row, col), SM:546-554. */
 int sstg_remove(sstg_graph *g, uint32_t row, uint32_t col);
 
+/* What the last insert_batch / remove_batch call did (summed over its
+ * internal chunks).  No reference counterpart: the reference reports only
+ * wall time (test.hpp:668-687); these counts give the bytes each kernel had
+ * to move. */
+typedef struct sstg_update_stats {
+  uint64_t batch;          /* pairs passed in */
+  uint64_t effective;      /* pairs that changed the graph */
+  uint64_t rows;           /* rows touched */
+  uint64_t tile_rows;      /* rows merged in place through LDS */
+  uint64_t tile_inputs;    /* their merge inputs (moved suffix + batch run) */
+  uint64_t big_rows;       /* rows merged over the flat index space */
+  uint64_t staged;         /* elements copied to scratch for big rows */
+  uint64_t merged;         /* flat work items of the big-row merge */
+  uint64_t allocated;      /* slot elements taken at the pool tail */
+  uint64_t rebuilds;       /* pool rebuilds so far on this graph */
+} sstg_update_stats;
+int sstg_last_update_stats(sstg_graph *g, sstg_update_stats *out);
+
 /* ---- queries ----------------------------------------------------------- */
 /* SparseMatrixV::has(row, col), SM:720-726. */
 int sstg_has(sstg_graph *g, uint32_t row, uint32_t col, int *out);

@@ -54,6 +54,16 @@ class IdOutOfRange(SSTGError, ValueError):
     pass
 
 
+class UpdateStats(ctypes.Structure):
+    """sstg_update_stats: what the last batch update did."""
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "batch", "effective", "rows", "tile_rows", "tile_inputs", "big_rows",
+        "staged", "merged", "allocated", "rebuilds")]
+
+    def as_dict(self):
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -80,6 +90,7 @@ def lib():
         "sstg_insert": (i32, [vp, u32, u32]),
         "sstg_remove": (i32, [vp, u32, u32]),
         "sstg_has": (i32, [vp, u32, u32, P(i32)]),
+        "sstg_last_update_stats": (i32, [vp, P(UpdateStats)]),
         "sstg_has_batch": (i32, [vp, vp, vp, u64, vp, u32]),
         "sstg_num_nodes": (i32, [vp, P(u32)]),
         "sstg_num_edges": (i32, [vp, P(u64)]),
@@ -252,6 +263,12 @@ class SparseMatrixV:
     def remove_batch(self, src, dst=None, semi_sorted=False):
         self._batch(lib().sstg_remove_batch, src, dst,
                     SSTG_SEMI_SORTED if semi_sorted else 0, "remove_batch")
+
+    def last_update_stats(self):
+        st = UpdateStats()
+        _check(lib().sstg_last_update_stats(self._h, ctypes.byref(st)),
+               "last_update_stats")
+        return st.as_dict()
 
     def insert(self, row, col):
         _check(lib().sstg_insert(self._h, row, col), "insert")

@@ -274,6 +274,12 @@ int sstg_remove(sstg_graph *g, uint32_t row, uint32_t col) {
   return graph_update(g, &row, &col, 1, false, false, false);
 }
 
+int sstg_last_update_stats(sstg_graph *g, sstg_update_stats *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  *out = g->last;
+  return SSTG_OK;
+}
+
 int sstg_has(sstg_graph *g, uint32_t row, uint32_t col, int *out) {
   if (!g || !out) return SSTG_E_INVAL;
   if (row >= g->n_rows || col >= g->n_cols) return SSTG_E_RANGE;

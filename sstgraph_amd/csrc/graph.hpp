@@ -18,6 +18,7 @@ struct sstg_graph {
   uint64_t pool_tail = 0;     // first free element
   uint64_t garbage = 0;       // elements in abandoned slots
   uint64_t rebuilds = 0;
+  sstg_update_stats last{};
   // per-call scratch
   sstg::Workspace ws;
   // small pinned host block for read-backs
@@ -80,10 +81,23 @@ enum : int {
   kWsMisc = 14,
   kWsOut = 15,
   kWsOut2 = 80,
+  kWsSegTile = 81,
+  kWsSegIsBig = 82,
+  kWsSegIsTile = 83,
+  kWsBigIdx = 84,
+  kWsTileIdx = 85,
+  kWsBigStage = 86,
+  kWsBigWork = 87,
+  kWsTileLen = 88,
+  kWsTileSrc = 89,
+  kWsTileDst = 90,
+  kWsTileAlen = 91,
+  kWsTileBst = 92,
+  kWsTileBlk = 93,
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range
-  kWsSlots = 96,
+  kWsSlots = 112,
 };
 
 }  // namespace sstg

@@ -44,8 +44,18 @@ enum {
   kCtrWork = 5,     // flat work items of the merge step
   kCtrGarbage = 6,  // elements in slots abandoned by relocation
   kCtrScratch = 7,
+  kCtrBig = 12,     // big rows
+  kCtrTile = 13,    // elements (inputs) merged by the LDS tile kernel
+  kCtrTiles = 14,   // tile rows
   kCtrCount = 16,
 };
+
+// Rows whose merge inputs (suffix + batch run) fit in kTileMax elements are
+// merged in place by one block through LDS; larger rows are staged.
+constexpr int kTile = 2048;
+constexpr int kTileMax = 2048;
+constexpr int kTileLds = kTile + kTileMax;
+constexpr int kTileItems = kTileLds / kT;
 
 __device__ __forceinline__ uint64_t grow_cap(uint64_t d, uint32_t n_cols) {
   uint64_t c = 4;
@@ -98,21 +108,25 @@ __global__ void k_heads(const uint64_t *B, const uint64_t *ctr, uint64_t nmax,
   flags[i] = (i < b) && (i == 0 || (B[i] >> 32) != (B[i - 1] >> 32));
 }
 
-// One thread per touched row (s < t); rows s in [t, nmax] get zero lengths so
-// the scans over nmax + 1 entries give exact offsets and totals.
+// One thread per touched row (s < t) and one per unused slot s in [t, nmax]
+// (zero lengths, so scans over nmax + 1 entries give exact totals).  Each
+// row is classed as a tile row (merge inputs <= kTileMax, merged in place
+// through LDS) or a big row (suffix staged, merged over a flat index space).
 __global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
                           const uint64_t *seg_idx, DevGraph g, int insert,
                           uint32_t n_cols, uint64_t nmax, uint32_t *seg_v,
                           uint32_t *seg_p0, uint32_t *seg_olddeg,
-                          uint64_t *seg_stage, uint64_t *seg_alloc,
-                          uint64_t *seg_work, uint64_t *garbage) {
+                          uint64_t *seg_alloc, uint32_t *seg_stage,
+                          uint64_t *seg_work, uint32_t *seg_tile,
+                          uint8_t *is_big, uint8_t *is_tile,
+                          uint64_t *garbage) {
   uint64_t s = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (s > nmax) return;
   uint64_t t = ctr[kCtrT], b = ctr[kCtrB];
   if (s >= t) {
-    seg_stage[s] = 0;
     seg_alloc[s] = 0;
-    seg_work[s] = 0;
+    is_big[s] = 0;
+    is_tile[s] = 0;
     return;
   }
   uint64_t bs = seg_idx[s], be = (s + 1 < t) ? seg_idx[s + 1] : b;
@@ -137,12 +151,42 @@ __global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
     stage = od - p0;
     work = stage;
   }
+  // merge inputs: the moved suffix (or the whole row when relocating) plus
+  // the batch run
+  uint64_t inputs = (alloc ? (uint64_t)od : stage) + blen;
+  bool tile = inputs <= (uint64_t)kTileMax;
   seg_v[s] = v;
   seg_p0[s] = (uint32_t)p0;
   seg_olddeg[s] = od;
-  seg_stage[s] = stage;
   seg_alloc[s] = alloc;
+  seg_stage[s] = (uint32_t)stage;
   seg_work[s] = work;
+  seg_tile[s] = (uint32_t)inputs;
+  is_big[s] = !tile;
+  is_tile[s] = tile;
+}
+
+// Lengths of the compacted classes: big rows (stage, work) and tile rows
+// (inputs); entries past a class count are zero.
+__global__ void k_class_lengths(const uint64_t *ctr, uint64_t nmax,
+                                const uint64_t *big_idx,
+                                const uint64_t *tile_idx,
+                                const uint32_t *seg_stage,
+                                const uint64_t *seg_work,
+                                const uint32_t *seg_tile, uint64_t *big_stage,
+                                uint64_t *big_work, uint64_t *tile_len) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i > nmax) return;
+  uint64_t nb = ctr[kCtrBig], nt = ctr[kCtrTiles];
+  if (i < nb) {
+    uint64_t s = big_idx[i];
+    big_stage[i] = seg_stage[s];
+    big_work[i] = seg_work[s];
+  } else {
+    big_stage[i] = 0;
+    big_work[i] = 0;
+  }
+  tile_len[i] = i < nt ? seg_tile[tile_idx[i]] : 0;
 }
 
 struct SegArgs {
@@ -151,23 +195,96 @@ struct SegArgs {
   const uint32_t *seg_v;
   const uint32_t *seg_p0;
   const uint32_t *seg_olddeg;
-  const uint64_t *stage_off;  // exclusive, t + 1 valid entries
-  const uint64_t *alloc_off;
-  const uint64_t *work_off;
+  const uint64_t *alloc_off;  // per segment, exclusive, t + 1 entries
   uint64_t t, b;
   uint64_t pool_tail;
   uint32_t *scratch;
+  // big rows, compacted: segment ids and exclusive stage / work offsets
+  const uint64_t *big_idx;
+  const uint64_t *stage_off;
+  const uint64_t *work_off;
+  uint64_t nb;
+  // tile rows, compacted: segment ids and exclusive input offsets
+  const uint64_t *tile_idx;
+  const uint64_t *tile_off;
+  uint64_t nt;
 };
 
-__global__ void k_stage(SegArgs a, DevGraph g, uint64_t total) {
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+// First index in [0, n] with a[index] >= x, found by one full wave with
+// 64-way probes (about log64(n) rounds).  a is non-decreasing.
+__device__ __forceinline__ uint64_t wave_lower_bound(const uint64_t *a,
+                                                     uint64_t n, uint64_t x) {
+  const uint32_t lane = lane_id();
+  uint64_t lo = 0, hi = n;  // answer in [lo, hi]
+  while (hi - lo > (uint64_t)kWave) {
+    uint64_t step = (hi - lo + kWave - 1) / kWave;
+    uint64_t p = lo + lane * step;
+    bool ge = p >= hi || a[p] >= x;
+    uint64_t m = __ballot(ge);
+    uint32_t f = m ? (uint32_t)__ffsll((unsigned long long)m) - 1 : kWave;
+    if (f == 0) return lo;
+    uint64_t nlo = lo + (uint64_t)(f - 1) * step + 1;
+    uint64_t nhi = lo + (uint64_t)f * step;
+    lo = nlo;
+    hi = nhi < hi ? nhi : hi;
+  }
+  uint64_t p = lo + lane;
+  bool ge = p >= hi || a[p] >= x;
+  uint64_t m = __ballot(ge);
+  return lo + (uint32_t)__ffsll((unsigned long long)m) - 1;
+}
+
+// Segment range touched by the flat range [e_lo, e_hi) of this block:
+// segments [s_lo, s_hi] of the k offsets off[0..k] (wave 0 searches, the
+// block reads the result from LDS).
+__device__ __forceinline__ void block_segments(const uint64_t *off, uint64_t k,
+                                               uint64_t e_lo, uint64_t e_hi,
+                                               uint64_t *s_lo, uint64_t *s_hi) {
+  __shared__ uint64_t r[2];
+  if (threadIdx.x < kWave) {
+    // largest s with off[s] <= e  ==  lower_bound(off, k + 1, e + 1) - 1
+    uint64_t a = wave_lower_bound(off, k + 1, e_lo + 1) - 1;
+    uint64_t b = wave_lower_bound(off, k + 1, e_hi) - 1;
+    if (threadIdx.x == 0) {
+      r[0] = a;
+      r[1] = b < k ? b : k - 1;
+    }
+  }
+  __syncthreads();
+  *s_lo = r[0];
+  *s_hi = r[1];
+}
+
+// largest s in [lo, hi] with off[s] <= x
+__device__ __forceinline__ uint64_t find_in(const uint64_t *off, uint64_t lo,
+                                            uint64_t hi, uint64_t x) {
+  uint64_t h = hi + 1;
+  while (h - lo > 1) {
+    uint64_t m = (lo + h) >> 1;
+    if (off[m] <= x)
+      lo = m;
+    else
+      h = m;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kT)
+    k_stage(SegArgs a, DevGraph g, uint64_t total) {
+  const uint64_t blo = (uint64_t)blockIdx.x * kT * kChunk;
+  const uint64_t bhi = blo + (uint64_t)kT * kChunk < total
+                           ? blo + (uint64_t)kT * kChunk : total;
+  uint64_t s_lo, s_hi;
+  block_segments(a.stage_off, a.nb, blo, bhi, &s_lo, &s_hi);
+  uint64_t e = blo + (uint64_t)threadIdx.x * kChunk;
   if (e >= total) return;
-  uint64_t s = find_segment(a.stage_off, a.t, e);
   uint64_t end = e + kChunk < total ? e + kChunk : total;
+  uint64_t i = find_in(a.stage_off, s_lo, s_hi, e);
   for (; e < end; e++) {
-    while (e >= a.stage_off[s + 1]) s++;
+    if (e >= a.stage_off[i + 1]) i = find_in(a.stage_off, i, s_hi, e);
+    uint64_t s = a.big_idx[i];
     uint32_t v = a.seg_v[s];
-    a.scratch[e] = g.pool[g.off[v] + a.seg_p0[s] + (e - a.stage_off[s])];
+    a.scratch[e] = g.pool[g.off[v] + a.seg_p0[s] + (e - a.stage_off[i])];
   }
 }
 
@@ -180,9 +297,11 @@ struct MergeSeg {
   uint64_t begin, end;  // flat work range
 };
 
+// Big row i (compacted index): inputs, output and flat work range.
 __device__ __forceinline__ MergeSeg load_seg(const SegArgs &a, const DevGraph &g,
-                                             uint64_t s, bool insert) {
+                                             uint64_t i, bool insert) {
   MergeSeg m;
+  uint64_t s = a.big_idx[i];
   uint64_t bs = a.seg_idx[s], be = (s + 1 < a.t) ? a.seg_idx[s + 1] : a.b;
   m.Bk = a.B + bs;
   m.blen = be - bs;
@@ -193,21 +312,27 @@ __device__ __forceinline__ MergeSeg load_seg(const SegArgs &a, const DevGraph &g
     m.alen = a.seg_olddeg[s];
     m.dst = g.pool + a.pool_tail + a.alloc_off[s];
   } else {
-    m.A = a.scratch + a.stage_off[s];
-    m.alen = a.stage_off[s + 1] - a.stage_off[s];
+    m.A = a.scratch + a.stage_off[i];
+    m.alen = a.stage_off[i + 1] - a.stage_off[i];
     m.dst = g.pool + g.off[v] + a.seg_p0[s];
   }
-  m.begin = a.work_off[s];
-  m.end = a.work_off[s + 1];
+  m.begin = a.work_off[i];
+  m.end = a.work_off[i + 1];
   return m;
 }
 
-__global__ void k_merge_insert(SegArgs a, DevGraph g, uint64_t total) {
-  uint64_t o = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+__global__ void __launch_bounds__(kT)
+    k_merge_insert(SegArgs a, DevGraph g, uint64_t total) {
+  const uint64_t blo = (uint64_t)blockIdx.x * kT * kChunk;
+  const uint64_t bhi = blo + (uint64_t)kT * kChunk < total
+                           ? blo + (uint64_t)kT * kChunk : total;
+  uint64_t s_lo, s_hi;
+  block_segments(a.work_off, a.nb, blo, bhi, &s_lo, &s_hi);
+  uint64_t o = blo + (uint64_t)threadIdx.x * kChunk;
   if (o >= total) return;
   uint64_t end = o + kChunk < total ? o + kChunk : total;
-  uint64_t s = find_segment(a.work_off, a.t, o);
-  MergeSeg m = load_seg(a, g, s, true);
+  uint64_t i = find_in(a.work_off, s_lo, s_hi, o);
+  MergeSeg m = load_seg(a, g, i, true);
   // merge-path split of output index j between A and the row's batch run
   uint64_t j = o - m.begin;
   uint64_t lo = j > m.blen ? j - m.blen : 0, hi = j < m.alen ? j : m.alen;
@@ -218,36 +343,42 @@ __global__ void k_merge_insert(SegArgs a, DevGraph g, uint64_t total) {
     else
       hi = mid;
   }
-  uint64_t i = lo, bi = j - lo;
+  uint64_t ia = lo, ib = j - lo;
   for (; o < end; o++) {
-    while (o >= m.end) {
-      s++;
-      m = load_seg(a, g, s, true);
+    if (o >= m.end) {  // big rows all have work > 0: the next row starts here
+      i++;
+      m = load_seg(a, g, i, true);
       j = 0;
-      i = 0;
-      bi = 0;
+      ia = 0;
+      ib = 0;
     }
     uint32_t x;
-    if (bi >= m.blen || (i < m.alen && m.A[i] < (uint32_t)m.Bk[bi]))
-      x = m.A[i++];
+    if (ib >= m.blen || (ia < m.alen && m.A[ia] < (uint32_t)m.Bk[ib]))
+      x = m.A[ia++];
     else
-      x = (uint32_t)m.Bk[bi++];
+      x = (uint32_t)m.Bk[ib++];
     m.dst[j++] = x;
   }
 }
 
-__global__ void k_compact_remove(SegArgs a, DevGraph g, uint64_t total) {
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+__global__ void __launch_bounds__(kT)
+    k_compact_remove(SegArgs a, DevGraph g, uint64_t total) {
+  const uint64_t blo = (uint64_t)blockIdx.x * kT * kChunk;
+  const uint64_t bhi = blo + (uint64_t)kT * kChunk < total
+                           ? blo + (uint64_t)kT * kChunk : total;
+  uint64_t s_lo, s_hi;
+  block_segments(a.work_off, a.nb, blo, bhi, &s_lo, &s_hi);
+  uint64_t e = blo + (uint64_t)threadIdx.x * kChunk;
   if (e >= total) return;
   uint64_t end = e + kChunk < total ? e + kChunk : total;
-  uint64_t s = find_segment(a.work_off, a.t, e);
-  MergeSeg m = load_seg(a, g, s, false);
-  uint64_t i = e - m.begin;
-  // batch elements below A[i] were removed before position i
+  uint64_t i = find_in(a.work_off, s_lo, s_hi, e);
+  MergeSeg m = load_seg(a, g, i, false);
+  uint64_t ia = e - m.begin;
+  // batch elements below A[ia] were removed before position ia
   uint64_t rb = 0;
   {
     uint64_t lo = 0, hi = m.blen;
-    uint32_t x = m.A[i];
+    uint32_t x = m.A[ia];
     while (lo < hi) {
       uint64_t mid = (lo + hi) >> 1;
       if ((uint32_t)m.Bk[mid] < x)
@@ -258,18 +389,188 @@ __global__ void k_compact_remove(SegArgs a, DevGraph g, uint64_t total) {
     rb = lo;
   }
   for (; e < end; e++) {
-    while (e >= m.end) {
-      s++;
-      m = load_seg(a, g, s, false);
-      i = 0;
+    if (e >= m.end) {
+      i++;
+      m = load_seg(a, g, i, false);
+      ia = 0;
       rb = 0;
     }
-    uint32_t x = m.A[i];
+    uint32_t x = m.A[ia];
     if (rb < m.blen && (uint32_t)m.Bk[rb] == x)
       rb++;
     else
-      m.dst[i - rb] = x;
-    i++;
+      m.dst[ia - rb] = x;
+    ia++;
+  }
+}
+
+// In-place merge of tile rows through LDS.  Block b owns the tile rows whose
+// input range starts in [b*kTile, (b+1)*kTile); it loads every input of those
+// rows (old suffix or whole old row, then the batch run) into LDS,
+// synchronises, and only then writes, so a row is never read after it has
+// been overwritten.  Old element x (index j) lands at j + |{batch < x}|
+// (insert), or at j - |{batch < x}| unless it is in the batch (remove);
+// batch element y (index jb, insert) lands at jb + |{old < y}|.
+// Row description of one tile row: where its inputs come from and where its
+// output goes.
+__device__ __forceinline__ void tile_row_info(const SegArgs &a,
+                                              const DevGraph &g, uint64_t i,
+                                              int insert, uint64_t *src,
+                                              uint64_t *dst, uint32_t *alen,
+                                              uint32_t *bstart) {
+  uint64_t s = a.tile_idx[i];
+  uint32_t v = a.seg_v[s];
+  *bstart = (uint32_t)a.seg_idx[s];
+  if (insert && a.alloc_off[s + 1] > a.alloc_off[s]) {  // relocated
+    *src = g.off[v];
+    *alen = a.seg_olddeg[s];
+    *dst = a.pool_tail + a.alloc_off[s];
+  } else {
+    uint32_t p0 = a.seg_p0[s];
+    *src = g.off[v] + p0;
+    *alen = a.seg_olddeg[s] - p0;
+    *dst = g.off[v] + p0;
+  }
+}
+
+constexpr int kTileRows = 1024;  // rows per block described in LDS
+
+// Flat descriptors of the tile rows (written after any pool rebuild, so the
+// row offsets they hold are current): input source, output destination,
+// old-part length and start of the batch run.
+__global__ void k_tile_rows(SegArgs a, DevGraph g, int insert,
+                            uint64_t *t_src, uint64_t *t_dst,
+                            uint32_t *t_alen, uint32_t *t_bst) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= a.nt) return;
+  uint64_t src, dst;
+  uint32_t alen, bst;
+  tile_row_info(a, g, i, insert, &src, &dst, &alen, &bst);
+  t_src[i] = src;
+  t_dst[i] = dst;
+  t_alen[i] = alen;
+  t_bst[i] = bst;
+}
+
+// First tile row of each block: blk_r0[b] = lower_bound(tile_off, b*kTile).
+__global__ void k_tile_blocks(const uint64_t *tile_off, uint64_t nt,
+                              uint64_t nblocks, uint64_t *blk_r0) {
+  uint64_t b = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (b > nblocks) return;
+  blk_r0[b] = b == nblocks ? nt
+                           : lower_bound_dev(tile_off, nt, b * (uint64_t)kTile);
+}
+
+__global__ void __launch_bounds__(kT)
+    k_merge_tile(SegArgs a, DevGraph g, int insert, const uint64_t *blk_r0,
+                 const uint64_t *t_src, const uint64_t *t_dst,
+                 const uint32_t *t_alen, const uint32_t *t_bst) {
+  __shared__ uint32_t buf[kTileLds];
+  __shared__ uint32_t rstart[kTileRows + 1];  // input offset relative to e0
+  __shared__ uint32_t ralen[kTileRows];
+  __shared__ uint32_t rbstart[kTileRows];
+  __shared__ uint64_t rsrc[kTileRows];
+  __shared__ uint64_t rdst[kTileRows];
+  const uint64_t r0 = blk_r0[blockIdx.x], r1 = blk_r0[blockIdx.x + 1];
+  if (r0 >= r1) return;
+  const uint64_t e0 = a.tile_off[r0], e1 = a.tile_off[r1];
+  const uint32_t W = (uint32_t)(e1 - e0);
+  const uint32_t nrows = (uint32_t)(r1 - r0);
+  const bool table = nrows <= (uint32_t)kTileRows;
+  if (table) {
+    for (uint32_t r = threadIdx.x; r < nrows; r += kT) {
+      rstart[r] = (uint32_t)(a.tile_off[r0 + r] - e0);
+      ralen[r] = t_alen[r0 + r];
+      rbstart[r] = t_bst[r0 + r];
+      rsrc[r] = t_src[r0 + r];
+      rdst[r] = t_dst[r0 + r];
+    }
+    if (threadIdx.x == 0) rstart[nrows] = W;
+    __syncthreads();
+  }
+  uint16_t row_of[kTileItems];
+#pragma unroll
+  for (int k = 0; k < kTileItems; k++) {
+    const uint32_t e = threadIdx.x + (uint32_t)k * kT;
+    row_of[k] = 0;
+    if (e < W) {
+      uint32_t r;
+      uint64_t src;
+      uint32_t alen, bst, j;
+      if (table) {
+        uint32_t l = 0, h = nrows;  // largest r with rstart[r] <= e
+        while (h - l > 1) {
+          uint32_t m = (l + h) >> 1;
+          if (rstart[m] <= e)
+            l = m;
+          else
+            h = m;
+        }
+        r = l;
+        src = rsrc[r];
+        alen = ralen[r];
+        bst = rbstart[r];
+        j = e - rstart[r];
+      } else {
+        uint64_t i = find_in(a.tile_off, r0, r1 - 1, e0 + e);
+        uint64_t dst;
+        tile_row_info(a, g, i, insert, &src, &dst, &alen, &bst);
+        r = (uint32_t)(i - r0);
+        j = (uint32_t)(e0 + e - a.tile_off[i]);
+      }
+      row_of[k] = (uint16_t)r;
+      buf[e] = j < alen ? g.pool[src + j] : (uint32_t)a.B[bst + (j - alen)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kTileItems; k++) {
+    const uint32_t e = threadIdx.x + (uint32_t)k * kT;
+    if (e >= W) continue;
+    const uint32_t r = row_of[k];
+    uint32_t base, alen, blen;
+    uint64_t dst;
+    if (table) {
+      base = rstart[r];
+      blen = rstart[r + 1] - base - ralen[r];
+      alen = ralen[r];
+      dst = rdst[r];
+    } else {
+      uint64_t i = r0 + r;
+      uint64_t src;
+      uint32_t bst;
+      tile_row_info(a, g, i, insert, &src, &dst, &alen, &bst);
+      base = (uint32_t)(a.tile_off[i] - e0);
+      blen = (uint32_t)(a.tile_off[i + 1] - a.tile_off[i]) - alen;
+    }
+    const uint32_t j = e - base;
+    const uint32_t *la = buf + base, *lb = buf + base + alen;
+    const uint32_t x = buf[e];
+    uint32_t *out = g.pool + dst;
+    if (j < alen) {
+      uint32_t l = 0, h = blen;  // batch elements below x
+      while (l < h) {
+        uint32_t m = (l + h) >> 1;
+        if (lb[m] < x)
+          l = m + 1;
+        else
+          h = m;
+      }
+      if (insert)
+        out[j + l] = x;
+      else if (!(l < blen && lb[l] == x))
+        out[j - l] = x;
+    } else if (insert) {
+      uint32_t l = 0, h = alen;  // old elements below x
+      while (l < h) {
+        uint32_t m = (l + h) >> 1;
+        if (la[m] < x)
+          l = m + 1;
+        else
+          h = m;
+      }
+      out[(j - alen) + l] = x;
+    }
   }
 }
 
@@ -515,23 +816,47 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
               (const uint64_t *)ctr, n, flags);
   SSTG_TRY(select_flagged_u64(flags, nullptr, n, seg_idx, ctr + kCtrT, ws,
                               kWsPrim, s));
-  uint32_t *seg_v = nullptr, *seg_p0 = nullptr, *seg_od = nullptr;
-  uint64_t *seg_stage = nullptr, *seg_alloc = nullptr, *seg_work = nullptr;
+  uint32_t *seg_v = nullptr, *seg_p0 = nullptr, *seg_od = nullptr,
+           *seg_stage = nullptr, *seg_tile = nullptr;
+  uint64_t *seg_alloc = nullptr, *seg_work = nullptr, *big_idx = nullptr,
+           *tile_idx = nullptr, *big_stage = nullptr, *big_work = nullptr,
+           *tile_len = nullptr;
+  uint8_t *is_big = nullptr, *is_tile = nullptr;
   SSTG_TRY(ws.get(kWsSegV, nseg + 1, &seg_v));
   SSTG_TRY(ws.get(kWsSegP0, nseg + 1, &seg_p0));
   SSTG_TRY(ws.get(kWsSegOldDeg, nseg + 1, &seg_od));
   SSTG_TRY(ws.get(kWsSegStage, nseg + 1, &seg_stage));
   SSTG_TRY(ws.get(kWsSegAlloc, nseg + 1, &seg_alloc));
   SSTG_TRY(ws.get(kWsSegWork, nseg + 1, &seg_work));
+  SSTG_TRY(ws.get(kWsSegTile, nseg + 1, &seg_tile));
+  SSTG_TRY(ws.get(kWsSegIsBig, nseg + 1, &is_big));
+  SSTG_TRY(ws.get(kWsSegIsTile, nseg + 1, &is_tile));
+  SSTG_TRY(ws.get(kWsBigIdx, nseg + 1, &big_idx));
+  SSTG_TRY(ws.get(kWsTileIdx, nseg + 1, &tile_idx));
+  SSTG_TRY(ws.get(kWsBigStage, nseg + 1, &big_stage));
+  SSTG_TRY(ws.get(kWsBigWork, nseg + 1, &big_work));
+  SSTG_TRY(ws.get(kWsTileLen, nseg + 1, &tile_len));
   SSTG_LAUNCH(k_segprep, blocks(nseg + 1), kT, 0, s, (const uint64_t *)B,
               (const uint64_t *)ctr, (const uint64_t *)seg_idx, g->view(),
               insert ? 1 : 0, g->n_cols, nseg, seg_v, seg_p0, seg_od,
-              seg_stage, seg_alloc, seg_work, ctr + kCtrGarbage);
-  SSTG_TRY(scan_exclusive_u64(seg_stage, seg_stage, nseg + 1, ctr + kCtrStage,
-                              ws, kWsPrim, s));
+              seg_alloc, seg_stage, seg_work, seg_tile, is_big, is_tile,
+              ctr + kCtrGarbage);
   SSTG_TRY(scan_exclusive_u64(seg_alloc, seg_alloc, nseg + 1, ctr + kCtrAlloc,
                               ws, kWsPrim, s));
-  SSTG_TRY(scan_exclusive_u64(seg_work, seg_work, nseg + 1, ctr + kCtrWork, ws,
+  SSTG_TRY(select_flagged_u64(is_big, nullptr, nseg + 1, big_idx,
+                              ctr + kCtrBig, ws, kWsPrim, s));
+  SSTG_TRY(select_flagged_u64(is_tile, nullptr, nseg + 1, tile_idx,
+                              ctr + kCtrTiles, ws, kWsPrim, s));
+  SSTG_LAUNCH(k_class_lengths, blocks(nseg + 1), kT, 0, s,
+              (const uint64_t *)ctr, nseg, (const uint64_t *)big_idx,
+              (const uint64_t *)tile_idx, (const uint32_t *)seg_stage,
+              (const uint64_t *)seg_work, (const uint32_t *)seg_tile,
+              big_stage, big_work, tile_len);
+  SSTG_TRY(scan_exclusive_u64(big_stage, big_stage, nseg + 1, ctr + kCtrStage,
+                              ws, kWsPrim, s));
+  SSTG_TRY(scan_exclusive_u64(big_work, big_work, nseg + 1, ctr + kCtrWork, ws,
+                              kWsPrim, s));
+  SSTG_TRY(scan_exclusive_u64(tile_len, tile_len, nseg + 1, ctr + kCtrTile, ws,
                               kWsPrim, s));
   SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, kCtrCount * sizeof(uint64_t),
                           hipMemcpyDeviceToHost, s));
@@ -540,7 +865,16 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   if (h[kCtrErr]) return SSTG_E_RANGE;
   const uint64_t b = h[kCtrB], t = h[kCtrT];
   const uint64_t stage_tot = h[kCtrStage], alloc_tot = h[kCtrAlloc],
-                 work_tot = h[kCtrWork], garbage_add = h[kCtrGarbage];
+                 work_tot = h[kCtrWork], garbage_add = h[kCtrGarbage],
+                 tile_tot = h[kCtrTile], nb = h[kCtrBig], nt = h[kCtrTiles];
+  g->last.effective += b;
+  g->last.rows += t;
+  g->last.tile_rows += nt;
+  g->last.tile_inputs += tile_tot;
+  g->last.big_rows += nb;
+  g->last.staged += stage_tot;
+  g->last.merged += work_tot;
+  g->last.allocated += alloc_tot;
   if (b == 0) return SSTG_OK;
   if (g->pool_tail + alloc_tot > g->pool_cap) {
     // slot sizes are kept, so the decisions made above stay valid
@@ -554,13 +888,36 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   a.seg_v = seg_v;
   a.seg_p0 = seg_p0;
   a.seg_olddeg = seg_od;
-  a.stage_off = seg_stage;
   a.alloc_off = seg_alloc;
-  a.work_off = seg_work;
   a.t = t;
   a.b = b;
   a.pool_tail = g->pool_tail;
   a.scratch = scratch;
+  a.big_idx = big_idx;
+  a.stage_off = big_stage;
+  a.work_off = big_work;
+  a.nb = nb;
+  a.tile_idx = tile_idx;
+  a.tile_off = tile_len;
+  a.nt = nt;
+  if (tile_tot) {
+    const uint64_t nblk = ceil_div(tile_tot, kTile);
+    uint64_t *t_src = nullptr, *t_dst = nullptr, *blk_r0 = nullptr;
+    uint32_t *t_alen = nullptr, *t_bst = nullptr;
+    SSTG_TRY(ws.get(kWsTileSrc, nt + 1, &t_src));
+    SSTG_TRY(ws.get(kWsTileDst, nt + 1, &t_dst));
+    SSTG_TRY(ws.get(kWsTileAlen, nt + 1, &t_alen));
+    SSTG_TRY(ws.get(kWsTileBst, nt + 1, &t_bst));
+    SSTG_TRY(ws.get(kWsTileBlk, nblk + 1, &blk_r0));
+    SSTG_LAUNCH(k_tile_rows, blocks(nt), kT, 0, s, a, g->view(),
+                insert ? 1 : 0, t_src, t_dst, t_alen, t_bst);
+    SSTG_LAUNCH(k_tile_blocks, blocks(nblk + 1), kT, 0, s,
+                (const uint64_t *)tile_len, nt, nblk, blk_r0);
+    SSTG_LAUNCH(k_merge_tile, nblk, kT, 0, s, a, g->view(), insert ? 1 : 0,
+                (const uint64_t *)blk_r0, (const uint64_t *)t_src,
+                (const uint64_t *)t_dst, (const uint32_t *)t_alen,
+                (const uint32_t *)t_bst);
+  }
   if (stage_tot)
     SSTG_LAUNCH(k_stage, chunk_blocks(stage_tot), kT, 0, s, a, g->view(),
                 stage_tot);
@@ -601,6 +958,8 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
 
 int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                  uint64_t n, bool insert, bool skip_self, bool device_ptrs) {
+  g->last = sstg_update_stats{};
+  g->last.batch = n;
   if (n == 0) return SSTG_OK;
   // batches above 2^31 pairs are applied in pieces; union and difference are
   // unaffected by the split
@@ -621,6 +980,7 @@ int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
     }
     SSTG_TRY(update_chunk(g, s, d, len, insert, skip_self));
   }
+  g->last.rebuilds = g->rebuilds;
   return SSTG_OK;
 }
 
