@@ -67,3 +67,11 @@ def test_product_never_references_the_oracle():
         p = os.path.join(ROOT, "include", f)
         if os.path.isfile(p):
             assert not pat.search(open(p).read()), f
+
+
+def test_cpp_facade_builds_and_skips_without_gpu():
+    from tests.cpp import build as cpp_build
+    out = cpp_build.build()
+    r = subprocess.run([out], capture_output=True, text=True, timeout=60)
+    if S.device_count() == 0:
+        assert r.returncode == 77 and "skipped" in r.stdout
