@@ -145,12 +145,47 @@ int sstg_bf(sstg_graph *g, uint32_t src, int32_t *out, uint32_t flags);
 /* TouchAll(g), TST:476: sum over all edges of the destination id. */
 int sstg_touch_all(sstg_graph *g, uint64_t *out);
 
+/* ---- many independent graphs processed together ------------------------
+ * No single reference counterpart: the reference processes one graph per
+ * call.  k graphs live in one device store (their disjoint union); one
+ * replay of a captured hipGraph runs BFS (from one source per graph), CC and
+ * PageRank on all of them.  Results are per vertex in graph order: entries
+ * [vbase[g], vbase[g+1]) belong to graph g, with vbase the prefix sum of
+ * n_per_graph; levels and labels are local to each graph. */
+typedef struct sstg_multi sstg_multi;
+int sstg_multi_create(uint32_t k, const uint32_t *n_per_graph,
+                      sstg_multi **out);
+int sstg_multi_destroy(sstg_multi *m);
+/* insert_batch (SM:556-643) into graph `graph`; ids are local to it. */
+int sstg_multi_insert_batch(sstg_multi *m, uint32_t graph, const uint32_t *src,
+                            const uint32_t *dst, uint64_t n, uint32_t flags);
+/* The union as an ordinary graph handle (owned by m; do not destroy). */
+int sstg_multi_graph(sstg_multi *m, sstg_graph **out);
+/* BFS from sources[g] in every graph g (levels as sstg_bfs), CC labels (as
+ * sstg_cc) and pr_iters PageRank iterations with each graph normalised by
+ * its own vertex count (FP64, or FP32 with Kahan sums if pr_f32), all in one
+ * hipGraph replay.  max_levels is the BFS depth recorded in the graph;
+ * deeper BFS levels continue with plain launches. */
+int sstg_multi_run(sstg_multi *m, const uint32_t *sources, uint32_t max_levels,
+                   uint32_t pr_iters, double damping, int pr_f32);
+/* Host copies of the last run's results (any pointer may be NULL); each
+ * array has sum(n_per_graph) entries.  levels_used = BFS levels executed. */
+int sstg_multi_results(sstg_multi *m, uint32_t *level, uint32_t *label,
+                       double *rank64, float *rank32, uint32_t *levels_used);
+
 /* ---- workload generator (rmat_util.h:82-131 rMat<uint32_t>) ------------ */
 /* Edges i in [first, first+count) of rMat(n, seed, a, b, c) into src/dst
  * (host or device arrays per flags).  n must be a power of two. */
 int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
               uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
               uint32_t flags);
+
+/* Seeded Erdos-Renyi edge stream on n vertices (no reference counterpart:
+ * the reference's ER tool, graph_utilities.cpp:168-212, is unseeded):
+ * edge i = (hash64(seed << 32 | 2i) mod n, hash64(seed << 32 | 2i+1) mod n)
+ * with rmat_util.h's hash64. */
+int sstg_er(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
+            uint32_t *src, uint32_t *dst, uint32_t flags);
 
 /* ---- device memory plumbing for callers without a HIP runtime of their
  * own (tests, benchmarks, FFI hosts) ------------------------------------ */

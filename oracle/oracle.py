@@ -43,6 +43,7 @@ def lib():
         "orc_aspen_ith_rand": (u64, [u64, u64]),
         "orc_log2_up": (u32, [u64]),
         "orc_rmat": (ctypes.c_int, [u32, u32, f64, f64, f64, u64, u64, _u32p, _u32p]),
+        "orc_er": (ctypes.c_int, [u32, u32, u64, u64, _u32p, _u32p]),
         "orc_graph_create": (vp, [u32, u32]),
         "orc_graph_destroy": (None, [vp]),
         "orc_insert_batch": (ctypes.c_int, [vp, _u32p, _u32p, u64]),
@@ -88,6 +89,14 @@ def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1):
     rc = lib().orc_rmat(n, seed, a, b, c, first, count, src, dst)
     if rc != 0:
         raise ValueError("rMat: n must be a power of two and a+b+c <= 1")
+    return src, dst
+
+
+def er(n, seed, count, first=0):
+    src = np.empty(count, dtype=np.uint32)
+    dst = np.empty(count, dtype=np.uint32)
+    if lib().orc_er(n, seed, first, count, src, dst) != 0:
+        raise ValueError("er: n must be positive")
     return src, dst
 
 

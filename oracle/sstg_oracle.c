@@ -126,6 +126,22 @@ int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
   return 0;
 }
 
+/* Seeded Erdos-Renyi edge stream (no reference counterpart; the reference's
+ * graph_utilities.cpp ER tool is unseeded): edge i = (hash64(seed<<32|2i) mod
+ * n, hash64(seed<<32|2i+1) mod n). */
+int orc_er(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
+           uint32_t *src, uint32_t *dst) {
+  if (n == 0) return -1;
+  uint64_t base = (uint64_t)seed << 32;
+#pragma omp parallel for schedule(static) if (count > 100000)
+  for (int64_t t = 0; t < (int64_t)count; t++) {
+    uint64_t i = first + (uint64_t)t;
+    src[t] = (uint32_t)(orc_hash64(base + 2 * i) % n);
+    dst[t] = (uint32_t)(orc_hash64(base + 2 * i + 1) % n);
+  }
+  return 0;
+}
+
 /* ===================================================================== */
 /* Parallel LSD radix sort of 64-bit keys (row << 32 | col)               */
 /* ===================================================================== */

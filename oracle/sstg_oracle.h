@@ -30,6 +30,10 @@ uint32_t orc_log2_up(uint64_t i);                /* rmat_util.h:72-80 */
 int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
              uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst);
 
+/* Seeded Erdos-Renyi edge stream (own definition, see sstg_oracle.c). */
+int orc_er(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
+           uint32_t *src, uint32_t *dst);
+
 /* ---- the tinyset-semantics graph store (SparseMatrixV<true>) ---------- */
 typedef struct orc_graph orc_graph;
 orc_graph *orc_graph_create(uint32_t n_rows, uint32_t n_cols); /* SM:73-87 */

@@ -109,6 +109,13 @@ def lib():
         "sstg_bf": (i32, [vp, u32, vp, u32]),
         "sstg_rmat": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                             ctypes.c_double, u64, u64, vp, vp, u32]),
+        "sstg_er": (i32, [u32, u32, u64, u64, vp, vp, u32]),
+        "sstg_multi_create": (i32, [u32, vp, P(vp)]),
+        "sstg_multi_destroy": (i32, [vp]),
+        "sstg_multi_insert_batch": (i32, [vp, u32, vp, vp, u64, u32]),
+        "sstg_multi_graph": (i32, [vp, P(vp)]),
+        "sstg_multi_run": (i32, [vp, vp, u32, u32, ctypes.c_double, i32]),
+        "sstg_multi_results": (i32, [vp, vp, vp, vp, vp, P(u32)]),
         "sstg_device_alloc": (i32, [u64, P(vp)]),
         "sstg_device_free": (i32, [vp]),
         "sstg_memcpy_h2d": (i32, [vp, vp, u64]),
@@ -213,6 +220,74 @@ def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1, device=False):
     _check(lib().sstg_rmat(n, seed, a, b, c, first, count, _ptr(s), _ptr(d), 0),
            "sstg_rmat")
     return s, d
+
+
+def er(n, seed, count, first=0, device=False):
+    """Seeded Erdos-Renyi edge stream (see sstg.h sstg_er), on the GPU."""
+    if device:
+        s, d = DeviceArray(count, np.uint32), DeviceArray(count, np.uint32)
+        _check(lib().sstg_er(n, seed, first, count, s.ptr, d.ptr,
+                             SSTG_DEVICE_PTRS), "sstg_er")
+        return s, d
+    s = np.empty(count, np.uint32)
+    d = np.empty(count, np.uint32)
+    _check(lib().sstg_er(n, seed, first, count, _ptr(s), _ptr(d), 0), "sstg_er")
+    return s, d
+
+
+class MultiGraph:
+    """k independent graphs in one device store; BFS + CC + PageRank on all
+    of them in one captured hipGraph replay (sstg_multi_*)."""
+
+    def __init__(self, sizes):
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        self.k = len(self.sizes)
+        self.vbase = np.concatenate([[0], np.cumsum(self.sizes.astype(np.int64))])
+        h = ctypes.c_void_p()
+        _check(lib().sstg_multi_create(self.k, _ptr(self.sizes), ctypes.byref(h)),
+               "sstg_multi_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().sstg_multi_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def insert_batch(self, graph, src, dst):
+        if isinstance(src, DeviceArray):
+            _check(lib().sstg_multi_insert_batch(self._h, graph, src.ptr, dst.ptr,
+                                                 min(src.count, dst.count),
+                                                 SSTG_DEVICE_PTRS), "multi insert")
+            return
+        s, d = _u32(src), _u32(dst)
+        _check(lib().sstg_multi_insert_batch(self._h, graph, _ptr(s), _ptr(d),
+                                             s.size, 0), "multi insert")
+
+    def run(self, sources, max_levels=64, pr_iters=10, damping=0.85, f32=False):
+        src = _u32(sources)
+        _check(lib().sstg_multi_run(self._h, _ptr(src), max_levels, pr_iters,
+                                    damping, 1 if f32 else 0), "multi run")
+
+    def results(self, f32=False):
+        n = int(self.vbase[-1])
+        level = np.empty(n, np.uint32)
+        label = np.empty(n, np.uint32)
+        rank = np.empty(n, np.float32 if f32 else np.float64)
+        used = ctypes.c_uint32(0)
+        _check(lib().sstg_multi_results(self._h, _ptr(level), _ptr(label),
+                                        None if f32 else _ptr(rank),
+                                        _ptr(rank) if f32 else None,
+                                        ctypes.byref(used)), "multi results")
+        return dict(level=level, label=label, rank=rank, levels_used=used.value)
+
+    def part(self, arr, g):
+        return arr[self.vbase[g]:self.vbase[g + 1]]
 
 
 class SparseMatrixV:

@@ -446,6 +446,29 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
   return rc;
 }
 
+int sstg_er(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
+            uint32_t *src, uint32_t *dst, uint32_t flags) {
+  if (count && (!src || !dst)) return SSTG_E_INVAL;
+  int dev = 0;
+  if (!device_ok(&dev)) return SSTG_E_NODEVICE;
+  if (flags & SSTG_DEVICE_PTRS) {
+    SSTG_TRY(er_generate(n, seed, first, count, src, dst, nullptr));
+    SSTG_HIP(hipStreamSynchronize(nullptr));
+    return SSTG_OK;
+  }
+  uint32_t *ds = nullptr, *dd = nullptr;
+  SSTG_HIP(hipMalloc(&ds, (count ? count : 1) * sizeof(uint32_t)));
+  SSTG_HIP(hipMalloc(&dd, (count ? count : 1) * sizeof(uint32_t)));
+  int rc = er_generate(n, seed, first, count, ds, dd, nullptr);
+  if (rc == SSTG_OK && count) {
+    hipMemcpy(src, ds, count * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    hipMemcpy(dst, dd, count * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  }
+  hipFree(ds);
+  hipFree(dd);
+  return rc;
+}
+
 int sstg_device_alloc(uint64_t bytes, void **out) {
   if (!out) return SSTG_E_INVAL;
   SSTG_HIP(hipMalloc(out, bytes ? bytes : 1));

@@ -34,6 +34,7 @@ struct EdgeMapCtx {
 };
 
 namespace em {
+namespace {  // internal linkage: this header is included by several files
 
 constexpr int kT = 256;
 constexpr int kChunk = 8;
@@ -305,6 +306,7 @@ __global__ void __launch_bounds__(kT)
   if (lig == 0) f.store(v, acc);
 }
 
+}  // namespace
 }  // namespace em
 
 inline int edge_map_ctx(sstg_graph *g, EdgeMapCtx *ctx, int slot) {
@@ -373,6 +375,7 @@ struct DegreeBins {
 };
 
 namespace em {
+namespace {  // internal linkage: this header is included by several files
 __global__ void k_bin_flags(const uint32_t *deg, uint32_t n, uint32_t lo,
                             uint32_t hi, uint8_t *flags) {
   uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
@@ -381,6 +384,7 @@ __global__ void k_bin_flags(const uint32_t *deg, uint32_t n, uint32_t lo,
     flags[v] = d >= lo && d <= hi;
   }
 }
+}  // namespace
 }  // namespace em
 
 inline int degree_bins(sstg_graph *g, DegreeBins *out) {

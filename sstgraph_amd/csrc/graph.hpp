@@ -58,6 +58,8 @@ int graph_touch_sum(sstg_graph *g, uint64_t *out);
 int graph_is_symmetric(sstg_graph *g, bool *out);
 
 // rmat.hip
+int er_generate(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
+                uint32_t *src, uint32_t *dst, hipStream_t s);
 int rmat_generate(uint32_t n, uint32_t seed, double a, double b, double c,
                   uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
                   hipStream_t s);
@@ -94,6 +96,7 @@ enum : int {
   kWsTileAlen = 91,
   kWsTileBst = 92,
   kWsTileBlk = 93,
+  kWsMulti = 94,  // 94..97
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range

@@ -72,7 +72,40 @@ __global__ void k_rmat(uint32_t n, uint32_t lg, uint32_t h, uint64_t ta,
   dst[t] = c;
 }
 
+__device__ __forceinline__ uint64_t hash64_dev(uint64_t u) {
+  uint64_t v = u * 3935559000370003845ull + 2691343689449507681ull;
+  v ^= v >> 21u;
+  v ^= v << 37u;
+  v ^= v >> 4u;
+  v *= 4768777513237032717ull;
+  v ^= v << 20u;
+  v ^= v >> 41u;
+  v ^= v << 5u;
+  return v;
+}
+
+// Seeded Erdos-Renyi G(n, m) edge stream: edge i = (hash64(seed<<32 | 2i)
+// mod n, hash64(seed<<32 | 2i+1) mod n).
+__global__ void k_er(uint32_t n, uint64_t seed, uint64_t first, uint64_t count,
+                     uint32_t *src, uint32_t *dst) {
+  uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  uint64_t i = first + t;
+  uint64_t base = seed << 32;
+  src[t] = (uint32_t)(hash64_dev(base + 2 * i) % n);
+  dst[t] = (uint32_t)(hash64_dev(base + 2 * i + 1) % n);
+}
+
 }  // namespace
+
+int er_generate(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
+                uint32_t *src, uint32_t *dst, hipStream_t s) {
+  if (n == 0) return SSTG_E_INVAL;
+  if (count == 0) return SSTG_OK;
+  SSTG_LAUNCH(k_er, ceil_div(count, 256), 256, 0, s, n, (uint64_t)seed, first,
+              count, src, dst);
+  return SSTG_OK;
+}
 
 int rmat_generate(uint32_t n, uint32_t seed, double a, double b, double c,
                   uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,

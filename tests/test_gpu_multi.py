@@ -1,0 +1,98 @@
+"""Many graphs in one device store (sstg_multi_*): BFS, CC and PageRank for
+all graphs from one captured hipGraph replay must equal each graph's own
+oracle results (levels and labels bit-exact, FP64 PR within 1e-9, FP32+Kahan
+PR within 1e-5)."""
+import numpy as np
+import pytest
+
+import sstgraph_amd as S
+from oracle import oracle as O
+from tests.gpu_common import need_gpu, rel_close, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(k, seed):
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for g in range(k):
+        scale = int(rng.integers(4, 12))
+        n = 1 << scale
+        if g % 2 == 0:
+            s, d = O.rmat(n, 1000 + g, int(rng.integers(8, 17)) * n)
+        else:
+            s, d = O.er(n, 2000 + g, n * int(rng.integers(2, 9)))
+        s, d = np.concatenate([s, d]), np.concatenate([d, s])
+        og = O.Graph(n)
+        og.insert_batch(s, d)
+        off, cols = og.csr()
+        src = int(np.argmax(np.diff(off)))
+        graphs.append((n, s, d, off, cols, src))
+    graphs.append((1, np.zeros(0, np.uint32), np.zeros(0, np.uint32),
+                   np.zeros(2, np.uint64), np.zeros(0, np.uint32), 0))
+    return graphs
+
+
+@pytest.mark.parametrize("max_levels", [64, 2])
+def test_multi_matches_per_graph_oracle(max_levels):
+    need_gpu()
+    graphs = _make(23, 5)
+    M = S.MultiGraph([g[0] for g in graphs])
+    for i, (n, s, d, off, cols, src) in enumerate(graphs):
+        if len(s):
+            M.insert_batch(i, s, d)
+    sources = [g[5] for g in graphs]
+    for f32 in (False, True):
+        say("multi run", max_levels, "f32" if f32 else "f64")
+        M.run(sources, max_levels=max_levels, pr_iters=10, f32=f32)
+        r = M.results(f32=f32)
+        for i, (n, s, d, off, cols, src) in enumerate(graphs):
+            lev, _ = O.bfs(off, cols, src)
+            assert np.array_equal(M.part(r["level"], i), lev), i
+            assert np.array_equal(M.part(r["label"], i), O.cc(off, cols)), i
+            want = O.pagerank(off, cols, 10, 0.85)
+            ok, w = rel_close(M.part(r["rank"], i), want, 1e-5 if f32 else 1e-9)
+            assert ok, (i, w)
+
+
+def test_multi_equals_single_graph_runs():
+    """Each graph's result is the same alone (its own SparseMatrixV) and in
+    the batch."""
+    need_gpu()
+    graphs = _make(6, 9)
+    M = S.MultiGraph([g[0] for g in graphs])
+    singles = []
+    for i, (n, s, d, off, cols, src) in enumerate(graphs):
+        if len(s):
+            M.insert_batch(i, s, d)
+        g = S.SparseMatrixV(n)
+        if len(s):
+            g.insert_batch(s, d)
+        singles.append(g)
+    M.run([g[5] for g in graphs], pr_iters=7)
+    r = M.results()
+    for i, g in enumerate(singles):
+        _, lev = S.EdgeMapVertexMap.BFS(g, graphs[i][5], levels=True)
+        assert np.array_equal(M.part(r["level"], i), lev)
+        assert np.array_equal(M.part(r["label"], i), S.EdgeMapVertexMap.CC(g))
+        ok, w = rel_close(M.part(r["rank"], i), S.EdgeMapVertexMap.PR_S(g, 7), 1e-12)
+        assert ok, w
+
+
+def test_multi_rejects_out_of_range_local_ids():
+    need_gpu()
+    M = S.MultiGraph([10, 20])
+    with pytest.raises(S.IdOutOfRange):
+        M.insert_batch(0, np.array([1, 10], np.uint32), np.array([2, 3], np.uint32))
+    M.insert_batch(1, np.array([1, 19], np.uint32), np.array([19, 1], np.uint32))
+    M.run([0, 1], pr_iters=1)
+    r = M.results()
+    assert M.part(r["level"], 1)[19] == 1
+
+
+def test_er_generator_device_matches_oracle():
+    need_gpu()
+    for n, seed, cnt, first in [(1000, 3, 5000, 0), (1 << 20, 7, 1 << 16, 12345)]:
+        s, d = S.er(n, seed, cnt, first=first)
+        os_, od = O.er(n, seed, cnt, first=first)
+        assert np.array_equal(s, os_) and np.array_equal(d, od)

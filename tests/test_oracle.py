@@ -324,3 +324,16 @@ def test_bf_unit_weights_equals_bfs_levels():
     unreached = lev == 0xFFFFFFFF
     assert np.all(dist[unreached] == (2**31 - 1) // 2)
     assert np.all(dist[~unreached] == lev[~unreached])
+
+
+def test_er_generator_is_uniform_and_seeded():
+    n = 1000
+    s, d = O.er(n, 11, 200000)
+    assert s.max() < n and d.max() < n
+    counts = np.bincount(np.concatenate([s, d]), minlength=n)
+    # 400 draws per vertex on average; a uniform stream stays within 6 sigma
+    assert abs(counts.mean() - 400) < 1e-9 and counts.std() < 6 * 20
+    s2, _ = O.er(n, 11, 10, first=5)
+    assert np.array_equal(s2, s[5:15])
+    s3, _ = O.er(n, 12, 10)
+    assert not np.array_equal(s3, s[:10])
