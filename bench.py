@@ -15,6 +15,8 @@ reported as GTEPS under "extra".
 
     python bench.py [--steps K] [--warmup W] [--scale 24] [--batch 1e7]
                     [--no-cpu] [--gpus 1]
+    python bench.py --workload batched   # 512 graphs, one hipGraph replay
+    python bench.py --workload serve     # served request stream, p50/p99
 """
 import argparse
 import ctypes
@@ -105,6 +107,241 @@ def cpu_baseline(off, cols, n, bs, bd, budget_s=30.0):
                        f"the scale-24 base graph, median of {len(times)} runs")
 
 
+def c4_specs(k=512, seed=3):
+    """Graph specs of the batched workload: scale = 8 + floor(11 U) in
+    [8, 18]; even graphs rMat with edge factor 8..32, odd graphs Erdos-Renyi
+    with average degree 4..64 (uniforms from hash64 of the seed)."""
+    specs = []
+    for g in range(k):
+        u = hash64_np(np.array([(seed << 32) | (3 * g + j) for j in range(3)],
+                               dtype=np.uint64)).astype(np.float64) / 2.0**64
+        scale = min(18, 8 + int(11 * u[0]))
+        if g % 2 == 0:
+            specs.append(("rmat", scale, 8 + int(25 * u[1]), 1000 + g))
+        else:
+            specs.append(("er", scale, 4 + int(61 * u[2]), 2000 + g))
+    return specs
+
+
+def c4_edges(S, spec):
+    """Symmetrised edge stream of one batched-workload graph, on the device."""
+    kind, scale, param, seed = spec
+    n = 1 << scale
+    count = param * n if kind == "rmat" else param * n // 2
+    L = S.lib()
+    src = S.DeviceArray(2 * count, np.uint32)
+    dst = S.DeviceArray(2 * count, np.uint32)
+    if kind == "rmat":
+        S._check(L.sstg_rmat(n, seed, 0.5, 0.1, 0.1, 0, count, src.ptr, dst.ptr,
+                             S.SSTG_DEVICE_PTRS), "rmat")
+    else:
+        S._check(L.sstg_er(n, seed, 0, count, src.ptr, dst.ptr,
+                           S.SSTG_DEVICE_PTRS), "er")
+    S._check(L.sstg_memcpy_d2d(ctypes.c_void_p(src.ptr.value + 4 * count),
+                               dst.ptr, 4 * count), "d2d")
+    S._check(L.sstg_memcpy_d2d(ctypes.c_void_p(dst.ptr.value + 4 * count),
+                               src.ptr, 4 * count), "d2d")
+    return n, src, dst
+
+
+def run_batched(args):
+    """512 graphs: BFS (from each graph's largest hub), CC and 10 PageRank
+    iterations on all of them from one hipGraph replay, against per-graph
+    plain launches of the same three algorithms."""
+    import sstgraph_amd as S
+    specs = c4_specs(args.graphs)
+    sizes = [1 << sp[1] for sp in specs]
+    M = S.MultiGraph(sizes)
+    plain = []
+    m_total = 0
+    t0 = time.perf_counter()
+    for g, sp in enumerate(specs):
+        n, src, dst = c4_edges(S, sp)
+        M.insert_batch(g, src, dst)
+        pg = S.SparseMatrixV(n)
+        pg.insert_batch(src, dst)
+        m_total += pg.M()
+        plain.append(pg)
+        src.free()
+        dst.free()
+    log(f"built {len(specs)} graphs ({sum(sizes)} vertices, {m_total} directed "
+        f"edges) in {time.perf_counter() - t0:.1f}s")
+    sources = [int(np.argmax(pg.degrees())) for pg in plain]
+    iters = 10
+    # batched: one hipGraph replay per step
+    for _ in range(args.warmup):
+        M.run(sources, pr_iters=iters)
+    S.synchronize()
+    tb = [timed(lambda: M.run(sources, pr_iters=iters)) for _ in range(args.steps)]
+    res64 = M.results()
+    M.run(sources, pr_iters=iters, f32=True)
+    res32 = M.results(f32=True)
+    rel = np.abs(res32["rank"].astype(np.float64) - res64["rank"]) / res64["rank"]
+    # plain: three calls per graph with device-resident outputs
+    EMV = S.EdgeMapVertexMap
+    outs = [(S.DeviceArray(pg.n_rows, np.int32), S.DeviceArray(pg.n_rows, np.uint32),
+             S.DeviceArray(pg.n_rows, np.float64)) for pg in plain]
+
+    def plain_pass():
+        for pg, srcv, (dp, dl, dr) in zip(plain, sources, outs):
+            EMV.BFS(pg, srcv, out=(None, dl))
+            S._check(S.lib().sstg_cc(pg.handle, dp.ptr, S.SSTG_DEVICE_PTRS), "cc")
+            EMV.PR_S(pg, iters, out=dr)
+        S.synchronize()
+
+    for _ in range(args.warmup):
+        plain_pass()
+    tp = [timed(plain_pass) for _ in range(max(1, args.steps // 2))]
+    # same results alone and batched (spot check on a few graphs)
+    for g in range(0, len(plain), max(1, len(plain) // 16)):
+        _, dl, dr = outs[g]
+        assert np.array_equal(M.part(res64["level"], g), dl.to_host())
+        pr = dr.to_host()
+        assert np.allclose(M.part(res64["rank"], g), pr, rtol=1e-9, atol=0)
+    t_b, t_p = float(np.median(tb)), float(np.median(tp))
+    out = {
+        "metric": "batched BFS+CC+PR(10) over 512 graphs: graph-algorithm passes/s",
+        "value": 1.0 / t_b, "unit": "passes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_b, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (PR), u32",
+        "data": "synthetic (rMat + seeded Erdos-Renyi)",
+        "config": {"workload": f"{len(specs)} graphs, scale 8..18, half rMat (edge "
+                               "factor 8..32), half ER (avg degree 4..64), seed 3",
+                   "vertices": int(sum(sizes)), "directed_edges": int(m_total)},
+        "extra": {"batched_ms": 1e3 * t_b, "plain_ms": 1e3 * t_p,
+                  "speedup_vs_plain": t_p / t_b,
+                  "edges_per_s_batched": m_total * (iters + 2) / t_b,
+                  "bfs_levels": res64["levels_used"],
+                  "pr_f32_kahan_max_rel_err_vs_f64": float(rel.max()),
+                  "pr_f32_kahan_mean_rel_err_vs_f64": float(rel.mean())},
+    }
+    log(f"batched {1e3 * t_b:.2f} ms vs plain {1e3 * t_p:.2f} ms per pass "
+        f"({t_p / t_b:.1f}x); FP32+Kahan PR max rel err {rel.max():.2e}")
+    print(json.dumps(out))
+
+
+def serve_trace(n, deg, count, rate, seed=4):
+    """Seeded request trace: Poisson arrivals at `rate` per second; 55% update
+    batches (70% inserts of fresh rMat edges with sizes log-uniform in
+    [1e2, 1e6], 30% removes of an earlier inserted batch), 25% BFS from a
+    random non-isolated vertex, 12% PageRank (10 iterations), 8% TC."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    nz = np.nonzero(deg)[0]
+    reqs, inserted = [], []
+    for i in range(count):
+        r = rng.random()
+        if r < 0.55:
+            if inserted and rng.random() < 0.3:
+                reqs.append(("remove", inserted[int(rng.integers(len(inserted)))]))
+            else:
+                size = int(10 ** rng.uniform(2, 6))
+                reqs.append(("insert", (size, 100 + i)))
+                inserted.append((size, 100 + i))
+        elif r < 0.80:
+            reqs.append(("bfs", int(nz[int(rng.integers(len(nz)))])))
+        elif r < 0.92:
+            reqs.append(("pr", 10))
+        else:
+            reqs.append(("tc", None))
+    return t, reqs
+
+
+def serve_once(S, g, n, t_arr, reqs, batches, slots, coalesce):
+    srv = S.Server(g, slots=slots, coalesce=coalesce)
+    tickets = []
+    t0 = time.perf_counter()
+    for ta, (kind, arg) in zip(t_arr, reqs):
+        delay = t0 + ta - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        if kind in ("insert", "remove"):
+            bs, bd = batches[arg]
+            tickets.append(srv.insert(bs, bd) if kind == "insert" else srv.remove(bs, bd))
+        elif kind == "bfs":
+            tickets.append(srv.bfs(arg))
+        elif kind == "pr":
+            tickets.append(srv.pagerank(arg))
+        else:
+            tickets.append(srv.tc())
+    infos = []
+    for t in tickets:
+        infos.append(srv.wait(t))
+        srv.release(t)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([i["t_done"] - i["t_arrive"] for i in infos])
+    first = min(i["t_arrive"] for i in infos)
+    span = max(i["t_done"] for i in infos) - first
+    per_kind = {}
+    for k in ("insert", "remove", "bfs", "pr", "tc"):
+        sel = np.array([r[0] == k for r in reqs])
+        if sel.any():
+            per_kind[k] = dict(count=int(sel.sum()),
+                               p50_ms=1e3 * float(np.percentile(lat[sel], 50)),
+                               p99_ms=1e3 * float(np.percentile(lat[sel], 99)))
+    assert all(i["status"] == 0 for i in infos)
+    return dict(p50_ms=1e3 * float(np.percentile(lat, 50)),
+                p99_ms=1e3 * float(np.percentile(lat, 99)),
+                mean_ms=1e3 * float(lat.mean()),
+                throughput_rps=len(reqs) / span, makespan_s=span,
+                device_update_batches=st["device_batches"], per_kind=per_kind)
+
+
+def run_serve(args):
+    """Served stream on a device-resident rMat graph (scale 25, seed 4):
+    continuously batched server (4 slots, coalesced updates) against a
+    serialised plain-launch loop (1 slot, no coalescing) on the same
+    trace."""
+    import sstgraph_amd as S
+    scale = args.serve_scale
+    n = 1 << scale
+    count = args.edge_factor * n
+    src, dst = device_symmetric_rmat(S, n, 4, count)
+    g = S.SparseMatrixV(n)
+    g.insert_batch(src, dst)
+    src.free()
+    dst.free()
+    m0 = g.M()
+    deg = g.degrees()
+    t_arr, reqs = serve_trace(n, deg, args.requests, args.rate)
+    batches = {}
+    for kind, arg in reqs:
+        if kind == "insert":
+            size, seed = arg
+            batches[arg] = S.rmat(n, seed, size)
+    log(f"serve: scale {scale}, m={m0}, {len(reqs)} requests at {args.rate}/s")
+    served = serve_once(S, g, n, t_arr, reqs, batches, slots=4, coalesce=True)
+    log(f"served: p50 {served['p50_ms']:.1f} ms p99 {served['p99_ms']:.1f} ms "
+        f"{served['throughput_rps']:.1f} req/s, {served['device_update_batches']} "
+        "device update batches")
+    # the serialised loop starts from the same graph
+    g.close()
+    src, dst = device_symmetric_rmat(S, n, 4, count)
+    g = S.SparseMatrixV(n)
+    g.insert_batch(src, dst)
+    src.free()
+    dst.free()
+    serial = serve_once(S, g, n, t_arr, reqs, batches, slots=1, coalesce=False)
+    log(f"serialised: p50 {serial['p50_ms']:.1f} ms p99 {serial['p99_ms']:.1f} ms "
+        f"{serial['throughput_rps']:.1f} req/s")
+    out = {
+        "metric": "served stream p99 latency (updates + BFS + PR + TC)",
+        "value": served["p99_ms"], "unit": "ms", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 0, "ms_per_step": served["mean_ms"],
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 / f64 (PR)", "data": "synthetic (rMat, seeded trace)",
+        "config": {"workload": f"served stream on rMat scale {scale} (seed 4, edge "
+                               f"factor {args.edge_factor}, symmetrised), Poisson "
+                               f"{args.rate}/s, {len(reqs)} requests",
+                   "directed_edges": int(m0)},
+        "extra": {"served": served, "serialised_plain_loop": serial,
+                  "p99_speedup": serial["p99_ms"] / served["p99_ms"],
+                  "throughput_ratio": served["throughput_rps"] / serial["throughput_rps"]},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,12 +353,22 @@ def main():
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-algos", action="store_true")
+    ap.add_argument("--workload", choices=["updates", "batched", "serve"],
+                    default="updates")
+    ap.add_argument("--graphs", type=int, default=512)
+    ap.add_argument("--serve-scale", type=int, default=25)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--rate", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "batched":
+        return run_batched(args)
+    if args.workload == "serve":
+        return run_serve(args)
     import sstgraph_amd as S
     L = S.lib()
     n = 1 << args.scale

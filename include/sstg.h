@@ -180,6 +180,52 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
               uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
               uint32_t flags);
 
+/* ---- serving: one device-resident graph, a stream of requests ----------
+ * No reference counterpart (the reference has no server).  Requests are
+ * ordered by arrival; a query sees exactly the updates that arrived before
+ * it.  Consecutive updates of one kind are applied as one device batch
+ * (continuous batching, when `coalesce` is set); queries run concurrently
+ * on `slots` slots (own HIP stream and workspace each), admitted in arrival
+ * order while a slot is free and their device-memory estimate fits the HBM
+ * budget (hbm_budget bytes, 0 = 7/8 of the free memory at creation). */
+#define SSTG_REQ_INSERT 1u /* insert_batch (SM:556-643) */
+#define SSTG_REQ_REMOVE 2u /* remove_batch (SM:645-718) */
+#define SSTG_REQ_BFS 3u    /* BFS levels from `source` (TST:486) */
+#define SSTG_REQ_PR 4u     /* `iters` PageRank iterations, FP64 (TST:527) */
+#define SSTG_REQ_TC 5u     /* triangle count (TST:606) */
+#define SSTG_REQ_CC 6u     /* component labels (TST:571) */
+typedef struct sstg_request {
+  uint32_t kind;
+  uint32_t source;                /* BFS */
+  uint32_t iters;                 /* PR */
+  double damping;                 /* PR */
+  const uint32_t *src, *dst;      /* updates: host arrays, copied at submit */
+  uint64_t n;
+} sstg_request;
+typedef struct sstg_request_info {
+  int status;
+  double t_arrive, t_start, t_done; /* host clock, seconds */
+  uint64_t scalar;                  /* TC count */
+  uint64_t coalesced;               /* requests in the same device batch */
+} sstg_request_info;
+typedef struct sstg_server sstg_server;
+int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
+                       int coalesce, sstg_server **out);
+int sstg_server_submit(sstg_server *s, const sstg_request *req,
+                       uint64_t *ticket);
+/* Blocks until the request is done; info may be NULL. */
+int sstg_server_wait(sstg_server *s, uint64_t ticket, sstg_request_info *info);
+/* Copies a finished query's result (BFS levels u32[n], PR f64[n], CC u32[n])
+ * to host memory of `bytes` bytes. */
+int sstg_server_fetch(sstg_server *s, uint64_t ticket, void *host_out,
+                      uint64_t bytes);
+/* Frees a finished request's record and device result. */
+int sstg_server_release(sstg_server *s, uint64_t ticket);
+int sstg_server_stats(sstg_server *s, uint64_t *device_batches,
+                      uint64_t *budget);
+/* Waits for every submitted request, then stops the server. */
+int sstg_server_destroy(sstg_server *s);
+
 /* Seeded Erdos-Renyi edge stream on n vertices (no reference counterpart:
  * the reference's ER tool, graph_utilities.cpp:168-212, is unseeded):
  * edge i = (hash64(seed << 32 | 2i) mod n, hash64(seed << 32 | 2i+1) mod n)

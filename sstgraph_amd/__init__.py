@@ -64,6 +64,22 @@ class UpdateStats(ctypes.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+REQ_INSERT, REQ_REMOVE, REQ_BFS, REQ_PR, REQ_TC, REQ_CC = 1, 2, 3, 4, 5, 6
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("source", ctypes.c_uint32),
+                ("iters", ctypes.c_uint32), ("damping", ctypes.c_double),
+                ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p),
+                ("n", ctypes.c_uint64)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("t_arrive", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("scalar", ctypes.c_uint64), ("coalesced", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -116,6 +132,13 @@ def lib():
         "sstg_multi_graph": (i32, [vp, P(vp)]),
         "sstg_multi_run": (i32, [vp, vp, u32, u32, ctypes.c_double, i32]),
         "sstg_multi_results": (i32, [vp, vp, vp, vp, vp, P(u32)]),
+        "sstg_server_create": (i32, [vp, u32, u64, i32, P(vp)]),
+        "sstg_server_submit": (i32, [vp, P(Request), P(u64)]),
+        "sstg_server_wait": (i32, [vp, u64, P(RequestInfo)]),
+        "sstg_server_fetch": (i32, [vp, u64, vp, u64]),
+        "sstg_server_release": (i32, [vp, u64]),
+        "sstg_server_stats": (i32, [vp, P(u64), P(u64)]),
+        "sstg_server_destroy": (i32, [vp]),
         "sstg_device_alloc": (i32, [u64, P(vp)]),
         "sstg_device_free": (i32, [vp]),
         "sstg_memcpy_h2d": (i32, [vp, vp, u64]),
@@ -524,3 +547,77 @@ def mem_info():
 
 def synchronize():
     _check(lib().sstg_device_synchronize(), "synchronize")
+
+
+class Server:
+    """Serving loop over one device-resident graph (sstg_server_*): updates
+    are applied in arrival order, consecutive same-kind updates as one device
+    batch when coalesce is set; queries run concurrently on `slots` slots
+    under an HBM budget."""
+
+    def __init__(self, g, slots=4, hbm_budget=0, coalesce=True):
+        self.g = g
+        h = ctypes.c_void_p()
+        _check(lib().sstg_server_create(g.handle, slots, hbm_budget,
+                                        1 if coalesce else 0, ctypes.byref(h)),
+               "sstg_server_create")
+        self._h = h
+        self._keep = {}
+
+    def _submit(self, req, keep=None):
+        t = ctypes.c_uint64(0)
+        _check(lib().sstg_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)),
+               "submit")
+        return t.value
+
+    def insert(self, src, dst):
+        s, d = _u32(src), _u32(dst)
+        return self._submit(Request(REQ_INSERT, 0, 0, 0.0, s.ctypes.data,
+                                    d.ctypes.data, s.size))
+
+    def remove(self, src, dst):
+        s, d = _u32(src), _u32(dst)
+        return self._submit(Request(REQ_REMOVE, 0, 0, 0.0, s.ctypes.data,
+                                    d.ctypes.data, s.size))
+
+    def bfs(self, source):
+        return self._submit(Request(REQ_BFS, source, 0, 0.0, None, None, 0))
+
+    def pagerank(self, iters=10, damping=0.85):
+        return self._submit(Request(REQ_PR, 0, iters, damping, None, None, 0))
+
+    def tc(self):
+        return self._submit(Request(REQ_TC, 0, 0, 0.0, None, None, 0))
+
+    def cc(self):
+        return self._submit(Request(REQ_CC, 0, 0, 0.0, None, None, 0))
+
+    def wait(self, ticket):
+        info = RequestInfo()
+        _check(lib().sstg_server_wait(self._h, ticket, ctypes.byref(info)), "wait")
+        return {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+
+    def fetch(self, ticket, dtype):
+        out = np.empty(self.g.n_rows, dtype=dtype)
+        _check(lib().sstg_server_fetch(self._h, ticket, _ptr(out), out.nbytes), "fetch")
+        return out
+
+    def release(self, ticket):
+        _check(lib().sstg_server_release(self._h, ticket), "release")
+
+    def stats(self):
+        b, bud = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(lib().sstg_server_stats(self._h, ctypes.byref(b), ctypes.byref(bud)),
+               "stats")
+        return {"device_batches": b.value, "budget": bud.value}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _check(lib().sstg_server_destroy(self._h), "server destroy")
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,486 @@
+// Serving loop over one device-resident graph: continuous batching of
+// update requests, concurrent queries, admission by free HBM.
+//
+// No reference counterpart: the reference has no server.  Its update loop
+// (include/SSTGraph/internal/test.hpp:636-690) applies one batch at a time
+// and its algorithm calls (test.hpp:476-618) run one at a time.
+//
+// Semantics.  Requests are ordered by arrival.  A query sees every update
+// that arrived before it and none that arrived after it, so its result is
+// the one it would get alone on the graph at that point of the stream.
+//  * Updates are writers: an update waits until all earlier queries are
+//    done.  All updates of the same kind that are waiting at the head of the
+//    queue are applied as ONE device batch (continuous batching): a run of
+//    inserts is their union, a run of removes their difference, exactly as
+//    applying them one by one.
+//  * Queries (BFS, PR, CC, TC) are readers: each runs on a free slot (its
+//    own HIP stream, workspace and read-only view of the store), so queries
+//    that arrive while others are on the GPU overlap with them.  They are
+//    admitted in arrival order while a slot is free and their device memory
+//    estimate fits the HBM budget; otherwise they wait.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "algo.hpp"
+#include "graph.hpp"
+
+namespace sstg {
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Request {
+  uint64_t ticket = 0;
+  sstg_request spec{};
+  std::vector<uint32_t> src, dst;  // update payload (host copy)
+  double t_arrive = 0, t_start = 0, t_done = 0;
+  uint64_t est_bytes = 0;
+  uint64_t coalesced = 1;  // requests applied in the same device batch
+  int status = SSTG_OK;
+  bool done = false;
+  void *result = nullptr;  // device result (queries)
+  uint64_t result_bytes = 0;
+  uint64_t scalar = 0;     // TC count
+};
+
+// A read-only view of the shared store with its own stream and workspace.
+struct Slot {
+  sstg_graph view;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  Request *req = nullptr;
+  bool stop = false;
+};
+
+bool is_update(uint32_t kind) {
+  return kind == SSTG_REQ_INSERT || kind == SSTG_REQ_REMOVE;
+}
+
+}  // namespace
+}  // namespace sstg
+
+struct sstg_server {
+  sstg_graph *g = nullptr;
+  std::vector<sstg::Slot *> slots;
+  uint64_t budget = 0, reserved = 0;
+  int coalesce = 1;
+  std::deque<sstg::Request *> pending;
+  std::map<uint64_t, sstg::Request *> all;
+  std::mutex mu;
+  std::condition_variable cv;       // scheduler wake-up
+  std::condition_variable done_cv;  // request completion
+  std::thread sched;
+  bool stop = false;
+  uint64_t next_ticket = 1;
+  uint64_t running = 0;  // queries on slots
+  uint64_t device_batches = 0;
+};
+
+namespace sstg {
+namespace {
+
+// Device bytes a query needs beyond the shared store (result + scratch).
+uint64_t estimate(const sstg_graph *g, const sstg_request &r) {
+  const uint64_t n = g->n_rows;
+  switch (r.kind) {
+    case SSTG_REQ_BFS:
+      return n * 4 + n * 24;            // levels + queue, flags, bitmaps
+    case SSTG_REQ_PR:
+      return n * 8 + n * 16 + n * 20;   // ranks + contribs + bins
+    case SSTG_REQ_CC:
+      return n * 8;
+    case SSTG_REQ_TC:
+      return n * 4;
+    default:
+      return 0;
+  }
+}
+
+void refresh_view(sstg_graph *v, const sstg_graph *g) {
+  v->n_rows = g->n_rows;
+  v->n_cols = g->n_cols;
+  v->m = g->m;
+  v->off = g->off;
+  v->deg = g->deg;
+  v->cap = g->cap;
+  v->pool = g->pool;
+  v->pool_cap = g->pool_cap;
+  v->pool_tail = g->pool_tail;
+  if (v->version != g->version) {
+    v->version = g->version;  // derived caches of the view are recomputed
+  }
+  v->sym_version = g->sym_version == g->version ? v->version : ~0ull;
+  v->symmetric = g->symmetric;
+}
+
+int run_query(sstg_graph *v, Request *r) {
+  const uint64_t n = v->n_rows;
+  switch (r->spec.kind) {
+    case SSTG_REQ_BFS: {
+      uint32_t *lev = nullptr;
+      SSTG_HIP(hipMalloc((void **)&lev, n * 4));
+      r->result = lev;
+      r->result_bytes = n * 4;
+      if (r->spec.source >= n) return SSTG_E_RANGE;
+      return algo_bfs(v, r->spec.source, nullptr, lev);
+    }
+    case SSTG_REQ_PR: {
+      double *rank = nullptr;
+      SSTG_HIP(hipMalloc((void **)&rank, n * 8));
+      r->result = rank;
+      r->result_bytes = n * 8;
+      return algo_pagerank(v, r->spec.iters, r->spec.damping, rank);
+    }
+    case SSTG_REQ_CC: {
+      uint32_t *lab = nullptr;
+      SSTG_HIP(hipMalloc((void **)&lab, n * 4));
+      r->result = lab;
+      r->result_bytes = n * 4;
+      return algo_cc(v, lab);
+    }
+    case SSTG_REQ_TC:
+      return algo_tc(v, &r->scalar);
+    default:
+      return SSTG_E_INVAL;
+  }
+}
+
+void slot_main(sstg_server *S, Slot *sl) {
+  hipSetDevice(S->g->device);
+  for (;;) {
+    Request *r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(sl->mu);
+      sl->cv.wait(lk, [&] { return sl->stop || sl->req; });
+      if (sl->stop && !sl->req) return;
+      r = sl->req;
+    }
+    int rc = run_query(&sl->view, r);
+    if (rc == SSTG_OK) {
+      hipError_t e = hipStreamSynchronize(sl->view.stream);
+      if (e != hipSuccess) rc = SSTG_E_DEVICE;
+    }
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      r->status = rc;
+      r->t_done = now_s();
+      r->done = true;
+      S->reserved -= r->est_bytes;
+      S->running--;
+      {
+        std::lock_guard<std::mutex> lk2(sl->mu);
+        sl->req = nullptr;
+      }
+    }
+    S->cv.notify_all();
+    S->done_cv.notify_all();
+  }
+}
+
+Slot *free_slot(sstg_server *S) {
+  for (Slot *sl : S->slots) {
+    std::lock_guard<std::mutex> lk(sl->mu);
+    if (!sl->req) return sl;
+  }
+  return nullptr;
+}
+
+// Applies the update run at the head of the queue (caller holds S->mu; the
+// lock is released while the device works).
+void apply_updates(sstg_server *S, std::unique_lock<std::mutex> &lk) {
+  std::vector<Request *> run;
+  const uint32_t kind = S->pending.front()->spec.kind;
+  while (!S->pending.empty() && S->pending.front()->spec.kind == kind &&
+         (S->coalesce || run.empty())) {
+    run.push_back(S->pending.front());
+    S->pending.pop_front();
+  }
+  uint64_t total = 0;
+  for (Request *r : run) total += r->src.size();
+  std::vector<uint32_t> src, dst;
+  src.reserve(total);
+  dst.reserve(total);
+  for (Request *r : run) {
+    src.insert(src.end(), r->src.begin(), r->src.end());
+    dst.insert(dst.end(), r->dst.begin(), r->dst.end());
+  }
+  const double t0 = now_s();
+  for (Request *r : run) r->t_start = t0;
+  lk.unlock();
+  hipSetDevice(S->g->device);
+  int rc = graph_update(S->g, src.data(), dst.data(), total,
+                        kind == SSTG_REQ_INSERT,
+                        kind == SSTG_REQ_INSERT, false);
+  lk.lock();
+  const double t1 = now_s();
+  if (rc == SSTG_E_RANGE && run.size() > 1) {
+    // one request holds a bad id: apply them one by one so only it fails
+    lk.unlock();
+    std::vector<int> rcs;
+    for (Request *r : run)
+      rcs.push_back(graph_update(S->g, r->src.data(), r->dst.data(),
+                                 r->src.size(), kind == SSTG_REQ_INSERT,
+                                 kind == SSTG_REQ_INSERT, false));
+    lk.lock();
+    for (size_t i = 0; i < run.size(); i++) run[i]->status = rcs[i];
+  } else {
+    for (Request *r : run) r->status = rc;
+  }
+  for (Request *r : run) {
+    r->t_done = t1;
+    r->done = true;
+    r->coalesced = run.size();
+    r->src.clear();
+    r->src.shrink_to_fit();
+    r->dst.clear();
+    r->dst.shrink_to_fit();
+  }
+  S->device_batches++;
+  S->done_cv.notify_all();
+}
+
+void sched_main(sstg_server *S) {
+  hipSetDevice(S->g->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    S->cv.wait(lk, [&] {
+      if (S->stop) return true;
+      if (S->pending.empty()) return false;
+      Request *h = S->pending.front();
+      if (is_update(h->spec.kind)) return S->running == 0;  // writer waits
+      return S->reserved + h->est_bytes <= S->budget && free_slot(S);
+    });
+    if (S->stop) return;
+    Request *h = S->pending.front();
+    if (is_update(h->spec.kind)) {
+      apply_updates(S, lk);
+      continue;
+    }
+    // admit queries in arrival order while slots and HBM allow
+    while (!S->pending.empty() && !is_update(S->pending.front()->spec.kind)) {
+      Request *q = S->pending.front();
+      if (S->reserved + q->est_bytes > S->budget) break;
+      Slot *sl = free_slot(S);
+      if (!sl) break;
+      S->pending.pop_front();
+      S->reserved += q->est_bytes;
+      S->running++;
+      q->t_start = now_s();
+      refresh_view(&sl->view, S->g);
+      {
+        std::lock_guard<std::mutex> lk2(sl->mu);
+        sl->req = q;
+      }
+      sl->cv.notify_one();
+    }
+  }
+}
+
+int view_init(sstg_graph *v, const sstg_graph *g) {
+  v->device = g->device;
+  SSTG_HIP(hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking));
+  v->ws.stream = v->stream;
+  SSTG_HIP(hipMalloc(&v->dev_ctr, 16 * sizeof(uint64_t)));
+  SSTG_HIP(hipHostMalloc(&v->host_rb, 64 * sizeof(uint64_t),
+                         hipHostMallocDefault));
+  v->version = ~0ull;
+  refresh_view(v, g);
+  return SSTG_OK;
+}
+
+void view_free(sstg_graph *v) {
+  if (v->stream) hipStreamSynchronize(v->stream);
+  v->ws.release_all();
+  hipFree(v->dev_ctr);
+  hipFree(v->deg_prefix);
+  if (v->host_rb) hipHostFree(v->host_rb);
+  if (v->stream) hipStreamDestroy(v->stream);
+  // the store arrays belong to the served graph
+  v->off = nullptr;
+  v->deg = v->cap = v->pool = nullptr;
+}
+
+}  // namespace
+}  // namespace sstg
+
+using namespace sstg;
+
+extern "C" {
+
+int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
+                       int coalesce, sstg_server **out) {
+  if (!g || !out || slots == 0 || slots > 64) return SSTG_E_INVAL;
+  *out = nullptr;
+  hipSetDevice(g->device);
+  bool sym = false;
+  SSTG_TRY(graph_is_symmetric(g, &sym));  // known once, then tracked
+  sstg_server *S = new (std::nothrow) sstg_server();
+  if (!S) return SSTG_E_NOMEM;
+  S->g = g;
+  S->coalesce = coalesce;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    delete S;
+    return SSTG_E_DEVICE;
+  }
+  // keep 1/8 of the free memory for update scratch and pool growth
+  uint64_t avail = free_b - free_b / 8;
+  S->budget = hbm_budget ? (hbm_budget < avail ? hbm_budget : avail) : avail;
+  for (uint32_t i = 0; i < slots; i++) {
+    Slot *sl = new Slot();
+    int rc = view_init(&sl->view, g);
+    if (rc) {
+      view_free(&sl->view);
+      delete sl;
+      sstg_server_destroy(S);
+      return rc;
+    }
+    S->slots.push_back(sl);
+  }
+  for (Slot *sl : S->slots) sl->th = std::thread(slot_main, S, sl);
+  S->sched = std::thread(sched_main, S);
+  *out = S;
+  return SSTG_OK;
+}
+
+int sstg_server_submit(sstg_server *S, const sstg_request *req,
+                       uint64_t *ticket) {
+  if (!S || !req || !ticket) return SSTG_E_INVAL;
+  if (req->kind < SSTG_REQ_INSERT || req->kind > SSTG_REQ_CC)
+    return SSTG_E_INVAL;
+  Request *r = new (std::nothrow) Request();
+  if (!r) return SSTG_E_NOMEM;
+  r->spec = *req;
+  r->t_arrive = now_s();
+  if (is_update(req->kind)) {
+    if (req->n && (!req->src || !req->dst)) {
+      delete r;
+      return SSTG_E_INVAL;
+    }
+    r->src.assign(req->src, req->src + req->n);
+    r->dst.assign(req->dst, req->dst + req->n);
+  } else {
+    r->est_bytes = estimate(S->g, *req);
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (r->est_bytes > S->budget) {
+    delete r;
+    return SSTG_E_NOMEM;  // could never be admitted
+  }
+  r->ticket = S->next_ticket++;
+  S->all[r->ticket] = r;
+  S->pending.push_back(r);
+  *ticket = r->ticket;
+  S->cv.notify_all();
+  return SSTG_OK;
+}
+
+int sstg_server_wait(sstg_server *S, uint64_t ticket, sstg_request_info *info) {
+  if (!S) return SSTG_E_INVAL;
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->all.find(ticket);
+  if (it == S->all.end()) return SSTG_E_INVAL;
+  Request *r = it->second;
+  S->done_cv.wait(lk, [&] { return r->done; });
+  if (info) {
+    info->status = r->status;
+    info->t_arrive = r->t_arrive;
+    info->t_start = r->t_start;
+    info->t_done = r->t_done;
+    info->scalar = r->scalar;
+    info->coalesced = r->coalesced;
+  }
+  return SSTG_OK;
+}
+
+int sstg_server_fetch(sstg_server *S, uint64_t ticket, void *host_out,
+                      uint64_t bytes) {
+  if (!S) return SSTG_E_INVAL;
+  Request *r = nullptr;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(ticket);
+    if (it == S->all.end()) return SSTG_E_INVAL;
+    r = it->second;
+    S->done_cv.wait(lk, [&] { return r->done; });
+  }
+  if (!r->result || !host_out) return SSTG_E_INVAL;
+  if (bytes < r->result_bytes) return SSTG_E_INVAL;
+  hipSetDevice(S->g->device);
+  SSTG_HIP(hipMemcpy(host_out, r->result, r->result_bytes,
+                     hipMemcpyDeviceToHost));
+  return SSTG_OK;
+}
+
+int sstg_server_release(sstg_server *S, uint64_t ticket) {
+  if (!S) return SSTG_E_INVAL;
+  Request *r = nullptr;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(ticket);
+    if (it == S->all.end()) return SSTG_E_INVAL;
+    r = it->second;
+    S->done_cv.wait(lk, [&] { return r->done; });
+    S->all.erase(it);
+  }
+  if (r->result) {
+    hipSetDevice(S->g->device);
+    hipFree(r->result);
+  }
+  delete r;
+  return SSTG_OK;
+}
+
+int sstg_server_stats(sstg_server *S, uint64_t *device_batches,
+                      uint64_t *budget) {
+  if (!S) return SSTG_E_INVAL;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (device_batches) *device_batches = S->device_batches;
+  if (budget) *budget = S->budget;
+  return SSTG_OK;
+}
+
+int sstg_server_destroy(sstg_server *S) {
+  if (!S) return SSTG_E_INVAL;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    // drain: wait for every admitted or pending request
+    S->done_cv.wait(lk, [&] {
+      for (auto &kv : S->all)
+        if (!kv.second->done) return false;
+      return true;
+    });
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  if (S->sched.joinable()) S->sched.join();
+  for (Slot *sl : S->slots) {
+    {
+      std::lock_guard<std::mutex> lk(sl->mu);
+      sl->stop = true;
+    }
+    sl->cv.notify_one();
+    if (sl->th.joinable()) sl->th.join();
+    view_free(&sl->view);
+    delete sl;
+  }
+  hipSetDevice(S->g->device);
+  for (auto &kv : S->all) {
+    if (kv.second->result) hipFree(kv.second->result);
+    delete kv.second;
+  }
+  delete S;
+  return SSTG_OK;
+}
+
+}  // extern "C"
