@@ -342,6 +342,55 @@ def run_serve(args):
     print(json.dumps(out))
 
 
+def run_algo(args):
+    """One algorithm alone on the scale-`--scale` rMat graph (seed 2), outputs
+    kept in HBM: `pagerank` runs PR_S(g, --pr-iters) per step, `bfs` runs BFS
+    from the largest hub per step.  Used for per-kernel profiles."""
+    import sstgraph_amd as S
+    EMV = S.EdgeMapVertexMap
+    n = 1 << args.scale
+    count = args.edge_factor * n
+    src, dst = device_symmetric_rmat(S, n, 2, count)
+    g = S.SparseMatrixV(n)
+    g.insert_batch(src, dst)
+    src.free()
+    dst.free()
+    m = g.M()
+    deg = g.degrees().astype(np.int64)
+    if args.workload == "pagerank":
+        out = S.DeviceArray(n, np.float64)
+        step = lambda: EMV.PR_S(g, args.pr_iters, out=out)
+        work = m * args.pr_iters
+        name = f"PageRank FP64 x{args.pr_iters}"
+    else:
+        hub = int(np.argmax(deg))
+        d_par = S.DeviceArray(n, np.int32)
+        d_lev = S.DeviceArray(n, np.uint32)
+        step = lambda: EMV.BFS(g, hub, out=(d_par, d_lev))
+        step()
+        lev = d_lev.to_host()
+        work = int(deg[lev != S.UNREACHED].sum())
+        name = f"BFS from vertex {hub}"
+    for _ in range(args.warmup):
+        step()
+    S.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    S.synchronize()
+    t = (time.perf_counter() - t0) / max(args.steps, 1)
+    log(f"{name}: {1e3 * t:.3f} ms per step, {work / t / 1e9:.2f} GTEPS")
+    print(json.dumps({
+        "metric": f"{name} traversed edges/s", "value": work / t, "unit": "edges/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.workload == "pagerank" else "u32",
+        "data": "synthetic (rMat generator of the reference, seeded)",
+        "config": {"workload": f"{name} on rMat scale {args.scale} (seed 2, edge "
+                               f"factor {args.edge_factor}, symmetrised)",
+                   "directed_edges": int(m)}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -353,7 +402,7 @@ def main():
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-algos", action="store_true")
-    ap.add_argument("--workload", choices=["updates", "batched", "serve"],
+    ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs"],
                     default="updates")
     ap.add_argument("--graphs", type=int, default=512)
     ap.add_argument("--serve-scale", type=int, default=25)
@@ -369,6 +418,8 @@ def main():
         return run_batched(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload in ("pagerank", "bfs"):
+        return run_algo(args)
     import sstgraph_amd as S
     L = S.lib()
     n = 1 << args.scale

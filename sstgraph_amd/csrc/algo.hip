@@ -537,6 +537,10 @@ int algo_tc(sstg_graph *g, uint64_t *out) {
   const uint32_t n = g->n_rows;
   *out = 0;
   if (n == 0 || g->m == 0) return SSTG_OK;
+  bool symmetric = false;
+  SSTG_TRY(graph_is_symmetric(g, &symmetric));
+  if (symmetric) return tc_oriented(g, out);
+  // directed rows: per stored edge, as the reference counts
   const uint64_t *prefix = nullptr;
   SSTG_TRY(graph_degree_prefix(g, &prefix));
   uint32_t *lower = nullptr;

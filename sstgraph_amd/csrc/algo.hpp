@@ -13,6 +13,8 @@ int algo_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,
 int algo_bc(sstg_graph *g, uint32_t src, double *out);
 int algo_cc(sstg_graph *g, uint32_t *out);
 int algo_tc(sstg_graph *g, uint64_t *out);
+// degree-oriented TC; valid only when every edge has its reverse (tc.hip)
+int tc_oriented(sstg_graph *g, uint64_t *out);
 int algo_bf(sstg_graph *g, uint32_t src, int32_t *out);
 int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
                           int early_end, uint64_t *out);

@@ -97,6 +97,10 @@ enum : int {
   kWsTileBst = 92,
   kWsTileBlk = 93,
   kWsMulti = 94,  // 94..97
+  kWsTcDag = 98,
+  kWsTcDplus = 99,
+  kWsTcList = 100,
+  kWsTcAcc = 101,
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range

@@ -100,7 +100,7 @@ uint64_t estimate(const sstg_graph *g, const sstg_request &r) {
     case SSTG_REQ_CC:
       return n * 8;
     case SSTG_REQ_TC:
-      return n * 4;
+      return g->m * 4 + n * 17;         // oriented lists + out-degrees, lists
     default:
       return 0;
   }
