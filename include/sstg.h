@@ -184,10 +184,15 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
  * No reference counterpart (the reference has no server).  Requests are
  * ordered by arrival; a query sees exactly the updates that arrived before
  * it.  Consecutive updates of one kind are applied as one device batch
- * (continuous batching, when `coalesce` is set); queries run concurrently
+ * (continuous batching, flag SSTG_SERVE_COALESCE); queries run concurrently
  * on `slots` slots (own HIP stream and workspace each), admitted in arrival
  * order while a slot is free and their device-memory estimate fits the HBM
- * budget (hbm_budget bytes, 0 = 7/8 of the free memory at creation). */
+ * budget (hbm_budget bytes, 0 = 7/8 of the free memory at creation).  With
+ * SSTG_SERVE_SNAPSHOTS an update does not wait for running queries: the
+ * graph moves to a device copy of its store and the queries finish on the
+ * old arrays (the copy is held against the budget while they run). */
+#define SSTG_SERVE_COALESCE 1u
+#define SSTG_SERVE_SNAPSHOTS 2u
 #define SSTG_REQ_INSERT 1u /* insert_batch (SM:556-643) */
 #define SSTG_REQ_REMOVE 2u /* remove_batch (SM:645-718) */
 #define SSTG_REQ_BFS 3u    /* BFS levels from `source` (TST:486) */
@@ -210,7 +215,7 @@ typedef struct sstg_request_info {
 } sstg_request_info;
 typedef struct sstg_server sstg_server;
 int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
-                       int coalesce, sstg_server **out);
+                       uint32_t flags, sstg_server **out);
 int sstg_server_submit(sstg_server *s, const sstg_request *req,
                        uint64_t *ticket);
 /* Blocks until the request is done; info may be NULL. */
@@ -221,8 +226,9 @@ int sstg_server_fetch(sstg_server *s, uint64_t ticket, void *host_out,
                       uint64_t bytes);
 /* Frees a finished request's record and device result. */
 int sstg_server_release(sstg_server *s, uint64_t ticket);
+/* device update batches applied, the HBM budget, store copies made */
 int sstg_server_stats(sstg_server *s, uint64_t *device_batches,
-                      uint64_t *budget);
+                      uint64_t *budget, uint64_t *snapshots);
 /* Waits for every submitted request, then stops the server. */
 int sstg_server_destroy(sstg_server *s);
 

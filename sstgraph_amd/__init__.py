@@ -34,6 +34,8 @@ SSTG_E_NODEVICE = -5
 SSTG_E_UNSUPPORTED = -6
 
 SSTG_DEVICE_PTRS = 0x1
+SSTG_SERVE_COALESCE = 0x1
+SSTG_SERVE_SNAPSHOTS = 0x2
 SSTG_SEMI_SORTED = 0x2
 SSTG_KEEP_SELF_LOOPS = 0x4
 
@@ -132,12 +134,12 @@ def lib():
         "sstg_multi_graph": (i32, [vp, P(vp)]),
         "sstg_multi_run": (i32, [vp, vp, u32, u32, ctypes.c_double, i32]),
         "sstg_multi_results": (i32, [vp, vp, vp, vp, vp, P(u32)]),
-        "sstg_server_create": (i32, [vp, u32, u64, i32, P(vp)]),
+        "sstg_server_create": (i32, [vp, u32, u64, u32, P(vp)]),
         "sstg_server_submit": (i32, [vp, P(Request), P(u64)]),
         "sstg_server_wait": (i32, [vp, u64, P(RequestInfo)]),
         "sstg_server_fetch": (i32, [vp, u64, vp, u64]),
         "sstg_server_release": (i32, [vp, u64]),
-        "sstg_server_stats": (i32, [vp, P(u64), P(u64)]),
+        "sstg_server_stats": (i32, [vp, P(u64), P(u64), P(u64)]),
         "sstg_server_destroy": (i32, [vp]),
         "sstg_device_alloc": (i32, [u64, P(vp)]),
         "sstg_device_free": (i32, [vp]),
@@ -553,13 +555,16 @@ class Server:
     """Serving loop over one device-resident graph (sstg_server_*): updates
     are applied in arrival order, consecutive same-kind updates as one device
     batch when coalesce is set; queries run concurrently on `slots` slots
-    under an HBM budget."""
+    under an HBM budget; with snapshots, updates proceed on a copy of the
+    store while earlier queries finish on the old one."""
 
-    def __init__(self, g, slots=4, hbm_budget=0, coalesce=True):
+    def __init__(self, g, slots=4, hbm_budget=0, coalesce=True, snapshots=False):
         self.g = g
         h = ctypes.c_void_p()
-        _check(lib().sstg_server_create(g.handle, slots, hbm_budget,
-                                        1 if coalesce else 0, ctypes.byref(h)),
+        flags = (SSTG_SERVE_COALESCE if coalesce else 0) | \
+            (SSTG_SERVE_SNAPSHOTS if snapshots else 0)
+        _check(lib().sstg_server_create(g.handle, slots, hbm_budget, flags,
+                                        ctypes.byref(h)),
                "sstg_server_create")
         self._h = h
         self._keep = {}
@@ -606,10 +611,10 @@ class Server:
         _check(lib().sstg_server_release(self._h, ticket), "release")
 
     def stats(self):
-        b, bud = ctypes.c_uint64(0), ctypes.c_uint64(0)
-        _check(lib().sstg_server_stats(self._h, ctypes.byref(b), ctypes.byref(bud)),
-               "stats")
-        return {"device_batches": b.value, "budget": bud.value}
+        b, bud, sn = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(lib().sstg_server_stats(self._h, ctypes.byref(b), ctypes.byref(bud),
+                                       ctypes.byref(sn)), "stats")
+        return {"device_batches": b.value, "budget": bud.value, "snapshots": sn.value}
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

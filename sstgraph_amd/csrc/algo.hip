@@ -539,13 +539,21 @@ int algo_tc(sstg_graph *g, uint64_t *out) {
   if (n == 0 || g->m == 0) return SSTG_OK;
   bool symmetric = false;
   SSTG_TRY(graph_is_symmetric(g, &symmetric));
-  if (symmetric) return tc_oriented(g, out);
-  // directed rows: per stored edge, as the reference counts
-  const uint64_t *prefix = nullptr;
-  SSTG_TRY(graph_degree_prefix(g, &prefix));
+  if (symmetric) {
+    int rc = tc_oriented(g, out);
+    if (rc != SSTG_E_UNSUPPORTED) return rc;
+  }
+  // rows as stored: L(x) = ids of row x below x, a prefix of the row
   uint32_t *lower = nullptr;
   SSTG_TRY(g->ws.get(kWsAlgo + 23, n, &lower));
   SSTG_LAUNCH(k_lower_degree, blocks(n), kT, 0, s, g->view(), lower);
+  if (!symmetric) {
+    int rc = count_list_triangles(g, g->off, g->pool, lower, out);
+    if (rc != SSTG_E_UNSUPPORTED) return rc;
+  }
+  // lists of 2^23 ids or more: one thread per stored edge
+  const uint64_t *prefix = nullptr;
+  SSTG_TRY(graph_degree_prefix(g, &prefix));
   unsigned long long *acc = (unsigned long long *)(g->dev_ctr + 8);
   SSTG_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), s));
   SSTG_LAUNCH(k_tc_edges, ceil_div(ceil_div(g->m, 4), kT), kT, 0, s, g->view(),

@@ -15,6 +15,11 @@ int algo_cc(sstg_graph *g, uint32_t *out);
 int algo_tc(sstg_graph *g, uint64_t *out);
 // degree-oriented TC; valid only when every edge has its reverse (tc.hip)
 int tc_oriented(sstg_graph *g, uint64_t *out);
+// sum over u, v in L(u) of |L(u) n L(v)|, L(x) = lists[base[x] .. +len[x])
+// sorted unique ids; SSTG_E_UNSUPPORTED when a list has 2^23 ids or more
+int count_list_triangles(sstg_graph *g, const uint64_t *base,
+                         const uint32_t *lists, const uint32_t *len,
+                         uint64_t *out);
 int algo_bf(sstg_graph *g, uint32_t src, int32_t *out);
 int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
                           int early_end, uint64_t *out);

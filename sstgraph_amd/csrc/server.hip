@@ -8,11 +8,17 @@
 // Semantics.  Requests are ordered by arrival.  A query sees every update
 // that arrived before it and none that arrived after it, so its result is
 // the one it would get alone on the graph at that point of the stream.
-//  * Updates are writers: an update waits until all earlier queries are
-//    done.  All updates of the same kind that are waiting at the head of the
-//    queue are applied as ONE device batch (continuous batching): a run of
-//    inserts is their union, a run of removes their difference, exactly as
-//    applying them one by one.
+//  * Updates are writers.  All updates of the same kind that are waiting at
+//    the head of the queue are applied as ONE device batch (continuous
+//    batching, SSTG_SERVE_COALESCE): a run of inserts is their union, a run
+//    of removes their difference, exactly as applying them one by one.
+//  * Without SSTG_SERVE_SNAPSHOTS an update waits until the earlier queries
+//    are done.  With it, an update that finds queries still reading the
+//    store first copies the store (device to device, about 2 x 4 bytes per
+//    stored edge of traffic) and applies the batch to the copy; the running
+//    queries keep the old arrays as an immutable snapshot, which returns to
+//    a spare-buffer cache when its last reader finishes.  The copy counts
+//    against the HBM budget while the snapshot lives.
 //  * Queries (BFS, PR, CC, TC) are readers: each runs on a free slot (its
 //    own HIP stream, workspace and read-only view of the store), so queries
 //    that arrive while others are on the GPU overlap with them.  They are
@@ -38,6 +44,18 @@ double now_s() {
       .count();
 }
 
+// One set of store arrays that queries read.  The live set aliases the
+// graph's arrays; a retired set owns the arrays it holds (the graph moved
+// on to a copy) and is recycled when its last reader is done.
+struct StoreSet {
+  uint64_t *off = nullptr;
+  uint32_t *deg = nullptr, *cap = nullptr, *pool = nullptr;
+  uint64_t pool_cap = 0;  // elements
+  uint32_t readers = 0;
+  bool owned = false;
+  uint64_t bytes = 0;     // device bytes held while owned
+};
+
 struct Request {
   uint64_t ticket = 0;
   sstg_request spec{};
@@ -49,6 +67,7 @@ struct Request {
   bool done = false;
   void *result = nullptr;  // device result (queries)
   uint64_t result_bytes = 0;
+  StoreSet *set = nullptr; // store arrays the query reads
   uint64_t scalar = 0;     // TC count
 };
 
@@ -74,6 +93,11 @@ struct sstg_server {
   std::vector<sstg::Slot *> slots;
   uint64_t budget = 0, reserved = 0;
   int coalesce = 1;
+  int snapshots = 0;
+  sstg::StoreSet *live = nullptr;        // set the next admitted query reads
+  std::vector<sstg::StoreSet> spare;     // recycled store arrays
+  std::multimap<uint64_t, void *> spare_results;  // bytes -> device buffer
+  uint64_t detaches = 0;
   std::deque<sstg::Request *> pending;
   std::map<uint64_t, sstg::Request *> all;
   std::mutex mu;
@@ -106,6 +130,115 @@ uint64_t estimate(const sstg_graph *g, const sstg_request &r) {
   }
 }
 
+uint64_t store_bytes(uint32_t n, uint64_t pool_cap) {
+  return (uint64_t)n * 16 + pool_cap * 4;
+}
+
+void free_set_arrays(StoreSet &t) {
+  hipFree(t.off);
+  hipFree(t.deg);
+  hipFree(t.cap);
+  hipFree(t.pool);
+}
+
+// Result buffers are recycled by size: hipFree would wait for the whole
+// device, including the other slots' queries.
+int result_alloc(sstg_server *S, uint64_t bytes, void **out) {
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->spare_results.find(bytes);
+    if (it != S->spare_results.end()) {
+      *out = it->second;
+      S->spare_results.erase(it);
+      return SSTG_OK;
+    }
+  }
+  SSTG_HIP(hipMalloc(out, bytes));
+  return SSTG_OK;
+}
+
+// A finished query drops its read of a store set (caller holds S->mu).  A
+// retired set without readers becomes a spare (its bytes stay reserved);
+// spares beyond one per slot plus one are freed.
+void drop_reader(sstg_server *S, StoreSet *t) {
+  if (!t) return;
+  t->readers--;
+  if (t->owned && t->readers == 0) {
+    S->spare.push_back(*t);
+    delete t;
+    while (S->spare.size() > S->slots.size() + 1) {
+      S->reserved -= S->spare.front().bytes;
+      free_set_arrays(S->spare.front());
+      S->spare.erase(S->spare.begin());
+    }
+  }
+}
+
+// index of a spare set large enough for the graph's store, or -1
+int find_spare(const sstg_server *S) {
+  for (size_t i = 0; i < S->spare.size(); i++)
+    if (S->spare[i].pool_cap >= S->g->pool_cap) return (int)i;
+  return -1;
+}
+
+// Gives the graph a private copy of its store so that the queries reading
+// the current arrays can keep them (caller holds S->mu; the copy is queued
+// on the graph's stream ahead of the update that follows).
+int detach_store(sstg_server *S) {
+  sstg_graph *g = S->g;
+  StoreSet *t = S->live;
+  const uint32_t n = g->n_rows;
+  StoreSet b;
+  const int k = find_spare(S);
+  if (k >= 0) {
+    b = S->spare[k];
+    S->spare.erase(S->spare.begin() + k);
+    S->reserved -= b.bytes;  // the spare becomes the graph's store
+  } else {
+    b.pool_cap = g->pool_cap;
+    if (hipMalloc(&b.off, (uint64_t)n * 8) != hipSuccess ||
+        hipMalloc(&b.deg, (uint64_t)n * 4) != hipSuccess ||
+        hipMalloc(&b.cap, (uint64_t)n * 4) != hipSuccess ||
+        hipMalloc(&b.pool, b.pool_cap * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      free_set_arrays(b);
+      return SSTG_E_NOMEM;
+    }
+  }
+  hipStream_t s = g->stream;
+  SSTG_HIP(hipMemcpyAsync(b.off, g->off, (uint64_t)n * 8,
+                          hipMemcpyDeviceToDevice, s));
+  SSTG_HIP(hipMemcpyAsync(b.deg, g->deg, (uint64_t)n * 4,
+                          hipMemcpyDeviceToDevice, s));
+  SSTG_HIP(hipMemcpyAsync(b.cap, g->cap, (uint64_t)n * 4,
+                          hipMemcpyDeviceToDevice, s));
+  if (g->pool_tail)
+    SSTG_HIP(hipMemcpyAsync(b.pool, g->pool, g->pool_tail * 4,
+                            hipMemcpyDeviceToDevice, s));
+  // the old arrays now belong to the snapshot its readers hold
+  t->owned = true;
+  t->bytes = store_bytes(n, t->pool_cap);
+  S->reserved += t->bytes;
+  g->off = b.off;
+  g->deg = b.deg;
+  g->cap = b.cap;
+  g->pool = b.pool;
+  g->pool_cap = b.pool_cap;
+  S->live = nullptr;
+  S->detaches++;
+  return SSTG_OK;
+}
+
+// May the update at the head of the queue start now?  (caller holds S->mu)
+bool update_may_start(sstg_server *S) {
+  if (!S->live || S->live->readers == 0) return true;
+  if (!S->snapshots) return false;
+  const uint64_t need = store_bytes(S->g->n_rows, S->g->pool_cap);
+  const int k = find_spare(S);
+  const uint64_t freed = k >= 0 ? S->spare[k].bytes : 0;
+  return S->reserved - freed + need <= S->budget;
+}
+
 void refresh_view(sstg_graph *v, const sstg_graph *g) {
   v->n_rows = g->n_rows;
   v->n_cols = g->n_cols;
@@ -123,12 +256,12 @@ void refresh_view(sstg_graph *v, const sstg_graph *g) {
   v->symmetric = g->symmetric;
 }
 
-int run_query(sstg_graph *v, Request *r) {
+int run_query(sstg_server *S, sstg_graph *v, Request *r) {
   const uint64_t n = v->n_rows;
   switch (r->spec.kind) {
     case SSTG_REQ_BFS: {
       uint32_t *lev = nullptr;
-      SSTG_HIP(hipMalloc((void **)&lev, n * 4));
+      SSTG_TRY(result_alloc(S, n * 4, (void **)&lev));
       r->result = lev;
       r->result_bytes = n * 4;
       if (r->spec.source >= n) return SSTG_E_RANGE;
@@ -136,14 +269,14 @@ int run_query(sstg_graph *v, Request *r) {
     }
     case SSTG_REQ_PR: {
       double *rank = nullptr;
-      SSTG_HIP(hipMalloc((void **)&rank, n * 8));
+      SSTG_TRY(result_alloc(S, n * 8, (void **)&rank));
       r->result = rank;
       r->result_bytes = n * 8;
       return algo_pagerank(v, r->spec.iters, r->spec.damping, rank);
     }
     case SSTG_REQ_CC: {
       uint32_t *lab = nullptr;
-      SSTG_HIP(hipMalloc((void **)&lab, n * 4));
+      SSTG_TRY(result_alloc(S, n * 4, (void **)&lab));
       r->result = lab;
       r->result_bytes = n * 4;
       return algo_cc(v, lab);
@@ -165,7 +298,7 @@ void slot_main(sstg_server *S, Slot *sl) {
       if (sl->stop && !sl->req) return;
       r = sl->req;
     }
-    int rc = run_query(&sl->view, r);
+    int rc = run_query(S, &sl->view, r);
     if (rc == SSTG_OK) {
       hipError_t e = hipStreamSynchronize(sl->view.stream);
       if (e != hipSuccess) rc = SSTG_E_DEVICE;
@@ -177,6 +310,8 @@ void slot_main(sstg_server *S, Slot *sl) {
       r->done = true;
       S->reserved -= r->est_bytes;
       S->running--;
+      drop_reader(S, r->set);
+      r->set = nullptr;
       {
         std::lock_guard<std::mutex> lk2(sl->mu);
         sl->req = nullptr;
@@ -216,6 +351,22 @@ void apply_updates(sstg_server *S, std::unique_lock<std::mutex> &lk) {
   }
   const double t0 = now_s();
   for (Request *r : run) r->t_start = t0;
+  if (S->live && S->live->readers) {
+    int rc = detach_store(S);
+    if (rc != SSTG_OK) {
+      // no room for a copy: fail the run rather than race the readers
+      for (Request *r : run) {
+        r->status = rc;
+        r->t_done = now_s();
+        r->done = true;
+      }
+      S->done_cv.notify_all();
+      return;
+    }
+  } else if (S->live) {
+    delete S->live;  // no readers: the update may move the graph's arrays
+    S->live = nullptr;
+  }
   lk.unlock();
   hipSetDevice(S->g->device);
   int rc = graph_update(S->g, src.data(), dst.data(), total,
@@ -257,7 +408,7 @@ void sched_main(sstg_server *S) {
       if (S->stop) return true;
       if (S->pending.empty()) return false;
       Request *h = S->pending.front();
-      if (is_update(h->spec.kind)) return S->running == 0;  // writer waits
+      if (is_update(h->spec.kind)) return update_may_start(S);
       return S->reserved + h->est_bytes <= S->budget && free_slot(S);
     });
     if (S->stop) return;
@@ -276,6 +427,16 @@ void sched_main(sstg_server *S) {
       S->reserved += q->est_bytes;
       S->running++;
       q->t_start = now_s();
+      if (!S->live) {
+        S->live = new StoreSet();
+        S->live->off = S->g->off;
+        S->live->deg = S->g->deg;
+        S->live->cap = S->g->cap;
+        S->live->pool = S->g->pool;
+        S->live->pool_cap = S->g->pool_cap;
+      }
+      q->set = S->live;
+      S->live->readers++;
       refresh_view(&sl->view, S->g);
       {
         std::lock_guard<std::mutex> lk2(sl->mu);
@@ -318,8 +479,9 @@ using namespace sstg;
 extern "C" {
 
 int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
-                       int coalesce, sstg_server **out) {
+                       uint32_t flags, sstg_server **out) {
   if (!g || !out || slots == 0 || slots > 64) return SSTG_E_INVAL;
+  if (flags & ~(SSTG_SERVE_COALESCE | SSTG_SERVE_SNAPSHOTS)) return SSTG_E_INVAL;
   *out = nullptr;
   hipSetDevice(g->device);
   bool sym = false;
@@ -327,7 +489,8 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
   sstg_server *S = new (std::nothrow) sstg_server();
   if (!S) return SSTG_E_NOMEM;
   S->g = g;
-  S->coalesce = coalesce;
+  S->coalesce = (flags & SSTG_SERVE_COALESCE) ? 1 : 0;
+  S->snapshots = (flags & SSTG_SERVE_SNAPSHOTS) ? 1 : 0;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
     delete S;
@@ -434,19 +597,20 @@ int sstg_server_release(sstg_server *S, uint64_t ticket) {
     S->all.erase(it);
   }
   if (r->result) {
-    hipSetDevice(S->g->device);
-    hipFree(r->result);
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->spare_results.emplace(r->result_bytes, r->result);
   }
   delete r;
   return SSTG_OK;
 }
 
 int sstg_server_stats(sstg_server *S, uint64_t *device_batches,
-                      uint64_t *budget) {
+                      uint64_t *budget, uint64_t *snapshots) {
   if (!S) return SSTG_E_INVAL;
   std::lock_guard<std::mutex> lk(S->mu);
   if (device_batches) *device_batches = S->device_batches;
   if (budget) *budget = S->budget;
+  if (snapshots) *snapshots = S->detaches;
   return SSTG_OK;
 }
 
@@ -479,6 +643,9 @@ int sstg_server_destroy(sstg_server *S) {
     if (kv.second->result) hipFree(kv.second->result);
     delete kv.second;
   }
+  for (auto &kv : S->spare_results) hipFree(kv.second);
+  for (StoreSet &t : S->spare) free_set_arrays(t);
+  delete S->live;  // after the drain no query reads it; never owned
   delete S;
   return SSTG_OK;
 }

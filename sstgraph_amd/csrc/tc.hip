@@ -11,6 +11,11 @@
 // oriented edge u -> v: a triangle a < b < c (by rank) is found exactly once,
 // at a -> b with c in both lists.
 //
+// The same counting kernels serve directed rows: the reference's per-edge
+// count is the sum over u and v in L(u) of |L(u) n L(v)| with L(x) the ids
+// of row x below x, a prefix of the sorted row, so the store itself is the
+// list array there (count_list_triangles with base = off, len = lower).
+//
 // Layout: the oriented list of u is written at the same flat position as
 // row u of the store's degree prefix (dag[prefix[u] .. + dplus[u]), sorted by
 // id), so no second scan is needed.
@@ -152,8 +157,8 @@ __global__ void __launch_bounds__(256)
 // One wave (block of 64) per vertex u with 2 <= |out(u)| <= MaxOut.
 template <uint32_t MaxOut>
 __global__ void __launch_bounds__(64)
-    k_tc_count_wave(const uint64_t *prefix, const uint32_t *dag,
-                    const uint32_t *dplus, const uint32_t *list,
+    k_tc_count_wave(const uint64_t *lbase, const uint32_t *lists,
+                    const uint32_t *llen, const uint32_t *list,
                     uint64_t count, unsigned long long *acc) {
   static_assert((MaxOut & (MaxOut - 1)) == 0, "table size must be 2^k");
   __shared__ uint32_t table[4 * MaxOut];
@@ -163,16 +168,16 @@ __global__ void __launch_bounds__(64)
   uint64_t found = 0;
   for (uint64_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t u = list[i];
-    const uint32_t d = dplus[u];
-    const uint32_t *row = dag + prefix[u];
+    const uint32_t d = llen[u];
+    const uint32_t *row = lists + lbase[u];
     const uint32_t bits = table_bits(d);
     for (uint32_t t = lane; t < (1u << bits); t += 64) table[t] = kEmpty;
     __syncthreads();
     for (uint32_t j = lane; j < d; j += 64) {
       const uint32_t x = row[j];
       hash_insert(table, x, bits);
-      pre[j] = dplus[x];
-      base[j] = prefix[x];
+      pre[j] = llen[x];
+      base[j] = lbase[x];
     }
     __syncthreads();
     // exclusive scan of the list lengths; pre[d] = total
@@ -186,7 +191,7 @@ __global__ void __launch_bounds__(64)
     }
     if (lane == 0) pre[d] = carry;
     __syncthreads();
-    found += probe_lists<64>(table, bits, pre, base, dag, carry, lane);
+    found += probe_lists<64>(table, bits, pre, base, lists, carry, lane);
     __syncthreads();  // table, pre and base are reused by the next vertex
   }
   found = wave_sum(found);
@@ -199,8 +204,8 @@ __global__ void __launch_bounds__(64)
 // tiles of kBlkTile ids; for each tile every list out(v), v in out(u), is
 // streamed through it in steps of kBlkChunk lists.
 __global__ void __launch_bounds__(256)
-    k_tc_count_block(const uint64_t *prefix, const uint32_t *dag,
-                     const uint32_t *dplus, const uint32_t *list,
+    k_tc_count_block(const uint64_t *lbase, const uint32_t *lists,
+                     const uint32_t *llen, const uint32_t *list,
                      uint64_t count, unsigned long long *acc) {
   __shared__ uint32_t table[kBlkTable];
   __shared__ uint32_t pre[kBlkChunk + 1];
@@ -209,8 +214,8 @@ __global__ void __launch_bounds__(256)
   uint64_t found = 0;
   for (uint64_t i = blockIdx.x; i < count; i += gridDim.x) {
     const uint32_t u = list[i];
-    const uint32_t d = dplus[u];
-    const uint32_t *row = dag + prefix[u];
+    const uint32_t d = llen[u];
+    const uint32_t *row = lists + lbase[u];
     for (uint32_t t0 = 0; t0 < d; t0 += kBlkTile) {
       const uint32_t tn = (d - t0) < kBlkTile ? (d - t0) : kBlkTile;
       const uint32_t bits = table_bits(tn);
@@ -223,15 +228,15 @@ __global__ void __launch_bounds__(256)
         uint32_t len = 0;
         if (tid < cn) {
           const uint32_t x = row[c0 + tid];
-          len = dplus[x];
-          base[tid] = prefix[x];
+          len = llen[x];
+          base[tid] = lbase[x];
         }
         uint32_t total = 0;
         const uint32_t off = em::block_scan_u32(len, &total);
         if (tid < cn) pre[tid] = off;
         if (tid == 0) pre[cn] = total;
         __syncthreads();
-        found += probe_lists<256>(table, bits, pre, base, dag, total, tid);
+        found += probe_lists<256>(table, bits, pre, base, lists, total, tid);
         __syncthreads();  // pre and base are rewritten by the next step
       }
     }
@@ -242,8 +247,24 @@ __global__ void __launch_bounds__(256)
               (unsigned long long)found);
 }
 
+// acc[i * kAccStride] for i < kAccLanes are the spread counters; the slot
+// after them holds the largest list length
 __global__ void k_tc_acc_clear(unsigned long long *acc) {
   acc[threadIdx.x * kAccStride] = 0;
+  if (threadIdx.x == 0) acc[kAccLanes * kAccStride] = 0;
+}
+
+__global__ void k_tc_max_len(const uint32_t *len, uint32_t n,
+                             unsigned long long *acc) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t v = i < n ? len[i] : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = __shfl_xor(v, o, 64);
+    v = t > v ? t : v;
+  }
+  if ((threadIdx.x & 63) == 0 && v)
+    atomicMax(&acc[kAccLanes * kAccStride], (unsigned long long)v);
 }
 
 __global__ void k_tc_acc_sum(unsigned long long *acc) {
@@ -258,6 +279,72 @@ uint32_t grid_for(uint64_t count) {
 
 }  // namespace
 
+int count_list_triangles(sstg_graph *g, const uint64_t *lbase,
+                         const uint32_t *lists, const uint32_t *llen,
+                         uint64_t *out) {
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  uint32_t *list = nullptr;
+  unsigned long long *acc = nullptr;
+  uint8_t *flags = nullptr;
+  uint64_t *tmp = nullptr;
+  SSTG_TRY(g->ws.get(kWsTcList, n, &list));
+  SSTG_TRY(g->ws.get(kWsTcAcc, (kAccLanes + 1) * kAccStride, &acc));
+  SSTG_TRY(g->ws.get(kWsAlgo + 24, n, &flags));
+  SSTG_TRY(g->ws.get(kWsAlgo + 25, n, &tmp));
+  SSTG_LAUNCH(k_tc_acc_clear, 1, kAccLanes, 0, s, acc);
+  // the flat probe index of one vertex (up to 512 lists, or 256 per block
+  // step) is 32-bit: lists of 2^23 ids or more go to the caller's fallback
+  SSTG_LAUNCH(k_tc_max_len, ceil_div(n, 256), 256, 0, s, llen, n, acc);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb + 17, acc + kAccLanes * kAccStride,
+                          sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  if (g->host_rb[17] >= (1u << 23)) return SSTG_E_UNSUPPORTED;
+  for (uint32_t c = 0; c <= kWaveTiers; c++) {
+    const uint32_t lo = c == 0 ? 2 : kTierOut[c - 1] + 1;
+    const uint32_t hi = c < kWaveTiers ? kTierOut[c] : 0xFFFFFFFFu;
+    SSTG_LAUNCH(em::k_bin_flags, ceil_div(n, em::kT), em::kT, 0, s, llen, n,
+                lo, hi, flags);
+    SSTG_TRY(select_flagged_u64(flags, nullptr, n, tmp, g->dev_ctr + 7, g->ws,
+                                kWsPrim, s));
+    SSTG_HIP(hipMemcpyAsync(g->host_rb + 16, g->dev_ctr + 7, sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    SSTG_HIP(hipStreamSynchronize(s));
+    const uint64_t cnt = g->host_rb[16];
+    if (!cnt) continue;
+    SSTG_TRY(narrow_u64_to_u32(tmp, cnt, list, s));
+    const uint32_t grid = grid_for(cnt);
+    const uint32_t *l = list;
+    switch (c) {
+      case 0:
+        SSTG_LAUNCH(k_tc_count_wave<kTierOut[0]>, grid, 64, 0, s, lbase, lists,
+                    llen, l, cnt, acc);
+        break;
+      case 1:
+        SSTG_LAUNCH(k_tc_count_wave<kTierOut[1]>, grid, 64, 0, s, lbase, lists,
+                    llen, l, cnt, acc);
+        break;
+      case 2:
+        SSTG_LAUNCH(k_tc_count_wave<kTierOut[2]>, grid, 64, 0, s, lbase, lists,
+                    llen, l, cnt, acc);
+        break;
+      case 3:
+        SSTG_LAUNCH(k_tc_count_wave<kTierOut[3]>, grid, 64, 0, s, lbase, lists,
+                    llen, l, cnt, acc);
+        break;
+      default:
+        SSTG_LAUNCH(k_tc_count_block, grid, 256, 0, s, lbase, lists, llen, l,
+                    cnt, acc);
+    }
+  }
+  SSTG_LAUNCH(k_tc_acc_sum, 1, kAccLanes, 0, s, acc);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, acc, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  *out = g->host_rb[0];
+  return SSTG_OK;
+}
+
 int tc_oriented(sstg_graph *g, uint64_t *out) {
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
@@ -265,16 +352,9 @@ int tc_oriented(sstg_graph *g, uint64_t *out) {
   SSTG_TRY(degree_bins(g, &bins));
   const uint64_t *prefix = nullptr;
   SSTG_TRY(graph_degree_prefix(g, &prefix));
-  uint32_t *dag = nullptr, *dplus = nullptr, *list = nullptr;
-  unsigned long long *acc = nullptr;
-  uint8_t *flags = nullptr;
-  uint64_t *tmp = nullptr;
+  uint32_t *dag = nullptr, *dplus = nullptr;
   SSTG_TRY(g->ws.get(kWsTcDag, g->m, &dag));
   SSTG_TRY(g->ws.get(kWsTcDplus, n, &dplus));
-  SSTG_TRY(g->ws.get(kWsTcList, n, &list));
-  SSTG_TRY(g->ws.get(kWsTcAcc, kAccLanes * kAccStride, &acc));
-  SSTG_TRY(g->ws.get(kWsAlgo + 24, n, &flags));
-  SSTG_TRY(g->ws.get(kWsAlgo + 25, n, &tmp));
   const DevGraph v = g->view();
   if (bins.count[0])
     SSTG_LAUNCH(k_tc_orient<1>, ceil_div(bins.count[0], 256), 256, 0, s, v,
@@ -289,50 +369,7 @@ int tc_oriented(sstg_graph *g, uint64_t *out) {
       SSTG_LAUNCH(k_tc_orient<64>, ceil_div(bins.count[b] * 64, 256), 256, 0,
                   s, v, (const uint32_t *)bins.list[b], bins.count[b], prefix,
                   dag, dplus);
-  SSTG_LAUNCH(k_tc_acc_clear, 1, kAccLanes, 0, s, acc);
-  for (uint32_t c = 0; c <= kWaveTiers; c++) {
-    const uint32_t lo = c == 0 ? 2 : kTierOut[c - 1] + 1;
-    const uint32_t hi = c < kWaveTiers ? kTierOut[c] : 0xFFFFFFFFu;
-    SSTG_LAUNCH(em::k_bin_flags, ceil_div(n, em::kT), em::kT, 0, s,
-                (const uint32_t *)dplus, n, lo, hi, flags);
-    SSTG_TRY(select_flagged_u64(flags, nullptr, n, tmp, g->dev_ctr + 7, g->ws,
-                                kWsPrim, s));
-    SSTG_HIP(hipMemcpyAsync(g->host_rb + 16, g->dev_ctr + 7, sizeof(uint64_t),
-                            hipMemcpyDeviceToHost, s));
-    SSTG_HIP(hipStreamSynchronize(s));
-    const uint64_t cnt = g->host_rb[16];
-    if (!cnt) continue;
-    SSTG_TRY(narrow_u64_to_u32(tmp, cnt, list, s));
-    const uint32_t grid = grid_for(cnt);
-    const uint32_t *l = list, *dg = dag, *dp = dplus;
-    switch (c) {
-      case 0:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[0]>, grid, 64, 0, s, prefix, dg,
-                    dp, l, cnt, acc);
-        break;
-      case 1:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[1]>, grid, 64, 0, s, prefix, dg,
-                    dp, l, cnt, acc);
-        break;
-      case 2:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[2]>, grid, 64, 0, s, prefix, dg,
-                    dp, l, cnt, acc);
-        break;
-      case 3:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[3]>, grid, 64, 0, s, prefix, dg,
-                    dp, l, cnt, acc);
-        break;
-      default:
-        SSTG_LAUNCH(k_tc_count_block, grid, 256, 0, s, prefix, dg, dp, l, cnt,
-                    acc);
-    }
-  }
-  SSTG_LAUNCH(k_tc_acc_sum, 1, kAccLanes, 0, s, acc);
-  SSTG_HIP(hipMemcpyAsync(g->host_rb, acc, sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-  SSTG_HIP(hipStreamSynchronize(s));
-  *out = g->host_rb[0];
-  return SSTG_OK;
+  return count_list_triangles(g, prefix, dag, dplus, out);
 }
 
 }  // namespace sstg

@@ -114,3 +114,20 @@ def test_directed_graph_keeps_per_edge_count():
     rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
     g = _graph(n, rows, cols)
     assert EMV.TC(g) == O.tc(off, cols)
+
+
+def test_directed_dense_rows_block_tier():
+    """Directed rows with more than 512 ids below the row: the per-edge rule
+    runs on the store's own rows through the block tier."""
+    rng = np.random.default_rng(23)
+    n = 1500
+    keep = rng.random((n, n)) < 0.7
+    np.fill_diagonal(keep, False)
+    s, d = np.nonzero(keep)
+    s, d = s.astype(np.uint32), d.astype(np.uint32)
+    off, cols = csr_from_edges(n, s, d)
+    g = _graph(n, s, d)
+    want = O.tc(off, cols)
+    say("directed dense count", want)
+    assert EMV.TC(g) == want
+
