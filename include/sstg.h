@@ -37,6 +37,7 @@ extern "C" {
 #define SSTG_E_DEVICE -4   /* HIP runtime error */
 #define SSTG_E_NODEVICE -5 /* no usable gfx950 device */
 #define SSTG_E_UNSUPPORTED -6
+#define SSTG_E_IO -7       /* a file could not be opened, read or written */
 
 /* flags */
 #define SSTG_DEVICE_PTRS 0x1u  /* array arguments are device pointers */
@@ -231,6 +232,50 @@ int sstg_server_stats(sstg_server *s, uint64_t *device_batches,
                       uint64_t *budget, uint64_t *snapshots);
 /* Waits for every submitted request, then stops the server. */
 int sstg_server_destroy(sstg_server *s);
+
+/* ---- relabelling, statistics, graph files ----------------------------- */
+/* map[i] = new label of vertex i: the rank of (degree(i), i) in descending
+ * order (SparseMatrix.hpp:737-749, degree_order_map), computed on the
+ * device; map has n_rows entries (host, or device with SSTG_DEVICE_PTRS). */
+int sstg_degree_order_map(sstg_graph *g, uint32_t *map, uint32_t flags);
+/* A new graph holding (map[i], map[j]) for every stored edge (i, j), self-
+ * loops kept (SparseMatrix.hpp:751-766, rewrite_graph).  SSTG_E_RANGE if a
+ * label is out of range. */
+int sstg_rewrite_graph(sstg_graph *g, const uint32_t *map, uint32_t flags,
+                       sstg_graph **out);
+/* print_statistics (SparseMatrix.hpp:340-426): the layout-free figures of
+ * the reference (elements, rows, columns, bytes, destinations by bits
+ * needed) and the HBM store's own. */
+typedef struct sstg_stats {
+  uint64_t edges;          /* M() */
+  uint32_t rows, cols;
+  uint64_t memory_bytes;   /* get_memory_size() */
+  uint64_t empty_rows;
+  uint64_t max_degree;
+  uint64_t slot_capacity;  /* sum of the rows' slot capacities (elements) */
+  uint64_t pool_capacity;  /* elements allocated for all slots */
+  uint64_t pool_used;      /* elements below the pool tail */
+  uint64_t pool_garbage;   /* elements in abandoned slots */
+  uint64_t rebuilds;       /* whole-pool rebuilds so far */
+  uint64_t dest_bits[33];  /* stored edges whose destination needs x bits */
+} sstg_stats;
+int sstg_statistics(sstg_graph *g, sstg_stats *out);
+/* get_edges_from_file (helpers.hpp:604-631): .adj / .adj.shuf
+ * (AdjacencyGraph or WeightedAdjacencyGraph; helpers.hpp:533-601) or .mtx
+ * (MatrixMarket coordinate, 1-indexed, every entry returned in both
+ * directions, square; helpers.hpp:382-531).  *src, *dst and, when weights
+ * is not NULL, *weights (the file's integer values, 1 for unweighted files)
+ * are malloc'd: free them with sstg_host_free.  Needs no device.
+ * SSTG_E_IO if the file cannot be read, SSTG_E_INVAL if it is malformed. */
+int sstg_edges_from_file(const char *path, uint32_t *n, uint64_t *m,
+                         uint32_t **src, uint32_t **dst, uint32_t **weights);
+void sstg_host_free(void *p);
+/* Reads a graph file into a new n x n graph (the harness's load step,
+ * test.hpp:414-449). */
+int sstg_load_file(const char *path, uint32_t flags, sstg_graph **out);
+/* Writes the graph as an AdjacencyGraph file (graph_utilities.cpp:151-183,
+ * write_unweighted_graph). */
+int sstg_write_adj(sstg_graph *g, const char *path);
 
 /* Seeded Erdos-Renyi edge stream on n vertices (no reference counterpart:
  * the reference's ER tool, graph_utilities.cpp:168-212, is unseeded):

@@ -20,8 +20,10 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -75,6 +77,9 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
   SparseMatrixV(el_t height, el_t width) : rows_(height), cols_(width) {
     detail::check(sstg_create(height, width, 0, &g_), "SparseMatrixV");
   }
+  // adopts a graph handle made by the library (rewrite_graph, load)
+  SparseMatrixV(sstg_graph *h, el_t height, el_t width)
+      : g_(h), rows_(height), cols_(width) {}
   SparseMatrixV(const SparseMatrixV &o) : rows_(o.rows_), cols_(o.cols_) {
     detail::check(sstg_create(rows_, cols_, 0, &g_), "SparseMatrixV copy");
     copy_from(o);
@@ -232,33 +237,58 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
         f(i, (uint64_t)snap_cols_[e]);
   }
 
-  // SparseMatrix.hpp:737-749: new label of vertex i by descending (degree, id)
+  // SparseMatrix.hpp:737-749: new label of vertex i by descending
+  // (degree, id), computed on the device
   [[nodiscard]] std::vector<uint64_t> degree_order_map() const {
-    refresh();
-    std::vector<std::pair<uint64_t, uint64_t>> lm(rows_);
-    for (uint64_t i = 0; i < rows_; i++)
-      lm[i] = {snap_off_[i + 1] - snap_off_[i], i};
-    std::sort(lm.begin(), lm.end(), std::greater<>());
-    std::vector<uint64_t> out(rows_);
-    for (uint64_t i = 0; i < rows_; i++) out[lm[i].second] = i;
-    return out;
+    std::vector<uint32_t> m(rows_);
+    detail::check(sstg_degree_order_map(g_, m.data(), 0), "degree_order_map");
+    return std::vector<uint64_t>(m.begin(), m.end());
   }
-  // SparseMatrix.hpp:751-766
+  // SparseMatrix.hpp:751-766: every edge relabelled on the device, self-
+  // loops kept (the reference re-inserts edge by edge with insert())
   [[nodiscard]] SparseMatrixV rewrite_graph(
       std::vector<uint64_t> new_label_map) const {
-    refresh();
-    SparseMatrixV out(get_rows(), get_cols());
-    std::vector<el_t> s(snap_cols_.size()), d(snap_cols_.size());
-    for (uint64_t i = 0; i < rows_; i++)
-      for (uint64_t e = snap_off_[i]; e < snap_off_[i + 1]; e++) {
-        s[e] = (el_t)new_label_map[i];
-        d[e] = (el_t)new_label_map[snap_cols_[e]];
-      }
-    // single inserts keep self-loops; the relabelled rows have none unless
-    // the source graph had them, so the batch form matches rewrite_graph's
-    // per-edge insert() except that self-loops are preserved explicitly
-    out.insert_batch(s.data(), d.data(), s.size(), SSTG_KEEP_SELF_LOOPS);
-    return out;
+    if (new_label_map.size() != rows_)
+      throw sstg_error(SSTG_E_INVAL, "rewrite_graph: one label per row");
+    std::vector<uint32_t> m(rows_);
+    for (uint64_t i = 0; i < rows_; i++) {
+      if (new_label_map[i] > 0xFFFFFFFFull)
+        throw sstg_error(SSTG_E_RANGE, "rewrite_graph");
+      m[i] = (uint32_t)new_label_map[i];
+    }
+    sstg_graph *h = nullptr;
+    detail::check(sstg_rewrite_graph(g_, m.data(), 0, &h), "rewrite_graph");
+    return SparseMatrixV(h, rows_, cols_);
+  }
+
+  // SparseMatrix.hpp:340-426: the reference's layout-free lines (its
+  // per-tinyset byte-width tables describe a CPU layout this store does
+  // not have), then the HBM store's own figures
+  void print_statistics() const {
+    sstg_stats st;
+    detail::check(sstg_statistics(g_, &st), "print_statistics");
+    std::printf("matrix stored in csr\n");
+    std::printf("there are %llu elements\n", (unsigned long long)st.edges);
+    std::printf("there are %u rows and %u columns\n", st.rows, st.cols);
+    std::printf("The graph uses %llu bytes of memory\n",
+                (unsigned long long)st.memory_bytes);
+    std::printf("%llu rows are empty; the largest row has %llu elements\n",
+                (unsigned long long)st.empty_rows,
+                (unsigned long long)st.max_degree);
+    std::printf("slots hold %llu elements; the pool has %llu (%llu below "
+                "the tail, %llu abandoned), %llu rebuilds\n",
+                (unsigned long long)st.slot_capacity,
+                (unsigned long long)st.pool_capacity,
+                (unsigned long long)st.pool_used,
+                (unsigned long long)st.pool_garbage,
+                (unsigned long long)st.rebuilds);
+    std::printf("count of outgoing edges which uses x bits\n");
+    unsigned long long total = 0;
+    for (int i = 0; i < 33; i++) {
+      total += st.dest_bits[i];
+      std::printf("%d, %llu, total so far = %llu\n", i,
+                  (unsigned long long)st.dest_bits[i], total);
+    }
   }
 
   sstg_graph *handle() const { return g_; }
@@ -300,6 +330,36 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
   mutable std::vector<uint64_t> snap_off_;
   mutable std::vector<el_t> snap_cols_;
 };
+
+// helpers.hpp:604-631: the edges of a .adj / .adj.shuf / .mtx file as a
+// malloc'd array the caller free()s (binary graphs; .mtx entries come in
+// both directions, 1-indexed ids shifted to 0).  Throws sstg_error where the
+// reference prints and exits.
+template <typename value_type = bool>
+std::tuple<el_t, el_t> *get_edges_from_file(const std::string &filename,
+                                            uint64_t *edge_count,
+                                            uint32_t *node_count,
+                                            bool print = true) {
+  static_assert(std::is_same_v<value_type, bool>,
+                "sstgraph_amd reads binary graphs");
+  (void)print;
+  uint32_t *s = nullptr, *d = nullptr;
+  detail::check(sstg_edges_from_file(filename.c_str(), node_count, edge_count,
+                                     &s, &d, nullptr),
+                "get_edges_from_file");
+  auto *out = (std::tuple<el_t, el_t> *)std::malloc(
+      (*edge_count ? *edge_count : 1) * sizeof(std::tuple<el_t, el_t>));
+  if (!out) {
+    sstg_host_free(s);
+    sstg_host_free(d);
+    throw sstg_error(SSTG_E_NOMEM, "get_edges_from_file");
+  }
+  for (uint64_t i = 0; i < *edge_count; i++)
+    new (&out[i]) std::tuple<el_t, el_t>(s[i], d[i]);
+  sstg_host_free(s);
+  sstg_host_free(d);
+  return out;
+}
 
 }  // namespace SSTGraph
 

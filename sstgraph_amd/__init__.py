@@ -32,6 +32,7 @@ SSTG_E_NOMEM = -3
 SSTG_E_DEVICE = -4
 SSTG_E_NODEVICE = -5
 SSTG_E_UNSUPPORTED = -6
+SSTG_E_IO = -7
 
 SSTG_DEVICE_PTRS = 0x1
 SSTG_SERVE_COALESCE = 0x1
@@ -54,6 +55,17 @@ class DeviceUnavailable(SSTGError):
 
 class IdOutOfRange(SSTGError, ValueError):
     pass
+
+
+class Stats(ctypes.Structure):
+    """sstg_stats (print_statistics figures)."""
+    _fields_ = [("edges", ctypes.c_uint64), ("rows", ctypes.c_uint32),
+                ("cols", ctypes.c_uint32), ("memory_bytes", ctypes.c_uint64),
+                ("empty_rows", ctypes.c_uint64), ("max_degree", ctypes.c_uint64),
+                ("slot_capacity", ctypes.c_uint64),
+                ("pool_capacity", ctypes.c_uint64), ("pool_used", ctypes.c_uint64),
+                ("pool_garbage", ctypes.c_uint64), ("rebuilds", ctypes.c_uint64),
+                ("dest_bits", ctypes.c_uint64 * 33)]
 
 
 class UpdateStats(ctypes.Structure):
@@ -141,6 +153,14 @@ def lib():
         "sstg_server_release": (i32, [vp, u64]),
         "sstg_server_stats": (i32, [vp, P(u64), P(u64), P(u64)]),
         "sstg_server_destroy": (i32, [vp]),
+        "sstg_degree_order_map": (i32, [vp, vp, u32]),
+        "sstg_rewrite_graph": (i32, [vp, vp, u32, P(vp)]),
+        "sstg_statistics": (i32, [vp, P(Stats)]),
+        "sstg_edges_from_file": (i32, [ctypes.c_char_p, P(u32), P(u64), P(vp),
+                                       P(vp), P(vp)]),
+        "sstg_host_free": (None, [vp]),
+        "sstg_load_file": (i32, [ctypes.c_char_p, u32, P(vp)]),
+        "sstg_write_adj": (i32, [vp, ctypes.c_char_p]),
         "sstg_device_alloc": (i32, [u64, P(vp)]),
         "sstg_device_free": (i32, [vp]),
         "sstg_memcpy_h2d": (i32, [vp, vp, u64]),
@@ -175,6 +195,8 @@ def _check(rc, what):
         raise DeviceUnavailable(rc, what)
     if rc == SSTG_E_RANGE:
         raise IdOutOfRange(rc, what)
+    if rc == SSTG_E_IO:
+        raise OSError(f"{what}: {lib().sstg_strerror(rc).decode()}")
     raise SSTGError(rc, what)
 
 
@@ -315,17 +337,101 @@ class MultiGraph:
         return arr[self.vbase[g]:self.vbase[g + 1]]
 
 
+def edges_from_file(path, weights=False):
+    """get_edges_from_file (helpers.hpp:604-631) for .adj / .adj.shuf / .mtx:
+    returns (n, src, dst) or (n, src, dst, weights) as numpy uint32 arrays.
+    Runs on the host; no device needed."""
+    n, m = ctypes.c_uint32(0), ctypes.c_uint64(0)
+    ps, pd, pw = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    _check(lib().sstg_edges_from_file(os.fsencode(path), ctypes.byref(n),
+                                      ctypes.byref(m), ctypes.byref(ps),
+                                      ctypes.byref(pd),
+                                      ctypes.byref(pw) if weights else None),
+           f"read {path}")
+    out = []
+    try:
+        for p in (ps, pd) + ((pw,) if weights else ()):
+            buf = (ctypes.c_uint32 * max(m.value, 1)).from_address(p.value)
+            out.append(np.frombuffer(buf, np.uint32, count=m.value).copy())
+    finally:
+        for p in (ps, pd, pw):
+            if p.value:
+                lib().sstg_host_free(p)
+    return (n.value, *out)
+
+
 class SparseMatrixV:
     """Device-resident SparseMatrixV<true> (CSR, binary): one sorted set of
     column ids per row."""
 
-    def __init__(self, height, width=None):
+    def __init__(self, height, width=None, _handle=None):
         self.n_rows = int(height)
         self.n_cols = int(height if width is None else width)
+        if _handle is not None:
+            self._h = _handle
+            return
         h = ctypes.c_void_p()
         _check(lib().sstg_create(self.n_rows, self.n_cols, 0, ctypes.byref(h)),
                "sstg_create")
         self._h = h
+
+    @classmethod
+    def from_file(cls, path):
+        """Reads a .adj / .mtx file into a new graph (sstg_load_file)."""
+        h = ctypes.c_void_p()
+        _check(lib().sstg_load_file(os.fsencode(path), 0, ctypes.byref(h)),
+               f"load {path}")
+        n = ctypes.c_uint32(0)
+        _check(lib().sstg_num_nodes(h, ctypes.byref(n)), "num_nodes")
+        return cls(n.value, n.value, _handle=h)
+
+    def write_adj(self, path):
+        """Writes an AdjacencyGraph file (graph_utilities.cpp:151-183)."""
+        _check(lib().sstg_write_adj(self._h, os.fsencode(path)), f"write {path}")
+
+    def degree_order_map(self):
+        """SparseMatrix.hpp:737-749: new label of each vertex by descending
+        (degree, id), computed on the device."""
+        out = np.empty(self.n_rows, np.uint32)
+        _check(lib().sstg_degree_order_map(self._h, _ptr(out), 0), "degree_order_map")
+        return out
+
+    def rewrite_graph(self, new_label_map):
+        """SparseMatrix.hpp:751-766: a new graph with every edge relabelled."""
+        m = _u32(new_label_map)
+        if m.size != self.n_rows:
+            raise ValueError("label map needs one entry per row")
+        h = ctypes.c_void_p()
+        _check(lib().sstg_rewrite_graph(self._h, m.ctypes.data, 0, ctypes.byref(h)),
+               "rewrite_graph")
+        return SparseMatrixV(self.n_rows, self.n_cols, _handle=h)
+
+    def statistics(self):
+        st = Stats()
+        _check(lib().sstg_statistics(self._h, ctypes.byref(st)), "statistics")
+        d = {k: getattr(st, k) for k, _ in Stats._fields_ if k != "dest_bits"}
+        d["dest_bits"] = list(st.dest_bits)
+        return d
+
+    def print_statistics(self, file=None):
+        """print_statistics (SparseMatrix.hpp:340-426): the layout-free lines
+        of the reference, then the HBM store's own figures."""
+        st = self.statistics()
+        say = lambda *a: print(*a, file=file)
+        say("matrix stored in csr")
+        say(f"there are {st['edges']} elements")
+        say(f"there are {st['rows']} rows and {st['cols']} columns")
+        say(f"The graph uses {st['memory_bytes']} bytes of memory")
+        say(f"{st['empty_rows']} rows are empty; the largest row has "
+            f"{st['max_degree']} elements")
+        say(f"slots hold {st['slot_capacity']} elements; the pool has "
+            f"{st['pool_capacity']} ({st['pool_used']} below the tail, "
+            f"{st['pool_garbage']} abandoned), {st['rebuilds']} rebuilds")
+        say("count of outgoing edges which uses x bits")
+        total = 0
+        for i, c in enumerate(st["dest_bits"]):
+            total += c
+            say(f"{i}, {c}, total so far = {total}")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -2,7 +2,8 @@
 
     python -m sstgraph_amd.build        # or __graft_entry__.build()
 
-Every .hip file under csrc/ is compiled by hipcc into _build/ and linked into
+Every .hip file under csrc/ is compiled by hipcc, and every host-only .cpp
+file by the host compiler, into _build/; the objects are linked into
 sstgraph_amd/libsstg.so next to this file.
 """
 import concurrent.futures as cf
@@ -28,9 +29,19 @@ CFLAGS = [
 ]
 
 
+# host-only sources (file readers over the public C ABI)
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread",
+            f"-I{os.path.join(ROOT, 'include')}"]
+
+
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    base, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(BUILD, base + (".o" if ext == ".hip" else ".cpp.o"))
+    if ext == ".hip":
+        cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
+    else:
+        cmd = [CXX] + CXXFLAGS + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r.returncode, r.stdout + r.stderr
 
@@ -38,7 +49,7 @@ def _compile(src):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
-                  if f.endswith(".hip"))
+                  if f.endswith(".hip") or f.endswith(".cpp"))
     objs = []
     errors = []
     workers = min(len(srcs), max(1, (os.cpu_count() or 2) // 2), 8)
@@ -53,7 +64,8 @@ def build(verbose=False):
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread",
+           "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

@@ -187,6 +187,8 @@ const char *sstg_strerror(int status) {
       return g_last_error[0] ? g_last_error : "HIP runtime error";
     case SSTG_E_NODEVICE:
       return "no gfx950 device";
+    case SSTG_E_IO:
+      return "file could not be opened, read or written";
     case SSTG_E_UNSUPPORTED:
       return "unsupported";
     default:
@@ -334,6 +336,55 @@ int sstg_memory_size(sstg_graph *g, uint64_t *out) {
   *out = sizeof(sstg_graph) + n * (sizeof(uint64_t) + 2 * sizeof(uint32_t)) +
          g->pool_cap * sizeof(uint32_t);
   return SSTG_OK;
+}
+
+int sstg_degree_order_map(sstg_graph *g, uint32_t *map, uint32_t flags) {
+  if (!g || !map) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return with_output<uint32_t>(g, map, g->n_rows, flags, kWsOut,
+                               [&](uint32_t *d) {
+                                 return graph_degree_order_map(g, d);
+                               });
+}
+
+int sstg_rewrite_graph(sstg_graph *g, const uint32_t *map, uint32_t flags,
+                       sstg_graph **out) {
+  if (!g || !map || !out) return SSTG_E_INVAL;
+  *out = nullptr;
+  hipSetDevice(g->device);
+  const uint32_t *dmap = map;
+  if (!(flags & SSTG_DEVICE_PTRS)) {
+    uint32_t *d = nullptr;
+    SSTG_TRY(g->ws.get(kWsOut, g->n_rows, &d));
+    SSTG_HIP(hipMemcpyAsync(d, map, (uint64_t)g->n_rows * 4,
+                            hipMemcpyHostToDevice, g->stream));
+    dmap = d;
+  }
+  sstg_graph *ng = nullptr;
+  SSTG_TRY(sstg_create(g->n_rows, g->n_cols, 0, &ng));
+  int rc = graph_rewrite(g, dmap, ng);
+  if (rc != SSTG_OK) {
+    sstg_destroy(ng);
+    return rc;
+  }
+  *out = ng;
+  return SSTG_OK;
+}
+
+int sstg_statistics(sstg_graph *g, sstg_stats *out) {
+  if (!g || !out) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  *out = sstg_stats{};
+  out->edges = g->m;
+  out->rows = g->n_rows;
+  out->cols = g->n_cols;
+  SSTG_TRY(sstg_memory_size(g, &out->memory_bytes));
+  out->pool_capacity = g->pool_cap;
+  out->pool_used = g->pool_tail;
+  out->pool_garbage = g->garbage;
+  out->rebuilds = g->rebuilds;
+  return graph_statistics(g, out->dest_bits, &out->empty_rows,
+                          &out->max_degree, &out->slot_capacity);
 }
 
 int sstg_touch_all_sum(sstg_graph *g, uint64_t *out) {

@@ -56,6 +56,11 @@ int graph_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
 int graph_touch_sum(sstg_graph *g, uint64_t *out);
 // algo.hip
 int graph_is_symmetric(sstg_graph *g, bool *out);
+// relabel.hip
+int graph_degree_order_map(sstg_graph *g, uint32_t *map);
+int graph_rewrite(sstg_graph *g, const uint32_t *map, sstg_graph *out);
+int graph_statistics(sstg_graph *g, uint64_t *hist33, uint64_t *empty_rows,
+                     uint64_t *max_degree, uint64_t *slot_capacity);
 
 // rmat.hip
 int er_generate(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
@@ -101,6 +106,10 @@ enum : int {
   kWsTcDplus = 99,
   kWsTcList = 100,
   kWsTcAcc = 101,
+  kWsRelKeys = 102,
+  kWsRelTmp = 103,
+  kWsRelSrc = 104,
+  kWsRelDst = 105,
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range

@@ -2,8 +2,10 @@
 // reference harness uses SparseMatrixV<true> and EdgeMapVertexMap
 // (include/SSTGraph/internal/test.hpp:396-697), checking every result
 // against the CPU oracle.  Exit code 0 = pass, 77 = no GPU (skipped).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <cstdlib>
 #include <tuple>
 #include <vector>
@@ -30,7 +32,7 @@ struct CountF {  // map_neighbors functor with the reference's contract
   }
 };
 
-int main() {
+int main(int argc, char **argv) {
   int ndev = 0;
   sstg_device_count(&ndev);
   if (ndev == 0) {
@@ -140,8 +142,16 @@ int main() {
   std::free(bf);
   CHECK(EdgeMapVertexMap::TouchAll(g) == orc_touch_all_sum(og));
 
-  // degree-order relabel keeps the triangle count and edge count
-  SSTGraph::SparseMatrixV<true> h = g.rewrite_graph(g.degree_order_map());
+  // degree-order relabel keeps the triangle count and edge count; the map is
+  // the reference's definition (SparseMatrix.hpp:737-749)
+  const std::vector<uint64_t> dmap = g.degree_order_map();
+  {
+    std::vector<std::pair<uint64_t, uint64_t>> lm(n);
+    for (uint32_t j = 0; j < n; j++) lm[j] = {off[j + 1] - off[j], j};
+    std::sort(lm.begin(), lm.end(), std::greater<>());
+    for (uint32_t j = 0; j < n; j++) CHECK(dmap[lm[j].second] == j);
+  }
+  SSTGraph::SparseMatrixV<true> h = g.rewrite_graph(dmap);
   CHECK(h.M() == g.M());
   CHECK(EdgeMapVertexMap::TC(h) == tc);
   // copies are deep
@@ -168,6 +178,19 @@ int main() {
   }
   CHECK(threw);
   orc_graph_destroy(og);
+
+  // a graph file read as the harness reads it (test.hpp:414-449)
+  if (argc > 1) {
+    uint64_t fm = 0;
+    uint32_t fn = 0;
+    auto *fe = SSTGraph::get_edges_from_file(argv[1], &fm, &fn);
+    SSTGraph::SparseMatrixV<true> fg(fn, fn);
+    fg.insert_batch(fe, fm);
+    std::free(fe);
+    CHECK(fn == 28 && fm == 420 && fg.M() == 420);
+    CHECK(EdgeMapVertexMap::TC(fg) == 420);
+    fg.print_statistics();
+  }
   std::printf("facade ok: n=%u m=%llu tc=%llu\n", n,
               (unsigned long long)off[n], (unsigned long long)tc);
   return 0;

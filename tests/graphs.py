@@ -7,18 +7,18 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def read_mtx(path):
-    """MatrixMarket coordinate pattern reader: 1-indexed, 'symmetric' files
-    are symmetrised -- the convention of the reference's loader
-    (helpers.hpp:379-528: each line (s, d) yields (s-1, d-1) and (d-1, s-1))."""
+    """MatrixMarket coordinate reader: 1-indexed, and every entry is taken in
+    both directions whether the file says symmetric or general -- the
+    convention of the reference's loader (helpers.hpp:505-521: each line
+    (s, d) yields (s-1, d-1) and (d-1, s-1))."""
     with open(path) as f:
-        header = f.readline().split()
+        f.readline()
         lines = [ln for ln in f if not ln.startswith("%")]
     r, c, m = (int(x) for x in lines[0].split())
     pairs = np.array([[int(x) for x in ln.split()[:2]] for ln in lines[1:1 + m]],
-                     dtype=np.int64) - 1
+                     dtype=np.int64).reshape(-1, 2) - 1
     src, dst = pairs[:, 0], pairs[:, 1]
-    if "symmetric" in header:
-        src, dst = np.concatenate([src, dst]), np.concatenate([dst, src])
+    src, dst = np.concatenate([src, dst]), np.concatenate([dst, src])
     return r, src.astype(np.uint32), dst.astype(np.uint32)
 
 
