@@ -6,6 +6,8 @@
 // row's neighbours are pool[off[v] .. off[v] + deg[v]) in ascending order.
 #include <cstdlib>
 
+#include <chrono>
+
 #include "algo.hpp"
 #include "edge_map.hpp"
 #include "primitives.hpp"
@@ -408,6 +410,7 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   uint32_t L = 0;
   uint8_t *flags = nullptr;
   const bool verbose = std::getenv("SSTG_VERBOSE") != nullptr;
+  auto t_level = std::chrono::steady_clock::now();
   while (nf > 0) {
     uint64_t unexplored = g->m > explored ? g->m - explored : 0;
     bool want_bu = !top_down_only &&
@@ -437,10 +440,17 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                                queue + qs + nf, &res));
       bottom_up = false;
     }
-    if (verbose)
-      std::fprintf(stderr, "bfs level %u %s nf=%llu mf=%llu -> next %llu\n", L,
+    if (verbose) {  // diagnostics: per-level wall time (synchronises)
+      SSTG_HIP(hipStreamSynchronize(s));
+      auto t1 = std::chrono::steady_clock::now();
+      std::fprintf(stderr,
+                   "bfs level %u %s nf=%llu mf=%llu -> next %llu  %.3f ms\n", L,
                    want_bu ? "pull" : "push", (unsigned long long)nf,
-                   (unsigned long long)mf, (unsigned long long)res.count);
+                   (unsigned long long)mf, (unsigned long long)res.count,
+                   std::chrono::duration<double, std::milli>(t1 - t_level)
+                       .count());
+      t_level = t1;
+    }
     qs += nf;
     nf = res.count;
     mf = res.degree_sum;

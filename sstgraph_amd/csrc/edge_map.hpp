@@ -31,6 +31,11 @@ struct EdgeMapCtx {
   uint64_t fcap = 0;
   int slot = 0;
   unsigned long long *ctr = nullptr;  // [0] count, [1] degree sum
+  // bottom-up active lists per degree bin (double-buffered); act_src < 0
+  // until the first bottom-up step
+  uint32_t *act[4][2] = {};
+  uint64_t act_n[4] = {0, 0, 0, 0};
+  int act_src = -1;
 };
 
 namespace em {
@@ -38,6 +43,7 @@ namespace {  // internal linkage: this header is included by several files
 
 constexpr int kT = 256;
 constexpr int kChunk = 8;
+constexpr int kBitmapRounds = 8;  // 512 vertices per wave in k_bitmap_from
 
 __global__ void k_clear_u32(uint32_t *p, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
@@ -168,33 +174,103 @@ __global__ void __launch_bounds__(kT)
 
 // Output frontier bitmap of a dense step: bit v = f.in_output(v); one lane
 // per vertex, a ballot per 64 vertices, no atomics.
-template <class F>
-__global__ void k_bitmap_from(uint32_t n, F f, uint32_t *bm, uint64_t words) {
-  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  bool in = v < n && f.in_output((uint32_t)v);
-  uint64_t m = __ballot(in);
-  uint32_t lane = lane_id();
-  uint64_t w = v >> 5;
-  if ((lane & 31) == 0 && w < words)
-    bm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+// Per-block count and degree sum stored at part[2 * block] (no atomics:
+// many blocks adding to one address serialise at the L2).
+__device__ __forceinline__ void block_partial(uint64_t cnt, uint64_t dsum,
+                                              unsigned long long *part) {
+  __shared__ unsigned long long bp[2][kT / kWave];
+  cnt = wave_sum(cnt);
+  dsum = wave_sum(dsum);
+  if (lane_id() == 0) {
+    bp[0][threadIdx.x / kWave] = cnt;
+    bp[1][threadIdx.x / kWave] = dsum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long c = 0, d = 0;
+    for (int w = 0; w < kT / kWave; w++) {
+      c += bp[0][w];
+      d += bp[1][w];
+    }
+    part[2 * (uint64_t)blockIdx.x] = c;
+    part[2 * (uint64_t)blockIdx.x + 1] = d;
+  }
 }
 
-// Dense pull over the vertex list list[0..count) with G lanes per vertex
-// (G divides 64).  A vertex whose cond holds scans its neighbours G at a
-// time and stops at the first chunk that holds a frontier member; the lowest
-// such lane is the source.
-template <class F, int G>
+// ctr[0..1] = the sums of the nb partial pairs (one block)
+__global__ void __launch_bounds__(kT)
+    k_sum_partials(const unsigned long long *part, uint64_t nb,
+                   unsigned long long *ctr) {
+  uint64_t c = 0, d = 0;
+  for (uint64_t b = threadIdx.x; b < nb; b += kT) {
+    c += part[2 * b];
+    d += part[2 * b + 1];
+  }
+  __shared__ unsigned long long bp[2][kT / kWave];
+  c = wave_sum(c);
+  d = wave_sum(d);
+  if (lane_id() == 0) {
+    bp[0][threadIdx.x / kWave] = c;
+    bp[1][threadIdx.x / kWave] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tc = 0, td = 0;
+    for (int w = 0; w < kT / kWave; w++) {
+      tc += bp[0][w];
+      td += bp[1][w];
+    }
+    ctr[0] = tc;
+    ctr[1] = td;
+  }
+}
+
+template <class F>
+__global__ void __launch_bounds__(kT)
+    k_bitmap_from(uint32_t n, F f, uint32_t *bm, uint64_t words, DevGraph g,
+                  unsigned long long *part) {
+  // kBitmapRounds words of 64 vertices per wave; the output vertices are
+  // counted (and their degrees summed) per block into part
+  uint64_t cnt = 0, dsum = 0;
+  const uint32_t lane = lane_id();
+  const uint64_t wave = ((uint64_t)blockIdx.x * kT + threadIdx.x) / kWave;
+  for (int r = 0; r < kBitmapRounds; r++) {
+    const uint64_t v = (wave * kBitmapRounds + r) * kWave + lane;
+    const bool in = v < n && f.in_output((uint32_t)v);
+    const uint64_t m = __ballot(in);
+    if (in) {
+      cnt++;
+      dsum += g.deg[v];
+    }
+    const uint64_t w = v >> 5;
+    if ((lane & 31) == 0 && w < words)
+      bm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+  }
+  block_partial(cnt, dsum, part);
+}
+
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;  // ids are below n_rows < 2^32
+
+// Bottom-up step over the active vertices list[0..count) with G lanes per
+// vertex (G divides 64).  A vertex whose cond holds tests one neighbour per
+// lane first, then U per lane per round: the U row elements, then their U
+// bitmap words, so each lane has U gathers in flight before it tests any of
+// them.  The first round with a frontier member ends the scan and the lowest
+// such lane's neighbour is the source.  keep[i] = 1 marks a vertex with
+// neighbours that found none: it stays active for the next bottom-up step.
+// No atomics: the counts come from the bitmap pass and the active list from
+// a compaction of keep.
+template <class F, int G, int U>
 __global__ void __launch_bounds__(kT)
     k_dense_pull(DevGraph g, const uint32_t *list, uint64_t count, F f,
-                 const uint32_t *bm_cur, uint32_t *bm_next,
-                 unsigned long long *ctr) {
+                 const uint32_t *bm_cur, uint8_t *keep) {
   const uint64_t gid =
       (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kT + threadIdx.x;
   const uint32_t lig = (uint32_t)(gid % G);
   const uint64_t i = gid / G;
   const uint32_t glane0 = lane_id() & ~(uint32_t)(G - 1);
   const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << glane0);
-  bool found = false;
+  bool found = false, active = false;
   uint32_t dv = 0;
   uint32_t v = 0;
   if (i < count) {
@@ -202,18 +278,27 @@ __global__ void __launch_bounds__(kT)
     if (f.cond(v)) {
       dv = g.deg[v];
       const uint32_t *row = g.pool + g.off[v];
-      for (uint32_t base = 0; base < dv; base += G) {
-        uint32_t k = base + lig;
-        bool hit = false;
-        uint32_t u = 0;
-        if (k < dv) {
-          u = row[k];
-          hit = (bm_cur[u >> 5] >> (u & 31)) & 1u;
+      // the first round tests one neighbour per lane (most unvisited
+      // vertices meet the frontier among their first neighbours), later
+      // rounds U per lane
+      for (uint32_t base = 0, uu = 1; base < dv; base += G * uu, uu = U) {
+        uint32_t u[U], w[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+          const uint32_t k = base + lig + j * G;
+          u[j] = ((uint32_t)j < uu && k < dv) ? row[k] : kNoVertex;
         }
-        uint64_t m = __ballot(hit) & gmask;
+#pragma unroll
+        for (int j = 0; j < U; j++)
+          w[j] = u[j] != kNoVertex ? bm_cur[u[j] >> 5] : 0u;
+        uint32_t mine = kNoVertex;
+#pragma unroll
+        for (int j = U - 1; j >= 0; j--)
+          if (u[j] != kNoVertex && ((w[j] >> (u[j] & 31)) & 1u)) mine = u[j];
+        const uint64_t m = __ballot(mine != kNoVertex) & gmask;
         if (m) {
-          uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
-          uint32_t su = __shfl(u, leader, kWave);
+          const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+          const uint32_t su = __shfl(mine, leader, kWave);
           if (lig == 0) {
             f.update(su, v);
             found = true;
@@ -221,43 +306,54 @@ __global__ void __launch_bounds__(kT)
           break;
         }
       }
+      active = !found && dv > 0;
     }
+    if (lig == 0) keep[i] = active ? 1 : 0;
   }
-  bool take = found && lig == 0;
-  block_count(take ? 1u : 0u, take ? dv : 0u, ctr);
 }
 
-// Dense pull with a whole block per (high-degree) vertex: the block scans
-// 256 neighbours per round and stops after the first round with a hit.
-template <class F>
+// Bottom-up step with a whole block per (high-degree) vertex: 256 x U
+// neighbours per round, all loads of a round issued before their tests.
+template <class F, int U>
 __global__ void __launch_bounds__(kT)
     k_dense_pull_block(DevGraph g, const uint32_t *list, uint64_t count, F f,
-                       const uint32_t *bm_cur, uint32_t *bm_next,
-                       unsigned long long *ctr) {
+                       const uint32_t *bm_cur, uint8_t *keep) {
   __shared__ uint32_t first_hit;
-  const uint32_t v = list[xcd_block(blockIdx.x, gridDim.x)];
-  if (!f.cond(v)) return;  // uniform across the block
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t v = list[i];
+  if (!f.cond(v)) {  // uniform across the block
+    if (threadIdx.x == 0) keep[i] = 0;
+    return;
+  }
   const uint32_t dv = g.deg[v];
   const uint32_t *row = g.pool + g.off[v];
-  if (threadIdx.x == 0) first_hit = 0xFFFFFFFFu;
+  if (threadIdx.x == 0) first_hit = kNoVertex;
   __syncthreads();
-  for (uint32_t base = 0; base < dv; base += kT) {
-    uint32_t k = base + threadIdx.x;
-    if (k < dv) {
-      uint32_t u = row[k];
-      if ((bm_cur[u >> 5] >> (u & 31)) & 1u) atomicMin(&first_hit, k);
+  for (uint32_t base = 0, uu = 1; base < dv; base += kT * uu, uu = U) {
+    uint32_t u[U], w[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const uint32_t k = base + threadIdx.x + j * kT;
+      u[j] = ((uint32_t)j < uu && k < dv) ? row[k] : kNoVertex;
     }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+      w[j] = u[j] != kNoVertex ? bm_cur[u[j] >> 5] : 0u;
+#pragma unroll
+    for (int j = 0; j < U; j++)
+      if (u[j] != kNoVertex && ((w[j] >> (u[j] & 31)) & 1u))
+        atomicMin(&first_hit, base + threadIdx.x + j * kT);
     __syncthreads();
-    uint32_t h = first_hit;
-    if (h != 0xFFFFFFFFu) {
+    const uint32_t h = first_hit;
+    if (h != kNoVertex) {
       if (threadIdx.x == 0) {
         f.update(row[h], v);
-        atomicAdd(&ctr[0], 1ull);
-        atomicAdd(&ctr[1], (unsigned long long)dv);
+        keep[i] = 0;
       }
       return;
     }
   }
+  if (threadIdx.x == 0) keep[i] = dv > 0 ? 1 : 0;
 }
 
 // ------------------------------------------------------------ vertex pull
@@ -426,32 +522,68 @@ int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
   hipStream_t s = g->stream;
   DegreeBins bins;
   SSTG_TRY(degree_bins(g, &bins));
-  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
+  // the first bottom-up step reads the degree bins; later ones the vertices
+  // the previous step left unvisited
+  const bool first = ctx.act_src < 0;
+  const int dst = first ? 0 : 1 - ctx.act_src;
+  for (int b = 0; b < 4; b++) {
+    const uint64_t cap = bins.count[b] ? bins.count[b] : 1;
+    SSTG_TRY(g->ws.get(kWsBottomUp + 2 * b + 0, cap, &ctx.act[b][0]));
+    SSTG_TRY(g->ws.get(kWsBottomUp + 2 * b + 1, cap, &ctx.act[b][1]));
+  }
+  // ctr[0..1]: found count and degree sum; ctr[3 + b]: active count of bin b
+  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 7 * sizeof(uint64_t), s));
+  uint8_t *keep = nullptr;
+  SSTG_TRY(g->ws.get(kWsBottomUp + 8, bins.count[0] + bins.count[1] +
+                                          bins.count[2] + bins.count[3] + 4,
+                     &keep));
   DevGraph v = g->view();
   const uint32_t *cur = ctx.bm_cur;
-  if (bins.count[0])
-    SSTG_LAUNCH((em::k_dense_pull<F, 1>), ceil_div(bins.count[0], em::kT),
-                em::kT, 0, s, v, (const uint32_t *)bins.list[0], bins.count[0],
-                f, cur, ctx.bm_next, ctx.ctr);
-  if (bins.count[1])
-    SSTG_LAUNCH((em::k_dense_pull<F, 8>), ceil_div(bins.count[1] * 8, em::kT),
-                em::kT, 0, s, v, (const uint32_t *)bins.list[1], bins.count[1],
-                f, cur, ctx.bm_next, ctx.ctr);
-  if (bins.count[2])
-    SSTG_LAUNCH((em::k_dense_pull<F, 64>),
-                ceil_div(bins.count[2] * 64, em::kT), em::kT, 0, s, v,
-                (const uint32_t *)bins.list[2], bins.count[2], f, cur,
-                ctx.bm_next, ctx.ctr);
-  if (bins.count[3])
-    SSTG_LAUNCH(em::k_dense_pull_block<F>, bins.count[3], em::kT, 0, s, v,
-                (const uint32_t *)bins.list[3], bins.count[3], f, cur,
-                ctx.bm_next, ctx.ctr);
-  SSTG_LAUNCH(em::k_bitmap_from<F>, ceil_div(ctx.words * 32, em::kT), em::kT,
-              0, s, g->n_rows, f, ctx.bm_next, ctx.words);
+  const uint32_t *src[4];
+  uint64_t cnt[4];
+  for (int b = 0; b < 4; b++) {
+    src[b] = first ? bins.list[b] : ctx.act[b][ctx.act_src];
+    cnt[b] = first ? bins.count[b] : ctx.act_n[b];
+  }
+  uint8_t *kb[4];
+  kb[0] = keep;
+  for (int b = 1; b < 4; b++) kb[b] = kb[b - 1] + bins.count[b - 1] + 1;
+  if (cnt[0])
+    SSTG_LAUNCH((em::k_dense_pull<F, 1, 8>), ceil_div(cnt[0], em::kT), em::kT,
+                0, s, v, src[0], cnt[0], f, cur, kb[0]);
+  if (cnt[1])
+    SSTG_LAUNCH((em::k_dense_pull<F, 8, 8>), ceil_div(cnt[1] * 8, em::kT),
+                em::kT, 0, s, v, src[1], cnt[1], f, cur, kb[1]);
+  if (cnt[2])
+    SSTG_LAUNCH((em::k_dense_pull<F, 64, 4>), ceil_div(cnt[2] * 64, em::kT),
+                em::kT, 0, s, v, src[2], cnt[2], f, cur, kb[2]);
+  if (cnt[3])
+    SSTG_LAUNCH((em::k_dense_pull_block<F, 4>), cnt[3], em::kT, 0, s, v,
+                src[3], cnt[3], f, cur, kb[3]);
+  for (int b = 0; b < 4; b++)
+    SSTG_TRY(select_flagged_u32(kb[b], src[b], cnt[b], ctx.act[b][dst],
+                                (uint64_t *)(ctx.ctr + 3 + b), g->ws, kWsPrim,
+                                s));
+  const uint64_t bm_waves = ceil_div(ctx.words * 32, (uint64_t)kWave);
+  const uint64_t bm_blocks =
+      ceil_div(bm_waves, (uint64_t)(em::kT / kWave) * em::kBitmapRounds);
+  unsigned long long *part = nullptr;
+  SSTG_TRY(g->ws.get(kWsBottomUp + 9, 2 * bm_blocks, &part));
+  SSTG_LAUNCH(em::k_bitmap_from<F>, bm_blocks, em::kT, 0, s, g->n_rows, f,
+              ctx.bm_next, ctx.words, v, part);
+  SSTG_LAUNCH(em::k_sum_partials, 1, em::kT, 0, s,
+              (const unsigned long long *)part, bm_blocks, ctx.ctr);
   uint32_t *t = ctx.bm_cur;
   ctx.bm_cur = ctx.bm_next;
   ctx.bm_next = t;
-  return read_result(g, ctx, res);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctx.ctr, 7 * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  res->count = g->host_rb[0];
+  res->degree_sum = g->host_rb[1];
+  for (int b = 0; b < 4; b++) ctx.act_n[b] = g->host_rb[3 + b];
+  ctx.act_src = dst;
+  return SSTG_OK;
 }
 
 template <class F>

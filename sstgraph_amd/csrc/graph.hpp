@@ -113,7 +113,9 @@ enum : int {
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range
-  kWsSlots = 112,
+  kWsBottomUp = 112,  // 112..119 active lists of the bottom-up steps,
+                      // 120 their keep flags, 121 per-block partial sums
+  kWsSlots = 122,
 };
 
 }  // namespace sstg

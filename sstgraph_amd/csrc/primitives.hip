@@ -124,9 +124,10 @@ __global__ void __launch_bounds__(kScanThreads)
   }
 }
 
+template <class V>
 __global__ void __launch_bounds__(kScanThreads)
-    k_select(const uint8_t *flags, const uint64_t *values, uint64_t n,
-             const uint64_t *offsets, uint64_t *out) {
+    k_select(const uint8_t *flags, const V *values, uint64_t n,
+             const uint64_t *offsets, V *out) {
   __shared__ uint64_t wtot[kScanThreads / kWave];
   uint64_t base = (uint64_t)blockIdx.x * kScanTile;
   uint64_t carry = offsets[blockIdx.x];
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(kScanThreads)
     uint64_t f = (i < n && flags[i]) ? 1 : 0;
     uint64_t tot;
     uint64_t ex = block_exclusive(f, wtot, &tot);
-    if (f) out[carry + ex] = values ? values[i] : i;
+    if (f) out[carry + ex] = values ? values[i] : (V)i;
     carry += tot;
   }
 }
@@ -275,9 +276,10 @@ int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
   return scan_impl<uint32_t>(in, out, n, d_total, ws, ws_base, s);
 }
 
-int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
-                       uint64_t n, uint64_t *out, uint64_t *d_count,
-                       Workspace &ws, int ws_base, hipStream_t s) {
+template <class V>
+static int select_flagged(const uint8_t *flags, const V *values, uint64_t n,
+                          V *out, uint64_t *d_count, Workspace &ws,
+                          int ws_base, hipStream_t s) {
   if (n == 0) {
     SSTG_HIP(hipMemsetAsync(d_count, 0, sizeof(uint64_t), s));
     return SSTG_OK;
@@ -288,9 +290,23 @@ int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
   SSTG_LAUNCH(k_count_flags, tiles, kScanThreads, 0, s, flags, n, counts);
   SSTG_TRY(scan_exclusive_u64(counts, counts, tiles, d_count, ws, ws_base + 1,
                               s));
-  SSTG_LAUNCH(k_select, tiles, kScanThreads, 0, s, flags, values, n,
+  SSTG_LAUNCH(k_select<V>, tiles, kScanThreads, 0, s, flags, values, n,
               (const uint64_t *)counts, out);
   return SSTG_OK;
+}
+
+int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
+                       uint64_t n, uint64_t *out, uint64_t *d_count,
+                       Workspace &ws, int ws_base, hipStream_t s) {
+  return select_flagged<uint64_t>(flags, values, n, out, d_count, ws, ws_base,
+                                  s);
+}
+
+int select_flagged_u32(const uint8_t *flags, const uint32_t *values,
+                       uint64_t n, uint32_t *out, uint64_t *d_count,
+                       Workspace &ws, int ws_base, hipStream_t s) {
+  return select_flagged<uint32_t>(flags, values, n, out, d_count, ws, ws_base,
+                                  s);
 }
 
 int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
