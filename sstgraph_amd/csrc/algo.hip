@@ -9,6 +9,7 @@
 #include <chrono>
 
 #include "algo.hpp"
+#include "cc.hpp"
 #include "edge_map.hpp"
 #include "primitives.hpp"
 
@@ -90,57 +91,10 @@ __global__ void k_pr_init(uint32_t n, const uint32_t *deg, Real x0, Real *rank,
 }
 
 // ------------------------------------------------------------------ CC
-__device__ __forceinline__ uint32_t uf_find(uint32_t *p, uint32_t x) {
-  uint32_t y = p[x];
-  while (y != x) {
-    uint32_t z = p[y];
-    if (z != y) p[x] = z;  // path halving; only ever lowers p[x]
-    x = y;
-    y = z;
-  }
-  return x;
-}
-
-__global__ void k_cc_init(DevGraph g, uint32_t *p) {
-  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (v >= g.n) return;
-  uint32_t d = g.deg[v];
-  uint32_t m = (uint32_t)v;
-  if (d) {
-    uint32_t first = g.pool[g.off[v]];  // rows are sorted: first = min
-    if (first < m) m = first;
-  }
-  p[v] = m;
-}
-
-// Hooks the larger root under the smaller one for every stored edge, flat
-// over edges (8 per thread).  p[x] <= x holds throughout, so each final root
-// is the smallest id of its component.
-__global__ void k_cc_hook(DevGraph g, const uint64_t *prefix, uint64_t m,
-                          uint32_t *p) {
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * 8;
-  if (e >= m) return;
-  uint64_t end = e + 8 < m ? e + 8 : m;
-  uint64_t v = find_segment(prefix, g.n, e);
-  for (; e < end; e++) {
-    while (e >= prefix[v + 1]) v++;
-    uint32_t u = (uint32_t)v;
-    uint32_t w = g.pool[g.off[v] + (e - prefix[v])];
-    uint32_t ru = uf_find(p, u), rw = uf_find(p, w);
-    while (ru != rw) {
-      uint32_t hi = ru > rw ? ru : rw, lo = ru > rw ? rw : ru;
-      uint32_t old = atomicCAS(&p[hi], hi, lo);
-      if (old == hi) break;
-      ru = uf_find(p, old);
-      rw = uf_find(p, lo);
-    }
-  }
-}
-
 __global__ void k_cc_final(uint32_t n, uint32_t *p, uint32_t *out) {
   uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (v >= n) return;
-  out[v] = uf_find(p, (uint32_t)v);
+  out[v] = cc::find(p, (uint32_t)v);
 }
 
 // ------------------------------------------------------------------ TC
@@ -530,12 +484,26 @@ int algo_cc(sstg_graph *g, uint32_t *out) {
   if (n == 0) return SSTG_OK;
   const uint64_t *prefix = nullptr;
   SSTG_TRY(graph_degree_prefix(g, &prefix));
-  uint32_t *p = nullptr;
+  uint32_t *p = nullptr, *giant = nullptr;
   SSTG_TRY(g->ws.get(kWsAlgo + 22, n, &p));
-  SSTG_LAUNCH(k_cc_init, blocks(n), kT, 0, s, g->view(), p);
-  if (g->m)
-    SSTG_LAUNCH(k_cc_hook, ceil_div(ceil_div(g->m, 8), kT), kT, 0, s,
-                g->view(), prefix, g->m, p);
+  SSTG_TRY(g->ws.get(kWsAlgo + 30, 1, &giant));
+  bool symmetric = false;
+  SSTG_TRY(graph_is_symmetric(g, &symmetric));
+  const DevGraph v = g->view();
+  if (g->m && symmetric) {
+    SSTG_LAUNCH(cc::k_identity, blocks(n), kT, 0, s, n, p);
+    SSTG_LAUNCH(cc::k_link_sample, blocks(n), kT, 0, s, v, p);
+    SSTG_LAUNCH(cc::k_compress, blocks(n), kT, 0, s, n, p);
+    SSTG_LAUNCH(cc::k_mode, 1, kT, 0, s, (const uint32_t *)nullptr, 1u, n, p,
+                giant);
+    SSTG_LAUNCH(cc::k_link_rest, blocks(n), kT, 0, s, v, p,
+                (const uint32_t *)giant, (const uint32_t *)nullptr);
+  } else {
+    SSTG_LAUNCH(cc::k_init, blocks(n), kT, 0, s, v, p);
+    if (g->m)
+      SSTG_LAUNCH(cc::k_hook_all, ceil_div(ceil_div(g->m, 8), kT), kT, 0, s, v,
+                prefix, g->m, p);
+  }
   SSTG_LAUNCH(k_cc_final, blocks(n), kT, 0, s, n, p, out);
   SSTG_HIP(hipStreamSynchronize(s));
   return SSTG_OK;

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "algo.hpp"
+#include "cc.hpp"
 #include "edge_map.hpp"
 #include "primitives.hpp"
 
@@ -33,7 +34,7 @@ struct sstg_multi {
   int exec_f32 = -1;
   // outputs of the captured run (device)
   uint32_t *level = nullptr, *label = nullptr, *sources = nullptr,
-           *uf = nullptr, *counts = nullptr;
+           *uf = nullptr, *counts = nullptr, *giant = nullptr;
   double *rank64 = nullptr, *c64a = nullptr, *c64b = nullptr;
   float *rank32 = nullptr, *c32a = nullptr, *c32b = nullptr;
 };
@@ -89,75 +90,43 @@ __global__ void k_mbfs_init(uint32_t n, const uint32_t *vbase,
   if (v < max_levels) counts[v] = (v == 0) ? 1u : 0u;  // counts[0] != 0
 }
 
-// Level L: every vertex of level L claims its unvisited neighbours.  Exits at
-// once when level L is empty (counts[L] == 0 after level L-1 ran).
-__global__ void k_mbfs_level(DevGraph g, uint32_t L, uint32_t *level,
-                             uint32_t *counts) {
+// Level L: every vertex of level L claims its unvisited neighbours, with G
+// lanes per vertex for the vertices of one degree bin.  counts[L + 1] is set
+// non-zero once a vertex of level L + 1 is found (read first, so only the
+// first few waves store to it); every launch exits at once when level L is
+// empty.
+template <int G>
+__global__ void __launch_bounds__(kT)
+    k_mbfs_level(DevGraph g, const uint32_t *list, uint64_t count, uint32_t L,
+                 uint32_t *level, uint32_t *counts) {
   if (counts[L] == 0) return;
-  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  uint32_t found = 0;
-  if (v < g.n && level[v] == L) {
-    uint32_t d = g.deg[v];
-    const uint32_t *row = g.pool + g.off[v];
-    for (uint32_t k = 0; k < d; k++) {
-      uint32_t w = row[k];
-      if (level[w] == kInf && atomicCAS(&level[w], kInf, L + 1) == kInf)
-        found++;
+  const uint64_t gid = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  const uint64_t i = gid / G;
+  const uint32_t lig = (uint32_t)(gid % G);
+  bool found = false;
+  if (i < count) {
+    const uint32_t v = list[i];
+    if (level[v] == L) {
+      const uint32_t d = g.deg[v];
+      const uint32_t *row = g.pool + g.off[v];
+      for (uint32_t k = lig; k < d; k += G) {
+        const uint32_t w = row[k];
+        if (level[w] == kInf && atomicCAS(&level[w], kInf, L + 1) == kInf)
+          found = true;
+      }
     }
   }
-  found = (uint32_t)wave_sum((uint64_t)found);
-  if ((threadIdx.x & (kWave - 1)) == 0 && found) atomicAdd(&counts[L + 1], found);
+  if (__ballot(found) && lane_id() == 0 && counts[L + 1] == 0) counts[L + 1] = 1;
 }
 
-// ---- CC over the union: same union-find as the single-graph CC
-__device__ __forceinline__ uint32_t uf_find(uint32_t *p, uint32_t x) {
-  uint32_t y = p[x];
-  while (y != x) {
-    uint32_t z = p[y];
-    if (z != y) p[x] = z;
-    x = y;
-    y = z;
-  }
-  return x;
-}
-
-__global__ void k_mcc_init(DevGraph g, uint32_t *p) {
-  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (v >= g.n) return;
-  uint32_t d = g.deg[v], m = (uint32_t)v;
-  if (d) {
-    uint32_t first = g.pool[g.off[v]];
-    if (first < m) m = first;
-  }
-  p[v] = m;
-}
-
-__global__ void k_mcc_hook(DevGraph g, const uint64_t *prefix, uint64_t m,
-                           uint32_t *p) {
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * 8;
-  if (e >= m) return;
-  uint64_t end = e + 8 < m ? e + 8 : m;
-  uint64_t v = find_segment(prefix, g.n, e);
-  for (; e < end; e++) {
-    while (e >= prefix[v + 1]) v++;
-    uint32_t w = g.pool[g.off[v] + (e - prefix[v])];
-    uint32_t ru = uf_find(p, (uint32_t)v), rw = uf_find(p, w);
-    while (ru != rw) {
-      uint32_t hi = ru > rw ? ru : rw, lo = ru > rw ? rw : ru;
-      uint32_t old = atomicCAS(&p[hi], hi, lo);
-      if (old == hi) break;
-      ru = uf_find(p, old);
-      rw = uf_find(p, lo);
-    }
-  }
-}
-
+// ---- CC over the union: the single-graph union-find kernels (cc.hpp)
+// with one large component per graph
 __global__ void k_mcc_final(uint32_t n, const uint32_t *vbase,
                             const uint32_t *graph_of, uint32_t *p,
                             uint32_t *label) {
   uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (v >= n) return;
-  label[v] = uf_find(p, (uint32_t)v) - vbase[graph_of[v]];
+  label[v] = cc::find(p, (uint32_t)v) - vbase[graph_of[v]];
 }
 
 // ---- PageRank over the union with per-graph normalisation
@@ -252,6 +221,31 @@ int record_pagerank(sstg_multi *M, const DegreeBins &bins, uint32_t iters,
 
 }  // namespace
 
+// One BFS level over the union: a launch per degree bin (1, 8, 64 or 256
+// lanes per vertex).
+static void launch_level(sstg_multi *M, const DegreeBins &bins, uint32_t L,
+                         hipStream_t s) {
+  const DevGraph v = M->u->view();
+  if (bins.count[0])
+    hipLaunchKernelGGL(k_mbfs_level<1>, dim3(ceil_div(bins.count[0], kT)),
+                       dim3(kT), 0, s, v, (const uint32_t *)bins.list[0],
+                       bins.count[0], L, M->level, M->counts);
+  if (bins.count[1])
+    hipLaunchKernelGGL(k_mbfs_level<8>, dim3(ceil_div(bins.count[1] * 8, kT)),
+                       dim3(kT), 0, s, v, (const uint32_t *)bins.list[1],
+                       bins.count[1], L, M->level, M->counts);
+  if (bins.count[2])
+    hipLaunchKernelGGL(k_mbfs_level<64>,
+                       dim3(ceil_div(bins.count[2] * 64, kT)), dim3(kT), 0, s,
+                       v, (const uint32_t *)bins.list[2], bins.count[2], L,
+                       M->level, M->counts);
+  if (bins.count[3])
+    hipLaunchKernelGGL(k_mbfs_level<256>,
+                       dim3(ceil_div(bins.count[3] * 256, kT)), dim3(kT), 0, s,
+                       v, (const uint32_t *)bins.list[3], bins.count[3], L,
+                       M->level, M->counts);
+}
+
 // Records BFS (levels 0..max_levels), CC and PR into one hipGraph.
 static int capture(sstg_multi *M, uint32_t max_levels, uint32_t iters,
                    double damping, bool f32) {
@@ -263,6 +257,8 @@ static int capture(sstg_multi *M, uint32_t max_levels, uint32_t iters,
   SSTG_TRY(graph_degree_prefix(g, &prefix));
   DegreeBins bins;
   SSTG_TRY(degree_bins(g, &bins));
+  bool symmetric = false;
+  SSTG_TRY(graph_is_symmetric(g, &symmetric));
   SSTG_HIP(hipStreamSynchronize(s));
   // the event profiler must not record into a capturing stream
   Profiler &prof = profiler();
@@ -284,15 +280,31 @@ static int capture(sstg_multi *M, uint32_t max_levels, uint32_t iters,
                        (const uint32_t *)M->d_graph_of,
                        (const uint32_t *)M->sources, M->level, M->counts,
                        kMultiCounts);
-    for (uint32_t L = 0; L < max_levels; L++)
-      hipLaunchKernelGGL(k_mbfs_level, dim3(blocks(n)), dim3(kT), 0, s,
-                         g->view(), L, M->level, M->counts);
+    for (uint32_t L = 0; L < max_levels; L++) launch_level(M, bins, L, s);
     // CC
-    hipLaunchKernelGGL(k_mcc_init, dim3(blocks(n)), dim3(kT), 0, s, g->view(),
-                       M->uf);
-    if (g->m)
-      hipLaunchKernelGGL(k_mcc_hook, dim3(ceil_div(ceil_div(g->m, 8), kT)),
-                         dim3(kT), 0, s, g->view(), prefix, g->m, M->uf);
+    const DevGraph v = g->view();
+    if (g->m && symmetric) {
+      hipLaunchKernelGGL(cc::k_identity, dim3(blocks(n)), dim3(kT), 0, s, n,
+                         M->uf);
+      hipLaunchKernelGGL(cc::k_link_sample, dim3(blocks(n)), dim3(kT), 0, s, v,
+                         M->uf);
+      hipLaunchKernelGGL(cc::k_compress, dim3(blocks(n)), dim3(kT), 0, s, n,
+                         M->uf);
+      hipLaunchKernelGGL(cc::k_mode,
+                         dim3(ceil_div((uint64_t)M->k * kWave, kT)), dim3(kT),
+                         0, s, (const uint32_t *)M->d_vbase, M->k, n, M->uf,
+                         M->giant);
+      hipLaunchKernelGGL(cc::k_link_rest, dim3(blocks(n)), dim3(kT), 0, s, v,
+                         M->uf, (const uint32_t *)M->giant,
+                         (const uint32_t *)M->d_graph_of);
+    } else {
+      hipLaunchKernelGGL(cc::k_init, dim3(blocks(n)), dim3(kT), 0, s, v,
+                         M->uf);
+      if (g->m)
+        hipLaunchKernelGGL(cc::k_hook_all,
+                           dim3(ceil_div(ceil_div(g->m, 8), kT)), dim3(kT), 0,
+                           s, v, prefix, g->m, M->uf);
+    }
     hipLaunchKernelGGL(k_mcc_final, dim3(blocks(n)), dim3(kT), 0, s, n,
                        (const uint32_t *)M->d_vbase,
                        (const uint32_t *)M->d_graph_of, M->uf, M->label);
@@ -363,6 +375,7 @@ int sstg_multi_create(uint32_t k, const uint32_t *n_per_graph,
       hipMalloc(&M->uf, n * sizeof(uint32_t)) ||
       hipMalloc(&M->sources, k * sizeof(uint32_t)) ||
       hipMalloc(&M->counts, kMultiCounts * sizeof(uint32_t)) ||
+      hipMalloc(&M->giant, k * sizeof(uint32_t)) ||
       hipMalloc(&M->rank64, n * sizeof(double)) ||
       hipMalloc(&M->c64a, n * sizeof(double)) ||
       hipMalloc(&M->c64b, n * sizeof(double)) ||
@@ -387,7 +400,8 @@ int sstg_multi_destroy(sstg_multi *M) {
   if (M->graph) hipGraphDestroy(M->graph);
   for (void *p : {(void *)M->d_vbase, (void *)M->d_graph_of, (void *)M->level,
                   (void *)M->label, (void *)M->uf, (void *)M->sources,
-                  (void *)M->counts, (void *)M->rank64, (void *)M->c64a,
+                  (void *)M->counts, (void *)M->giant, (void *)M->rank64,
+                  (void *)M->c64a,
                   (void *)M->c64b, (void *)M->rank32, (void *)M->c32a,
                   (void *)M->c32b})
     if (p) hipFree(p);
@@ -452,8 +466,10 @@ int sstg_multi_run(sstg_multi *M, const uint32_t *sources, uint32_t max_levels,
     SSTG_HIP(hipStreamSynchronize(g->stream));
     if (c == 0) break;
     if (L + 1 >= kMultiCounts) return SSTG_E_UNSUPPORTED;
-    SSTG_LAUNCH(k_mbfs_level, blocks(g->n_rows), kT, 0, g->stream, g->view(),
-                L, M->level, M->counts);
+    DegreeBins bins;
+    SSTG_TRY(degree_bins(g, &bins));
+    launch_level(M, bins, L, g->stream);
+    SSTG_HIP(hipGetLastError());
     L++;
   }
   M->levels_run = L;

@@ -194,3 +194,38 @@ def test_symmetry_tracking_through_updates():
     g.remove_batch(np.concatenate([s[:100], d[:100]]), np.concatenate([d[:100], s[:100]]))
     og.remove_batch(np.concatenate([s[:100], d[:100]]), np.concatenate([d[:100], s[:100]]))
     check("symmetric removal")
+
+
+def test_cc_many_components():
+    """A large rMat component next to cliques, paths, stars, isolated
+    vertices and self-loops: the sampled large component is skipped after
+    the first links, every other vertex links all of its edges, and the
+    labels (smallest id per component) equal the oracle's."""
+    rng = np.random.default_rng(31)
+    n = 1 << 16
+    s, d = O.rmat(1 << 15, 77, 8 << 15)      # ids below 2^15
+    parts_s, parts_d = [s], [d]
+    base = 1 << 15
+    for k in range(200):                     # small components above 2^15
+        size = int(rng.integers(2, 40))
+        ids = base + np.arange(size, dtype=np.uint32)
+        base += size
+        if k % 3 == 0:                       # clique
+            a, b = np.triu_indices(size, 1)
+            parts_s.append(ids[a]); parts_d.append(ids[b])
+        elif k % 3 == 1:                     # path
+            parts_s.append(ids[:-1]); parts_d.append(ids[1:])
+        else:                                # star
+            parts_s.append(np.full(size - 1, ids[0], np.uint32)); parts_d.append(ids[1:])
+    S_ = np.concatenate(parts_s).astype(np.uint32)
+    D_ = np.concatenate(parts_d).astype(np.uint32)
+    S2, D2 = np.concatenate([S_, D_]), np.concatenate([D_, S_])
+    off, cols = csr_from_edges(n, S2, D2)
+    g = _device_graph(n, off, cols)
+    g.insert(n - 1, n - 1)                   # a self-loop on an isolated vertex
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+    og.insert(n - 1, n - 1)
+    o_off, o_cols = og.csr()
+    assert np.array_equal(EMV.CC(g), O.cc(o_off, o_cols))
+

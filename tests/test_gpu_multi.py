@@ -12,7 +12,7 @@ from tests.gpu_common import need_gpu, rel_close, say
 pytestmark = pytest.mark.gpu
 
 
-def _make(k, seed):
+def _make(k, seed, symmetric=True):
     rng = np.random.default_rng(seed)
     graphs = []
     for g in range(k):
@@ -22,7 +22,8 @@ def _make(k, seed):
             s, d = O.rmat(n, 1000 + g, int(rng.integers(8, 17)) * n)
         else:
             s, d = O.er(n, 2000 + g, n * int(rng.integers(2, 9)))
-        s, d = np.concatenate([s, d]), np.concatenate([d, s])
+        if symmetric:
+            s, d = np.concatenate([s, d]), np.concatenate([d, s])
         og = O.Graph(n)
         og.insert_batch(s, d)
         off, cols = og.csr()
@@ -53,6 +54,24 @@ def test_multi_matches_per_graph_oracle(max_levels):
             want = O.pagerank(off, cols, 10, 0.85)
             ok, w = rel_close(M.part(r["rank"], i), want, 1e-5 if f32 else 1e-9)
             assert ok, (i, w)
+
+
+def test_multi_directed_graphs():
+    """Rows without their reverses: BFS stays top-down and CC links every
+    stored edge (no large-component skip)."""
+    need_gpu()
+    graphs = _make(9, 8, symmetric=False)
+    M = S.MultiGraph([g[0] for g in graphs])
+    for i, (n, s, d, off, cols, src) in enumerate(graphs):
+        if len(s):
+            M.insert_batch(i, s, d)
+    M.run([g[5] for g in graphs], max_levels=64, pr_iters=10)
+    r = M.results()
+    for i, (n, s, d, off, cols, src) in enumerate(graphs):
+        assert np.array_equal(M.part(r["level"], i), O.bfs(off, cols, src)[0]), i
+        assert np.array_equal(M.part(r["label"], i), O.cc(off, cols)), i
+        ok, w = rel_close(M.part(r["rank"], i), O.pagerank(off, cols, 10, 0.85), 1e-9)
+        assert ok, (i, w)
 
 
 def test_multi_equals_single_graph_runs():
