@@ -247,10 +247,12 @@ def serve_trace(n, deg, count, rate, seed=4):
     return t, reqs
 
 
-def serve_once(S, g, t_arr, reqs, batches, slots, coalesce, snapshots):
+def serve_once(S, g, t_arr, reqs, batches, slots, coalesce, snapshots,
+               one_heavy=False):
     """Submits the trace (arrival offsets t_arr, or all at once when None)
     and returns latency and throughput figures."""
-    srv = S.Server(g, slots=slots, coalesce=coalesce, snapshots=snapshots)
+    srv = S.Server(g, slots=slots, coalesce=coalesce, snapshots=snapshots,
+                   one_heavy=one_heavy)
     tickets = []
     t0 = time.perf_counter()
     for k, (kind, arg) in enumerate(reqs):
@@ -297,16 +299,18 @@ def serve_once(S, g, t_arr, reqs, batches, slots, coalesce, snapshots):
 SERVE_MODES = (
     ("serialised", dict(slots=1, coalesce=False, snapshots=False)),
     ("concurrent", dict(slots=4, coalesce=True, snapshots=False)),
-    ("served", dict(slots=4, coalesce=True, snapshots=True)),
+    ("snapshots", dict(slots=4, coalesce=True, snapshots=True)),
+    ("served", dict(slots=4, coalesce=True, snapshots=True, one_heavy=True)),
 )
 
 
 def run_serve(args):
     """Served stream on a device-resident rMat graph (scale --serve-scale,
-    seed 4, symmetrised).  Three servers see the same seeded trace:
-    the serialised plain-launch loop (one request at a time, one device
-    batch per update), concurrent slots with coalesced updates, and the
-    same with store snapshots (updates do not wait for running queries).
+    seed 4, symmetrised).  Four servers see the same seeded trace: the
+    serialised plain-launch loop (one request at a time, one device batch
+    per update), concurrent slots with coalesced updates, the same with
+    store snapshots (updates do not wait for queries), and the served mode
+    that also runs one PageRank/TC at a time while lighter requests pass.
     Each first takes the whole trace at once (capacity, requests/s), then
     the trace again with Poisson arrivals at 70% of the serialised loop's
     capacity (latency from arrival to result, p50/p99)."""

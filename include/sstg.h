@@ -189,11 +189,13 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
  * on `slots` slots (own HIP stream and workspace each), admitted in arrival
  * order while a slot is free and their device-memory estimate fits the HBM
  * budget (hbm_budget bytes, 0 = 7/8 of the free memory at creation).  With
- * SSTG_SERVE_SNAPSHOTS an update does not wait for running queries: the
- * graph moves to a device copy of its store and the queries finish on the
- * old arrays (the copy is held against the budget while they run). */
+ * SSTG_SERVE_SNAPSHOTS an update does not wait for running or queued
+ * queries: the graph moves to a device copy of its store and those queries
+ * run on the old arrays (held against the budget while they are needed). */
 #define SSTG_SERVE_COALESCE 1u
 #define SSTG_SERVE_SNAPSHOTS 2u
+/* at most one PageRank or TC at a time; lighter queries pass a waiting one */
+#define SSTG_SERVE_ONE_HEAVY 4u
 #define SSTG_REQ_INSERT 1u /* insert_batch (SM:556-643) */
 #define SSTG_REQ_REMOVE 2u /* remove_batch (SM:645-718) */
 #define SSTG_REQ_BFS 3u    /* BFS levels from `source` (TST:486) */

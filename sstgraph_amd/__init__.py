@@ -37,6 +37,7 @@ SSTG_E_IO = -7
 SSTG_DEVICE_PTRS = 0x1
 SSTG_SERVE_COALESCE = 0x1
 SSTG_SERVE_SNAPSHOTS = 0x2
+SSTG_SERVE_ONE_HEAVY = 0x4
 SSTG_SEMI_SORTED = 0x2
 SSTG_KEEP_SELF_LOOPS = 0x4
 
@@ -664,11 +665,13 @@ class Server:
     under an HBM budget; with snapshots, updates proceed on a copy of the
     store while earlier queries finish on the old one."""
 
-    def __init__(self, g, slots=4, hbm_budget=0, coalesce=True, snapshots=False):
+    def __init__(self, g, slots=4, hbm_budget=0, coalesce=True, snapshots=False,
+                 one_heavy=False):
         self.g = g
         h = ctypes.c_void_p()
         flags = (SSTG_SERVE_COALESCE if coalesce else 0) | \
-            (SSTG_SERVE_SNAPSHOTS if snapshots else 0)
+            (SSTG_SERVE_SNAPSHOTS if snapshots else 0) | \
+            (SSTG_SERVE_ONE_HEAVY if one_heavy else 0)
         _check(lib().sstg_server_create(g.handle, slots, hbm_budget, flags,
                                         ctypes.byref(h)),
                "sstg_server_create")

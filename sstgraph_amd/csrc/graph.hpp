@@ -110,6 +110,7 @@ enum : int {
   kWsRelTmp = 103,
   kWsRelSrc = 104,
   kWsRelDst = 105,
+  kWsTouch = 106,
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range

@@ -23,7 +23,13 @@
 //    own HIP stream, workspace and read-only view of the store), so queries
 //    that arrive while others are on the GPU overlap with them.  They are
 //    admitted in arrival order while a slot is free and their device memory
-//    estimate fits the HBM budget; otherwise they wait.
+//    estimate fits the HBM budget; otherwise they wait.  With
+//    SSTG_SERVE_ONE_HEAVY at most one PageRank or TC runs at a time, and
+//    lighter queries admitted behind a waiting one pass it (readers in any
+//    order see the same store).
+//  * With SSTG_SERVE_SNAPSHOTS a query still waiting when an update behind
+//    it could run is pinned to the current store arrays, so the update goes
+//    ahead on a copy and the query later runs on the version it arrived at.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -51,9 +57,13 @@ struct StoreSet {
   uint64_t *off = nullptr;
   uint32_t *deg = nullptr, *cap = nullptr, *pool = nullptr;
   uint64_t pool_cap = 0;  // elements
-  uint32_t readers = 0;
+  uint32_t readers = 0;   // running or pinned queries
   bool owned = false;
   uint64_t bytes = 0;     // device bytes held while owned
+  // the graph as it stood when the set was taken
+  uint32_t n_rows = 0, n_cols = 0;
+  uint64_t m = 0, pool_tail = 0, version = 0;
+  bool sym_known = false, symmetric = false;
 };
 
 struct Request {
@@ -85,6 +95,10 @@ bool is_update(uint32_t kind) {
   return kind == SSTG_REQ_INSERT || kind == SSTG_REQ_REMOVE;
 }
 
+bool is_heavy(uint32_t kind) {
+  return kind == SSTG_REQ_PR || kind == SSTG_REQ_TC;
+}
+
 }  // namespace
 }  // namespace sstg
 
@@ -94,8 +108,19 @@ struct sstg_server {
   uint64_t budget = 0, reserved = 0;
   int coalesce = 1;
   int snapshots = 0;
+  int one_heavy = 0;
+  uint32_t heavy_running = 0;
+  uint32_t owned_live = 0;  // store copies some query still reads
   sstg::StoreSet *live = nullptr;        // set the next admitted query reads
   std::vector<sstg::StoreSet> spare;     // recycled store arrays
+  std::vector<sstg::StoreSet> to_free;   // extra spares for the reaper
+  std::thread reaper;
+  // the update run on the writer thread (one at a time)
+  std::thread writer;
+  std::vector<sstg::Request *> write_run;
+  std::vector<uint32_t> write_src, write_dst;
+  bool writing = false;
+  uint64_t write_ticket = 0;
   std::multimap<uint64_t, void *> spare_results;  // bytes -> device buffer
   uint64_t detaches = 0;
   std::deque<sstg::Request *> pending;
@@ -158,19 +183,38 @@ int result_alloc(sstg_server *S, uint64_t bytes, void **out) {
 }
 
 // A finished query drops its read of a store set (caller holds S->mu).  A
-// retired set without readers becomes a spare (its bytes stay reserved);
-// spares beyond one per slot plus one are freed.
+// retired set without readers becomes a spare for the next store copy (its
+// bytes stay reserved); spares beyond one per slot plus two go to the
+// reaper thread, which frees them outside the lock -- hipFree waits for the
+// whole device, and the scheduler must not wait with it.
 void drop_reader(sstg_server *S, StoreSet *t) {
   if (!t) return;
   t->readers--;
   if (t->owned && t->readers == 0) {
+    S->owned_live--;
     S->spare.push_back(*t);
     delete t;
-    while (S->spare.size() > S->slots.size() + 1) {
-      S->reserved -= S->spare.front().bytes;
-      free_set_arrays(S->spare.front());
+    while (S->spare.size() > S->slots.size() + 2) {
+      S->to_free.push_back(S->spare.front());
       S->spare.erase(S->spare.begin());
+      S->cv.notify_all();
     }
+  }
+}
+
+void reaper_main(sstg_server *S) {
+  hipSetDevice(S->g->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    S->cv.wait(lk, [&] { return S->stop || !S->to_free.empty(); });
+    if (S->to_free.empty()) return;  // stopping with nothing left
+    StoreSet t = S->to_free.back();
+    S->to_free.pop_back();
+    lk.unlock();
+    free_set_arrays(t);
+    lk.lock();
+    S->reserved -= t.bytes;  // released only once actually freed
+    S->cv.notify_all();
   }
 }
 
@@ -217,6 +261,7 @@ int detach_store(sstg_server *S) {
                             hipMemcpyDeviceToDevice, s));
   // the old arrays now belong to the snapshot its readers hold
   t->owned = true;
+  S->owned_live++;
   t->bytes = store_bytes(n, t->pool_cap);
   S->reserved += t->bytes;
   g->off = b.off;
@@ -233,10 +278,65 @@ int detach_store(sstg_server *S) {
 bool update_may_start(sstg_server *S) {
   if (!S->live || S->live->readers == 0) return true;
   if (!S->snapshots) return false;
+  // at most one live store copy per slot: beyond that the update waits for
+  // readers to finish, as without snapshots
+  if (S->owned_live >= S->slots.size()) return false;
   const uint64_t need = store_bytes(S->g->n_rows, S->g->pool_cap);
-  const int k = find_spare(S);
+  int k = find_spare(S);
+  if (k < 0 && S->reserved + need > S->budget) {
+    // spares too small for the grown pool only hold the budget: free them
+    // (rare: hipFree waits for the device)
+    for (size_t i = 0; i < S->spare.size();) {
+      if (S->spare[i].pool_cap < S->g->pool_cap) {
+        S->reserved -= S->spare[i].bytes;
+        free_set_arrays(S->spare[i]);
+        S->spare.erase(S->spare.begin() + i);
+      } else {
+        i++;
+      }
+    }
+    k = find_spare(S);
+  }
   const uint64_t freed = k >= 0 ? S->spare[k].bytes : 0;
   return S->reserved - freed + need <= S->budget;
+}
+
+// The set the next admitted or pinned query reads (caller holds S->mu).
+StoreSet *live_set(sstg_server *S) {
+  if (!S->live) {
+    const sstg_graph *g = S->g;
+    StoreSet *t = new StoreSet();
+    t->off = g->off;
+    t->deg = g->deg;
+    t->cap = g->cap;
+    t->pool = g->pool;
+    t->pool_cap = g->pool_cap;
+    t->n_rows = g->n_rows;
+    t->n_cols = g->n_cols;
+    t->m = g->m;
+    t->pool_tail = g->pool_tail;
+    t->version = g->version;
+    t->sym_known = g->sym_version == g->version;
+    t->symmetric = g->symmetric;
+    S->live = t;
+  }
+  return S->live;
+}
+
+// a slot's view of the store as the set captured it
+void view_from_set(sstg_graph *v, const StoreSet *t) {
+  v->n_rows = t->n_rows;
+  v->n_cols = t->n_cols;
+  v->m = t->m;
+  v->off = t->off;
+  v->deg = t->deg;
+  v->cap = t->cap;
+  v->pool = t->pool;
+  v->pool_cap = t->pool_cap;
+  v->pool_tail = t->pool_tail;
+  v->version = t->version;  // the view's derived caches follow the version
+  v->sym_version = t->sym_known ? t->version : ~0ull;
+  v->symmetric = t->symmetric;
 }
 
 void refresh_view(sstg_graph *v, const sstg_graph *g) {
@@ -310,6 +410,7 @@ void slot_main(sstg_server *S, Slot *sl) {
       r->done = true;
       S->reserved -= r->est_bytes;
       S->running--;
+      if (is_heavy(r->spec.kind)) S->heavy_running--;
       drop_reader(S, r->set);
       r->set = nullptr;
       {
@@ -330,25 +431,17 @@ Slot *free_slot(sstg_server *S) {
   return nullptr;
 }
 
-// Applies the update run at the head of the queue (caller holds S->mu; the
-// lock is released while the device works).
-void apply_updates(sstg_server *S, std::unique_lock<std::mutex> &lk) {
+// Starts the update run at pending[at] on the writer thread (caller holds
+// S->mu).  Queries ahead of it are admitted or pinned to the current store,
+// so the graph can move to a copy of its arrays while they are read.
+void start_updates(sstg_server *S, size_t at) {
   std::vector<Request *> run;
-  const uint32_t kind = S->pending.front()->spec.kind;
-  while (!S->pending.empty() && S->pending.front()->spec.kind == kind &&
-         (S->coalesce || run.empty())) {
-    run.push_back(S->pending.front());
-    S->pending.pop_front();
-  }
-  uint64_t total = 0;
-  for (Request *r : run) total += r->src.size();
-  std::vector<uint32_t> src, dst;
-  src.reserve(total);
-  dst.reserve(total);
-  for (Request *r : run) {
-    src.insert(src.end(), r->src.begin(), r->src.end());
-    dst.insert(dst.end(), r->dst.begin(), r->dst.end());
-  }
+  const uint32_t kind = S->pending[at]->spec.kind;
+  size_t e = at;
+  while (e < S->pending.size() && S->pending[e]->spec.kind == kind &&
+         (S->coalesce || run.empty()))
+    run.push_back(S->pending[e++]);
+  S->pending.erase(S->pending.begin() + at, S->pending.begin() + e);
   const double t0 = now_s();
   for (Request *r : run) r->t_start = t0;
   if (S->live && S->live->readers) {
@@ -367,83 +460,133 @@ void apply_updates(sstg_server *S, std::unique_lock<std::mutex> &lk) {
     delete S->live;  // no readers: the update may move the graph's arrays
     S->live = nullptr;
   }
-  lk.unlock();
-  hipSetDevice(S->g->device);
-  int rc = graph_update(S->g, src.data(), dst.data(), total,
-                        kind == SSTG_REQ_INSERT,
-                        kind == SSTG_REQ_INSERT, false);
-  lk.lock();
-  const double t1 = now_s();
-  if (rc == SSTG_E_RANGE && run.size() > 1) {
-    // one request holds a bad id: apply them one by one so only it fails
-    lk.unlock();
-    std::vector<int> rcs;
-    for (Request *r : run)
-      rcs.push_back(graph_update(S->g, r->src.data(), r->dst.data(),
-                                 r->src.size(), kind == SSTG_REQ_INSERT,
-                                 kind == SSTG_REQ_INSERT, false));
-    lk.lock();
-    for (size_t i = 0; i < run.size(); i++) run[i]->status = rcs[i];
-  } else {
-    for (Request *r : run) r->status = rc;
-  }
+  uint64_t total = 0;
+  for (Request *r : run) total += r->src.size();
+  S->write_src.clear();
+  S->write_dst.clear();
+  S->write_src.reserve(total);
+  S->write_dst.reserve(total);
   for (Request *r : run) {
-    r->t_done = t1;
-    r->done = true;
-    r->coalesced = run.size();
-    r->src.clear();
-    r->src.shrink_to_fit();
-    r->dst.clear();
-    r->dst.shrink_to_fit();
+    S->write_src.insert(S->write_src.end(), r->src.begin(), r->src.end());
+    S->write_dst.insert(S->write_dst.end(), r->dst.begin(), r->dst.end());
   }
-  S->device_batches++;
-  S->done_cv.notify_all();
+  S->write_run = run;
+  S->write_ticket = run.front()->ticket;
+  S->writing = true;
+  S->cv.notify_all();
+}
+
+void writer_main(sstg_server *S) {
+  hipSetDevice(S->g->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    S->cv.wait(lk, [&] { return S->stop || !S->write_run.empty(); });
+    if (S->write_run.empty()) return;  // stopping
+    std::vector<Request *> run = S->write_run;
+    const uint32_t kind = run.front()->spec.kind;
+    const bool ins = kind == SSTG_REQ_INSERT;
+    lk.unlock();
+    int rc = graph_update(S->g, S->write_src.data(), S->write_dst.data(),
+                          S->write_src.size(), ins, ins, false);
+    std::vector<int> rcs(run.size(), rc);
+    if (rc == SSTG_E_RANGE && run.size() > 1) {
+      // one request holds a bad id: apply them one by one so only it fails
+      for (size_t i = 0; i < run.size(); i++)
+        rcs[i] = graph_update(S->g, run[i]->src.data(), run[i]->dst.data(),
+                              run[i]->src.size(), ins, ins, false);
+    }
+    lk.lock();
+    const double t1 = now_s();
+    for (size_t i = 0; i < run.size(); i++) {
+      Request *r = run[i];
+      r->status = rcs[i];
+      r->t_done = t1;
+      r->done = true;
+      r->coalesced = run.size();
+      r->src.clear();
+      r->src.shrink_to_fit();
+      r->dst.clear();
+      r->dst.shrink_to_fit();
+    }
+    S->write_run.clear();
+    S->writing = false;
+    S->device_batches++;
+    S->cv.notify_all();
+    S->done_cv.notify_all();
+  }
+}
+
+bool admissible(sstg_server *S, const Request *q) {
+  if (S->reserved + q->est_bytes > S->budget) return false;
+  if (S->one_heavy && is_heavy(q->spec.kind) && S->heavy_running) return false;
+  return free_slot(S) != nullptr;
+}
+
+void admit(sstg_server *S, Request *q) {
+  Slot *sl = free_slot(S);
+  S->reserved += q->est_bytes;
+  S->running++;
+  if (is_heavy(q->spec.kind)) S->heavy_running++;
+  q->t_start = now_s();
+  if (!q->set) {  // not pinned: reads the current store
+    q->set = live_set(S);
+    q->set->readers++;
+  }
+  view_from_set(&sl->view, q->set);
+  {
+    std::lock_guard<std::mutex> lk2(sl->mu);
+    sl->req = q;
+  }
+  sl->cv.notify_one();
+}
+
+// One pass over the queue in arrival order (caller holds S->mu): admits what
+// it can, pins what must wait, and starts the first update run once no query
+// ahead of it is left unpinned.  Returns whether anything changed.
+bool sched_step(sstg_server *S) {
+  bool update_waiting = false;
+  for (Request *r : S->pending)
+    if (is_update(r->spec.kind)) {
+      update_waiting = true;
+      break;
+    }
+  bool changed = false;
+  bool unpinned_ahead = false;
+  for (size_t i = 0; i < S->pending.size(); i++) {
+    Request *r = S->pending[i];
+    // while an update run is applied, only the queries ahead of it (all
+    // pinned to their store) may start
+    if (S->writing && r->ticket > S->write_ticket) return changed;
+    if (is_update(r->spec.kind)) {
+      if (!S->writing && !unpinned_ahead && update_may_start(S)) {
+        start_updates(S, i);
+        return true;
+      }
+      return changed;  // later queries must see this update
+    }
+    if (admissible(S, r)) {
+      S->pending.erase(S->pending.begin() + i);
+      admit(S, r);
+      i--;
+      changed = true;
+      continue;
+    }
+    if (S->snapshots && update_waiting && !r->set && !S->writing) {
+      r->set = live_set(S);  // pinned: the update behind it need not wait
+      r->set->readers++;
+      changed = true;
+    }
+    if (!r->set) unpinned_ahead = true;
+  }
+  return changed;
 }
 
 void sched_main(sstg_server *S) {
   hipSetDevice(S->g->device);
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
-    S->cv.wait(lk, [&] {
-      if (S->stop) return true;
-      if (S->pending.empty()) return false;
-      Request *h = S->pending.front();
-      if (is_update(h->spec.kind)) return update_may_start(S);
-      return S->reserved + h->est_bytes <= S->budget && free_slot(S);
-    });
     if (S->stop) return;
-    Request *h = S->pending.front();
-    if (is_update(h->spec.kind)) {
-      apply_updates(S, lk);
-      continue;
-    }
-    // admit queries in arrival order while slots and HBM allow
-    while (!S->pending.empty() && !is_update(S->pending.front()->spec.kind)) {
-      Request *q = S->pending.front();
-      if (S->reserved + q->est_bytes > S->budget) break;
-      Slot *sl = free_slot(S);
-      if (!sl) break;
-      S->pending.pop_front();
-      S->reserved += q->est_bytes;
-      S->running++;
-      q->t_start = now_s();
-      if (!S->live) {
-        S->live = new StoreSet();
-        S->live->off = S->g->off;
-        S->live->deg = S->g->deg;
-        S->live->cap = S->g->cap;
-        S->live->pool = S->g->pool;
-        S->live->pool_cap = S->g->pool_cap;
-      }
-      q->set = S->live;
-      S->live->readers++;
-      refresh_view(&sl->view, S->g);
-      {
-        std::lock_guard<std::mutex> lk2(sl->mu);
-        sl->req = q;
-      }
-      sl->cv.notify_one();
-    }
+    if (!sched_step(S)) S->cv.wait(lk);
   }
 }
 
@@ -481,7 +624,9 @@ extern "C" {
 int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
                        uint32_t flags, sstg_server **out) {
   if (!g || !out || slots == 0 || slots > 64) return SSTG_E_INVAL;
-  if (flags & ~(SSTG_SERVE_COALESCE | SSTG_SERVE_SNAPSHOTS)) return SSTG_E_INVAL;
+  if (flags & ~(SSTG_SERVE_COALESCE | SSTG_SERVE_SNAPSHOTS |
+                SSTG_SERVE_ONE_HEAVY))
+    return SSTG_E_INVAL;
   *out = nullptr;
   hipSetDevice(g->device);
   bool sym = false;
@@ -491,6 +636,7 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
   S->g = g;
   S->coalesce = (flags & SSTG_SERVE_COALESCE) ? 1 : 0;
   S->snapshots = (flags & SSTG_SERVE_SNAPSHOTS) ? 1 : 0;
+  S->one_heavy = (flags & SSTG_SERVE_ONE_HEAVY) ? 1 : 0;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
     delete S;
@@ -512,6 +658,8 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
   }
   for (Slot *sl : S->slots) sl->th = std::thread(slot_main, S, sl);
   S->sched = std::thread(sched_main, S);
+  S->reaper = std::thread(reaper_main, S);
+  S->writer = std::thread(writer_main, S);
   *out = S;
   return SSTG_OK;
 }
@@ -628,6 +776,8 @@ int sstg_server_destroy(sstg_server *S) {
   }
   S->cv.notify_all();
   if (S->sched.joinable()) S->sched.join();
+  if (S->reaper.joinable()) S->reaper.join();
+  if (S->writer.joinable()) S->writer.join();
   for (Slot *sl : S->slots) {
     {
       std::lock_guard<std::mutex> lk(sl->mu);
@@ -645,6 +795,7 @@ int sstg_server_destroy(sstg_server *S) {
   }
   for (auto &kv : S->spare_results) hipFree(kv.second);
   for (StoreSet &t : S->spare) free_set_arrays(t);
+  for (StoreSet &t : S->to_free) free_set_arrays(t);
   delete S->live;  // after the drain no query reads it; never owned
   delete S;
   return SSTG_OK;

@@ -22,11 +22,14 @@
 //
 // Counting stages out(u) in LDS as a linear-probing hash table (load <= 1/4)
 // and streams the lists out(v), v in out(u), through it, with the lanes spread
-// evenly over the concatenated lists.  Five launch tiers by |out(u)|:
-//   [2, 64], [65, 128], [129, 256], [257, 512]
+// evenly over the concatenated lists.  Six launch tiers by |out(u)|:
+//   [2, 16]    16 lanes per vertex, four vertices per wave
+//   [17, 64], [65, 128], [129, 256], [257, 512]
 //              one wave per vertex, 1.8 / 3.5 / 7 / 14 KB of LDS
 //   > 512      one block per vertex; out(u) is hashed in tiles of 4096 ids
 //              and every tile sees all of the lists out(v).
+#include <algorithm>
+
 #include "algo.hpp"
 #include "edge_map.hpp"
 #include "graph.hpp"
@@ -39,8 +42,8 @@ namespace {
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;  // never a vertex id (ids < n_rows)
 // wave tiers: |out(u)| upper bounds (powers of two, so that the table of
 // 4 * MaxOut slots holds every size table_bits() picks up to MaxOut)
-constexpr uint32_t kWaveTiers = 4;
-constexpr uint32_t kTierOut[kWaveTiers] = {64, 128, 256, 512};
+constexpr uint32_t kWaveTiers = 5;
+constexpr uint32_t kTierOut[kWaveTiers] = {16, 64, 128, 256, 512};
 constexpr uint32_t kMidOut = kTierOut[kWaveTiers - 1];
 constexpr uint32_t kBlkTile = 4096;
 constexpr uint32_t kBlkTable = 4 * kBlkTile;
@@ -154,44 +157,69 @@ __global__ void __launch_bounds__(256)
   if (lig == 0) dplus[u] = c;
 }
 
-// One wave (block of 64) per vertex u with 2 <= |out(u)| <= MaxOut.
-template <uint32_t MaxOut>
+// inclusive sum over groups of L lanes (L divides 64)
+template <uint32_t L>
+__device__ __forceinline__ uint32_t group_inclusive_sum(uint32_t v,
+                                                        uint32_t lig) {
+#pragma unroll
+  for (uint32_t o = 1; o < L; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, L);
+    if (lig >= o) v += t;
+  }
+  return v;
+}
+
+// L lanes per vertex u with 2 <= |out(u)| <= MaxOut, 64 / L vertices per
+// block of one wave, each group with its own LDS table: short lists keep
+// several vertices' setup loads in flight per wave.
+template <uint32_t MaxOut, uint32_t L>
 __global__ void __launch_bounds__(64)
     k_tc_count_wave(const uint64_t *lbase, const uint32_t *lists,
                     const uint32_t *llen, const uint32_t *list,
                     uint64_t count, unsigned long long *acc) {
   static_assert((MaxOut & (MaxOut - 1)) == 0, "table size must be 2^k");
-  __shared__ uint32_t table[4 * MaxOut];
-  __shared__ uint32_t pre[MaxOut + 1];
-  __shared__ uint64_t base[MaxOut];
-  const uint32_t lane = threadIdx.x;
+  static_assert(4 * MaxOut >= 64 && 64 % L == 0, "group shape");
+  constexpr uint32_t NG = 64 / L;
+  __shared__ uint32_t table[NG][4 * MaxOut];
+  __shared__ uint32_t pre[NG][MaxOut + 1];
+  __shared__ uint64_t base[NG][MaxOut];
+  const uint32_t lane = threadIdx.x, gi = lane / L, lig = lane % L;
+  uint32_t *tab = table[gi];
+  uint32_t *pr = pre[gi];
+  uint64_t *bs = base[gi];
   uint64_t found = 0;
-  for (uint64_t i = blockIdx.x; i < count; i += gridDim.x) {
-    const uint32_t u = list[i];
-    const uint32_t d = llen[u];
-    const uint32_t *row = lists + lbase[u];
+  for (uint64_t i0 = (uint64_t)blockIdx.x * NG; i0 < count;
+       i0 += (uint64_t)gridDim.x * NG) {  // uniform across the block
+    const uint64_t i = i0 + gi;
+    uint32_t d = 0;
+    const uint32_t *row = lists;
+    if (i < count) {
+      const uint32_t u = list[i];
+      d = llen[u];
+      row = lists + lbase[u];
+    }
     const uint32_t bits = table_bits(d);
-    for (uint32_t t = lane; t < (1u << bits); t += 64) table[t] = kEmpty;
+    for (uint32_t t = lig; t < (1u << bits); t += L) tab[t] = kEmpty;
     __syncthreads();
-    for (uint32_t j = lane; j < d; j += 64) {
+    for (uint32_t j = lig; j < d; j += L) {
       const uint32_t x = row[j];
-      hash_insert(table, x, bits);
-      pre[j] = llen[x];
-      base[j] = lbase[x];
+      hash_insert(tab, x, bits);
+      pr[j] = llen[x];
+      bs[j] = lbase[x];
     }
     __syncthreads();
     // exclusive scan of the list lengths; pre[d] = total
     uint32_t carry = 0;
-    for (uint32_t j0 = 0; j0 < d; j0 += 64) {
-      const uint32_t j = j0 + lane;
-      const uint32_t x = j < d ? pre[j] : 0;
-      const uint32_t inc = wave_inclusive_sum(x);
-      if (j < d) pre[j] = carry + inc - x;
-      carry += __shfl(inc, 63, 64);
+    for (uint32_t j0 = 0; j0 < d; j0 += L) {
+      const uint32_t j = j0 + lig;
+      const uint32_t x = j < d ? pr[j] : 0;
+      const uint32_t inc = group_inclusive_sum<L>(x, lig);
+      if (j < d) pr[j] = carry + inc - x;
+      carry += __shfl(inc, gi * L + L - 1, 64);
     }
-    if (lane == 0) pre[d] = carry;
+    if (lig == 0) pr[d] = carry;
     __syncthreads();
-    found += probe_lists<64>(table, bits, pre, base, lists, carry, lane);
+    found += probe_lists<L>(tab, bits, pr, bs, lists, carry, lig);
     __syncthreads();  // table, pre and base are reused by the next vertex
   }
   found = wave_sum(found);
@@ -254,17 +282,26 @@ __global__ void k_tc_acc_clear(unsigned long long *acc) {
   if (threadIdx.x == 0) acc[kAccLanes * kAccStride] = 0;
 }
 
-__global__ void k_tc_max_len(const uint32_t *len, uint32_t n,
-                             unsigned long long *acc) {
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  uint32_t v = i < n ? len[i] : 0;
+// largest len[i]: grid-stride, one atomic per block (few blocks, so the
+// single target address sees few atomics)
+__global__ void __launch_bounds__(256)
+    k_tc_max_len(const uint32_t *len, uint32_t n, unsigned long long *acc) {
+  __shared__ uint32_t wmax[4];
+  uint32_t v = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    v = len[i] > v ? len[i] : v;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const uint32_t t = __shfl_xor(v, o, 64);
     v = t > v ? t : v;
   }
-  if ((threadIdx.x & 63) == 0 && v)
-    atomicMax(&acc[kAccLanes * kAccStride], (unsigned long long)v);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x / 64] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++) v = wmax[w] > v ? wmax[w] : v;
+    if (v) atomicMax(&acc[kAccLanes * kAccStride], (unsigned long long)v);
+  }
 }
 
 __global__ void k_tc_acc_sum(unsigned long long *acc) {
@@ -295,7 +332,8 @@ int count_list_triangles(sstg_graph *g, const uint64_t *lbase,
   SSTG_LAUNCH(k_tc_acc_clear, 1, kAccLanes, 0, s, acc);
   // the flat probe index of one vertex (up to 512 lists, or 256 per block
   // step) is 32-bit: lists of 2^23 ids or more go to the caller's fallback
-  SSTG_LAUNCH(k_tc_max_len, ceil_div(n, 256), 256, 0, s, llen, n, acc);
+  SSTG_LAUNCH(k_tc_max_len, (uint32_t)std::min<uint64_t>(ceil_div(n, 256), 512),
+              256, 0, s, llen, n, acc);
   SSTG_HIP(hipMemcpyAsync(g->host_rb + 17, acc + kAccLanes * kAccStride,
                           sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   SSTG_HIP(hipStreamSynchronize(s));
@@ -317,20 +355,25 @@ int count_list_triangles(sstg_graph *g, const uint64_t *lbase,
     const uint32_t *l = list;
     switch (c) {
       case 0:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[0]>, grid, 64, 0, s, lbase, lists,
-                    llen, l, cnt, acc);
+        SSTG_LAUNCH((k_tc_count_wave<kTierOut[0], 16>),
+                    (uint32_t)std::min<uint64_t>(ceil_div(cnt, 4), 1u << 24), 64,
+                    0, s, lbase, lists, llen, l, cnt, acc);
         break;
       case 1:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[1]>, grid, 64, 0, s, lbase, lists,
-                    llen, l, cnt, acc);
+        SSTG_LAUNCH((k_tc_count_wave<kTierOut[1], 64>), grid, 64, 0, s, lbase,
+                    lists, llen, l, cnt, acc);
         break;
       case 2:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[2]>, grid, 64, 0, s, lbase, lists,
-                    llen, l, cnt, acc);
+        SSTG_LAUNCH((k_tc_count_wave<kTierOut[2], 64>), grid, 64, 0, s, lbase,
+                    lists, llen, l, cnt, acc);
         break;
       case 3:
-        SSTG_LAUNCH(k_tc_count_wave<kTierOut[3]>, grid, 64, 0, s, lbase, lists,
-                    llen, l, cnt, acc);
+        SSTG_LAUNCH((k_tc_count_wave<kTierOut[3], 64>), grid, 64, 0, s, lbase,
+                    lists, llen, l, cnt, acc);
+        break;
+      case 4:
+        SSTG_LAUNCH((k_tc_count_wave<kTierOut[4], 64>), grid, 64, 0, s, lbase,
+                    lists, llen, l, cnt, acc);
         break;
       default:
         SSTG_LAUNCH(k_tc_count_block, grid, 256, 0, s, lbase, lists, llen, l,

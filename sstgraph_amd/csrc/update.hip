@@ -663,8 +663,12 @@ __global__ void k_has(DevGraph g, const uint32_t *rows, const uint32_t *cols,
   out[i] = h;
 }
 
-__global__ void k_touch_sum(DevGraph g, const uint64_t *prefix, uint64_t m,
-                            unsigned long long *out) {
+// per-block sums of the stored ids into part[block] (no atomics: many
+// blocks adding to one address serialise)
+__global__ void __launch_bounds__(kT)
+    k_touch_sum(DevGraph g, const uint64_t *prefix, uint64_t m,
+                uint64_t *part) {
+  __shared__ uint64_t ws[kT / kWave];
   uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
   uint64_t acc = 0;
   if (e < m) {
@@ -676,8 +680,29 @@ __global__ void k_touch_sum(DevGraph g, const uint64_t *prefix, uint64_t m,
     }
   }
   acc = wave_sum(acc);
-  if ((threadIdx.x & (kWave - 1)) == 0 && acc)
-    atomicAdd(out, (unsigned long long)acc);
+  if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < kT / kWave; w++) t += ws[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// out = sum of part[0..nb) (one block)
+__global__ void __launch_bounds__(kT)
+    k_sum_u64(const uint64_t *part, uint64_t nb, uint64_t *out) {
+  __shared__ uint64_t ws[kT / kWave];
+  uint64_t t = 0;
+  for (uint64_t i = threadIdx.x; i < nb; i += kT) t += part[i];
+  t = wave_sum(t);
+  if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = 0;
+    for (int w = 0; w < kT / kWave; w++) a += ws[w];
+    *out = a;
+  }
 }
 
 inline uint64_t blocks(uint64_t n) { return ceil_div(n ? n : 1, kT); }
@@ -1054,11 +1079,16 @@ int graph_touch_sum(sstg_graph *g, uint64_t *out) {
   const uint64_t *prefix = nullptr;
   SSTG_TRY(graph_degree_prefix(g, &prefix));
   hipStream_t s = g->stream;
-  unsigned long long *acc = (unsigned long long *)(g->dev_ctr + kCtrScratch);
-  SSTG_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), s));
-  if (g->m)
-    SSTG_LAUNCH(k_touch_sum, chunk_blocks(g->m), kT, 0, s, g->view(), prefix,
-                g->m, acc);
+  uint64_t *acc = g->dev_ctr + kCtrScratch;
+  const uint64_t nb = chunk_blocks(g->m);
+  uint64_t *part = nullptr;
+  SSTG_TRY(g->ws.get(kWsTouch, nb, &part));
+  if (g->m) {
+    SSTG_LAUNCH(k_touch_sum, nb, kT, 0, s, g->view(), prefix, g->m, part);
+    SSTG_LAUNCH(k_sum_u64, 1, kT, 0, s, (const uint64_t *)part, nb, acc);
+  } else {
+    SSTG_HIP(hipMemsetAsync(acc, 0, sizeof(uint64_t), s));
+  }
   SSTG_HIP(hipMemcpyAsync(g->host_rb, acc, sizeof(uint64_t),
                           hipMemcpyDeviceToHost, s));
   SSTG_HIP(hipStreamSynchronize(s));

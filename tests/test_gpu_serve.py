@@ -1,8 +1,8 @@
 """Serving loop (sstg_server_*): every query result in a mixed stream must be
 the result the query would get alone on the graph as it stood at its arrival
 (BFS levels, CC labels and TC exact; PR within 1e-9), whether requests are
-coalesced and run concurrently (with or without store snapshots) or served
-one by one."""
+coalesced and run concurrently (with or without store snapshots, with or
+without one heavy query at a time) or served one by one."""
 import numpy as np
 import pytest
 
@@ -61,10 +61,12 @@ def _expected(n, base_s, base_d, reqs):
     return exp
 
 
-def _serve(n, base_s, base_d, reqs, slots, coalesce, snapshots=False):
+def _serve(n, base_s, base_d, reqs, slots, coalesce, snapshots=False,
+           one_heavy=False):
     g = S.SparseMatrixV(n)
     g.insert_batch(base_s, base_d)
-    srv = S.Server(g, slots=slots, coalesce=coalesce, snapshots=snapshots)
+    srv = S.Server(g, slots=slots, coalesce=coalesce, snapshots=snapshots,
+                   one_heavy=one_heavy)
     tickets = []
     for kind, a, b in reqs:  # submitted back to back: they queue and overlap
         if kind == "insert":
@@ -131,6 +133,11 @@ def test_served_results_equal_alone_results():
     _compare(reqs, got2, exp)
     say("store snapshots", stats2["snapshots"])
     assert stats2["snapshots"] >= 1
+    say("serve one heavy at a time, lighter requests passing, snapshots")
+    got3, _, stats3 = _serve(n, s, d, reqs, slots=4, coalesce=True,
+                             snapshots=True, one_heavy=True)
+    _compare(reqs, got3, exp)
+    say("store snapshots", stats3["snapshots"])
     say("serve serialised")
     got1, coalesced1, stats1 = _serve(n, s, d, reqs, slots=1, coalesce=False)
     _compare(reqs, got1, exp)
