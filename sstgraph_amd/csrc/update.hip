@@ -434,6 +434,10 @@ __device__ __forceinline__ void tile_row_info(const SegArgs &a,
 }
 
 constexpr int kTileRows = 1024;  // rows per block described in LDS
+// k_merge_tile runs 1024 threads over one LDS tile: the tile and row tables
+// take 45 KB, so larger blocks keep more waves per CU in flight
+constexpr int kMergeThreads = 1024;
+constexpr int kMergeItems = kTileLds / kMergeThreads;
 
 // Flat descriptors of the tile rows (written after any pool rebuild, so the
 // row offsets they hold are current): input source, output destination,
@@ -461,7 +465,7 @@ __global__ void k_tile_blocks(const uint64_t *tile_off, uint64_t nt,
                            : lower_bound_dev(tile_off, nt, b * (uint64_t)kTile);
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kMergeThreads)
     k_merge_tile(SegArgs a, DevGraph g, int insert, const uint64_t *blk_r0,
                  const uint64_t *t_src, const uint64_t *t_dst,
                  const uint32_t *t_alen, const uint32_t *t_bst) {
@@ -478,7 +482,7 @@ __global__ void __launch_bounds__(kT)
   const uint32_t nrows = (uint32_t)(r1 - r0);
   const bool table = nrows <= (uint32_t)kTileRows;
   if (table) {
-    for (uint32_t r = threadIdx.x; r < nrows; r += kT) {
+    for (uint32_t r = threadIdx.x; r < nrows; r += kMergeThreads) {
       rstart[r] = (uint32_t)(a.tile_off[r0 + r] - e0);
       ralen[r] = t_alen[r0 + r];
       rbstart[r] = t_bst[r0 + r];
@@ -488,10 +492,10 @@ __global__ void __launch_bounds__(kT)
     if (threadIdx.x == 0) rstart[nrows] = W;
     __syncthreads();
   }
-  uint16_t row_of[kTileItems];
+  uint16_t row_of[kMergeItems];
 #pragma unroll
-  for (int k = 0; k < kTileItems; k++) {
-    const uint32_t e = threadIdx.x + (uint32_t)k * kT;
+  for (int k = 0; k < kMergeItems; k++) {
+    const uint32_t e = threadIdx.x + (uint32_t)k * kMergeThreads;
     row_of[k] = 0;
     if (e < W) {
       uint32_t r;
@@ -524,8 +528,8 @@ __global__ void __launch_bounds__(kT)
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kTileItems; k++) {
-    const uint32_t e = threadIdx.x + (uint32_t)k * kT;
+  for (int k = 0; k < kMergeItems; k++) {
+    const uint32_t e = threadIdx.x + (uint32_t)k * kMergeThreads;
     if (e >= W) continue;
     const uint32_t r = row_of[k];
     uint32_t base, alen, blen;
@@ -938,7 +942,8 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                 insert ? 1 : 0, t_src, t_dst, t_alen, t_bst);
     SSTG_LAUNCH(k_tile_blocks, blocks(nblk + 1), kT, 0, s,
                 (const uint64_t *)tile_len, nt, nblk, blk_r0);
-    SSTG_LAUNCH(k_merge_tile, nblk, kT, 0, s, a, g->view(), insert ? 1 : 0,
+    SSTG_LAUNCH(k_merge_tile, nblk, kMergeThreads, 0, s, a, g->view(),
+                insert ? 1 : 0,
                 (const uint64_t *)blk_r0, (const uint64_t *)t_src,
                 (const uint64_t *)t_dst, (const uint32_t *)t_alen,
                 (const uint32_t *)t_bst);
