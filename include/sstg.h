@@ -73,6 +73,22 @@ int sstg_insert_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
 int sstg_remove_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                       uint64_t n, uint32_t flags);
 /* SparseMatrixV::insert(row, col), SM:536-544 (no self-loop skip). */
+/* Weighted batch insert (SparseMatrixV<true, uint32_t>::insert_batch with
+ * (row, col, value) tuples, SparseMatrix.hpp:556-643 and the valued PMA
+ * insert, PMA.hpp:1095-1146): the edges as insert_batch (self-loops
+ * skipped unless SSTG_KEEP_SELF_LOOPS), and every edge's weight set to its
+ * value, the last occurrence in the batch winning; a present edge's weight
+ * is overwritten.  Edges without a weight (inserted unweighted) weigh 1.
+ * remove_batch and remove also drop weights.  Weights are kept in a device
+ * hash table beside the store, made by the first weighted insert. */
+int sstg_insert_batch_weighted(sstg_graph *g, const uint32_t *src,
+                               const uint32_t *dst, const uint32_t *w,
+                               uint64_t n, uint32_t flags);
+/* value(row, col) (SparseMatrix.hpp:727-735): weight of each edge; 1 for a
+ * stored edge inserted without one, 0 for an absent edge. */
+int sstg_edge_weights(sstg_graph *g, const uint32_t *rows,
+                      const uint32_t *cols, uint64_t n, uint32_t *out,
+                      uint32_t flags);
 int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col);
 /* SparseMatrixV::remove(row, col), SM:546-554. */
 int sstg_remove(sstg_graph *g, uint32_t row, uint32_t col);
@@ -140,8 +156,11 @@ int sstg_bc(sstg_graph *g, uint32_t src, double *out, uint32_t flags);
 int sstg_cc(sstg_graph *g, uint32_t *out, uint32_t flags);
 /* TC(g), TST:606: number of triangles (each counted once). */
 int sstg_tc(sstg_graph *g, uint64_t *out);
-/* BF(g, src), TST:618 on a binary graph (every edge weighs 1): shortest
- * hop distances, INT32_MAX/2 for unreached vertices. */
+/* BF(g, src), TST:618 and TST:811: shortest distances from src,
+ * INT32_MAX/2 for unreached vertices.  Edge weights are those set by
+ * sstg_insert_batch_weighted (read as int32, 1 for an edge without one); a
+ * graph that never had a weighted insert weighs every edge 1.  At most
+ * n_rows relaxation rounds (a negative cycle ends there). */
 int sstg_bf(sstg_graph *g, uint32_t src, int32_t *out, uint32_t flags);
 /* TouchAll(g), TST:476: sum over all edges of the destination id. */
 int sstg_touch_all(sstg_graph *g, uint64_t *out);
@@ -191,7 +210,8 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
  * budget (hbm_budget bytes, 0 = 7/8 of the free memory at creation).  With
  * SSTG_SERVE_SNAPSHOTS an update does not wait for running or queued
  * queries: the graph moves to a device copy of its store and those queries
- * run on the old arrays (held against the budget while they are needed). */
+ * run on the old arrays (held against the budget while they are needed).
+ * A graph with edge weights is refused (SSTG_E_UNSUPPORTED). */
 #define SSTG_SERVE_COALESCE 1u
 #define SSTG_SERVE_SNAPSHOTS 2u
 /* at most one PageRank or TC at a time; lighter queries pass a waiting one */
@@ -273,7 +293,10 @@ int sstg_edges_from_file(const char *path, uint32_t *n, uint64_t *m,
                          uint32_t **src, uint32_t **dst, uint32_t **weights);
 void sstg_host_free(void *p);
 /* Reads a graph file into a new n x n graph (the harness's load step,
- * test.hpp:414-449). */
+ * test.hpp:414-449; with SSTG_LOAD_WEIGHTS the weighted load of
+ * real_graph_weights, test.hpp:699-720: the file's weights, 1 for an
+ * unweighted file). */
+#define SSTG_LOAD_WEIGHTS 0x8u
 int sstg_load_file(const char *path, uint32_t flags, sstg_graph **out);
 /* Writes the graph as an AdjacencyGraph file (graph_utilities.cpp:151-183,
  * write_unweighted_graph). */

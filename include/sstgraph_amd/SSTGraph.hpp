@@ -13,8 +13,9 @@
 //
 // Differences (kept deliberately): insert_batch/remove_batch do not reorder
 // the caller's array; ids out of range throw SSTGraph::sstg_error instead of
-// corrupting memory; only the binary CSR form (is_csr = true, no values) is
-// provided.  map_neighbors/map_range run on the host over a snapshot of the
+// corrupting memory; the CSR form (is_csr = true) is provided, binary or
+// with one uint32_t value per edge (SparseMatrixV<true, uint32_t>, the
+// weighted graph BF reads, test.hpp:699-846).  map_neighbors/map_range run on the host over a snapshot of the
 // rows exported from the device (refreshed after every mutation).
 #pragma once
 
@@ -63,15 +64,26 @@ template <typename... Ts> constexpr bool binary_types() {
     return false;
   }
 }
+template <typename... Ts> constexpr bool weight_types() {
+  if constexpr (sizeof...(Ts) == 1) {
+    return (std::is_same_v<uint32_t, Ts> && ...);
+  } else {
+    return false;
+  }
+}
 }  // namespace detail
 
 template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
   static_assert(is_csr_, "sstgraph_amd stores the CSR orientation only");
-  static_assert(detail::binary_types<Ts...>(),
-                "sstgraph_amd stores binary (unvalued) graphs");
+  static_assert(detail::binary_types<Ts...>() || detail::weight_types<Ts...>(),
+                "sstgraph_amd stores binary graphs or uint32_t edge weights");
+  static constexpr bool weighted = detail::weight_types<Ts...>();
 
  public:
-  using element_type = std::tuple<el_t, el_t>;
+  using element_type = std::conditional_t<weighted,
+                                          std::tuple<el_t, el_t, uint32_t>,
+                                          std::tuple<el_t, el_t>>;
+  using value_type = std::conditional_t<weighted, uint32_t, bool>;
 
   // SparseMatrix.hpp:73-87
   SparseMatrixV(el_t height, el_t width) : rows_(height), cols_(width) {
@@ -115,7 +127,7 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
 
   static constexpr bool is_csr() { return true; }
   static constexpr bool is_csc() { return false; }
-  static constexpr bool is_binary() { return true; }
+  static constexpr bool is_binary() { return !weighted; }
 
   // SparseMatrix.hpp:54-59, 150-161
   [[nodiscard]] uint32_t get_line_count() const { return rows_; }
@@ -145,11 +157,38 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
     return out != 0;
   }
 
+  // SparseMatrix.hpp:727-735: the edge's weight (1 if it was inserted
+  // without one), 0 if absent; has(row, col) on a binary graph
+  [[nodiscard]] value_type value(el_t row, el_t col) const {
+    if constexpr (weighted) {
+      uint32_t w = 0;
+      detail::check(sstg_edge_weights(g_, &row, &col, 1, &w, 0), "value");
+      return w;
+    } else {
+      return has(row, col);
+    }
+  }
+
   // SparseMatrix.hpp:536-554 (single insert keeps self-loops)
-  void insert(element_type e) { insert(std::get<0>(e), std::get<1>(e)); }
+  void insert(element_type e) {
+    if constexpr (weighted) {
+      el_t r = std::get<0>(e), c = std::get<1>(e);
+      uint32_t w = std::get<2>(e);
+      detail::check(sstg_insert_batch_weighted(g_, &r, &c, &w, 1,
+                                               SSTG_KEEP_SELF_LOOPS),
+                    "insert");
+      snap_valid_ = false;
+    } else {
+      insert(std::get<0>(e), std::get<1>(e));
+    }
+  }
   void insert(el_t row, el_t col) {
     detail::check(sstg_insert(g_, row, col), "insert");
     snap_valid_ = false;
+  }
+  void insert(el_t row, el_t col, uint32_t w) {
+    static_assert(weighted, "a binary graph takes no value");
+    insert(element_type(row, col, w));
   }
   void remove(el_t row, el_t col) {
     detail::check(sstg_remove(g_, row, col), "remove");
@@ -164,6 +203,14 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
   void remove_batch(std::tuple<el_t, el_t> *edges, uint64_t n,
                     bool semi_sorted = false) {
     batch(edges, n, semi_sorted, false);
+  }
+  // weighted structure-of-arrays form
+  void insert_batch(const el_t *src, const el_t *dst, const uint32_t *w,
+                    uint64_t n, uint32_t flags = 0) {
+    static_assert(weighted, "a binary graph takes no values");
+    detail::check(sstg_insert_batch_weighted(g_, src, dst, w, n, flags),
+                  "insert_batch");
+    snap_valid_ = false;
   }
   // Structure-of-arrays forms (host or, with SSTG_DEVICE_PTRS, device).
   void insert_batch(const el_t *src, const el_t *dst, uint64_t n,
@@ -248,6 +295,8 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
   // loops kept (the reference re-inserts edge by edge with insert())
   [[nodiscard]] SparseMatrixV rewrite_graph(
       std::vector<uint64_t> new_label_map) const {
+    // the reference re-inserts two-element tuples: binary graphs only
+    static_assert(!weighted, "rewrite_graph relabels binary graphs");
     if (new_label_map.size() != rows_)
       throw sstg_error(SSTG_E_INVAL, "rewrite_graph: one label per row");
     std::vector<uint32_t> m(rows_);
@@ -294,15 +343,25 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
   sstg_graph *handle() const { return g_; }
 
  private:
-  void batch(element_type *edges, uint64_t n, bool semi_sorted, bool ins) {
+  template <class E>
+  void batch(E *edges, uint64_t n, bool semi_sorted, bool ins) {
     std::vector<el_t> s(n), d(n);
+    std::vector<uint32_t> w;
+    constexpr bool valued = std::tuple_size_v<E> == 3;
+    if constexpr (valued) w.resize(n);
     for (uint64_t i = 0; i < n; i++) {  // std::get, never reinterpret_cast:
       s[i] = std::get<0>(edges[i]);       // libstdc++ stores get<1> first
       d[i] = std::get<1>(edges[i]);
+      if constexpr (valued) w[i] = std::get<2>(edges[i]);
     }
     uint32_t flags = semi_sorted ? SSTG_SEMI_SORTED : 0u;
-    int rc = ins ? sstg_insert_batch(g_, s.data(), d.data(), n, flags)
-                 : sstg_remove_batch(g_, s.data(), d.data(), n, flags);
+    int rc;
+    if constexpr (valued)
+      rc = sstg_insert_batch_weighted(g_, s.data(), d.data(), w.data(), n,
+                                      flags);
+    else
+      rc = ins ? sstg_insert_batch(g_, s.data(), d.data(), n, flags)
+               : sstg_remove_batch(g_, s.data(), d.data(), n, flags);
     detail::check(rc, ins ? "insert_batch" : "remove_batch");
     snap_valid_ = false;
   }
@@ -312,7 +371,17 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
     for (uint64_t i = 0; i < o.rows_; i++)
       for (uint64_t e = o.snap_off_[i]; e < o.snap_off_[i + 1]; e++)
         s[e] = (el_t)i;
-    insert_batch(s.data(), o.snap_cols_.data(), s.size(), SSTG_KEEP_SELF_LOOPS);
+    if constexpr (weighted) {
+      std::vector<uint32_t> w(s.size());
+      detail::check(sstg_edge_weights(o.g_, s.data(), o.snap_cols_.data(),
+                                      s.size(), w.data(), 0),
+                    "SparseMatrixV copy");
+      insert_batch(s.data(), o.snap_cols_.data(), w.data(), s.size(),
+                   SSTG_KEEP_SELF_LOOPS);
+    } else {
+      insert_batch(s.data(), o.snap_cols_.data(), s.size(),
+                   SSTG_KEEP_SELF_LOOPS);
+    }
   }
   void refresh() const {
     if (snap_valid_) return;
@@ -332,32 +401,43 @@ template <bool is_csr_ = true, typename... Ts> class SparseMatrixV {
 };
 
 // helpers.hpp:604-631: the edges of a .adj / .adj.shuf / .mtx file as a
-// malloc'd array the caller free()s (binary graphs; .mtx entries come in
-// both directions, 1-indexed ids shifted to 0).  Throws sstg_error where the
-// reference prints and exits.
+// malloc'd array the caller free()s (.mtx entries come in both directions,
+// 1-indexed ids shifted to 0).  value_type bool gives (row, col) tuples,
+// uint32_t gives (row, col, weight) with weight 1 from an unweighted file
+// (helpers.hpp:557-566).  Throws sstg_error where the reference prints and
+// exits.
 template <typename value_type = bool>
-std::tuple<el_t, el_t> *get_edges_from_file(const std::string &filename,
-                                            uint64_t *edge_count,
-                                            uint32_t *node_count,
-                                            bool print = true) {
-  static_assert(std::is_same_v<value_type, bool>,
-                "sstgraph_amd reads binary graphs");
+std::conditional_t<std::is_same_v<value_type, bool>, std::tuple<el_t, el_t>,
+                   std::tuple<el_t, el_t, value_type>> *
+get_edges_from_file(const std::string &filename, uint64_t *edge_count,
+                    uint32_t *node_count, bool print = true) {
+  static_assert(std::is_same_v<value_type, bool> ||
+                    std::is_same_v<value_type, uint32_t>,
+                "sstgraph_amd reads binary graphs or uint32_t weights");
+  constexpr bool valued = !std::is_same_v<value_type, bool>;
+  using E = std::conditional_t<valued, std::tuple<el_t, el_t, value_type>,
+                               std::tuple<el_t, el_t>>;
   (void)print;
-  uint32_t *s = nullptr, *d = nullptr;
+  uint32_t *s = nullptr, *d = nullptr, *w = nullptr;
   detail::check(sstg_edges_from_file(filename.c_str(), node_count, edge_count,
-                                     &s, &d, nullptr),
+                                     &s, &d, valued ? &w : nullptr),
                 "get_edges_from_file");
-  auto *out = (std::tuple<el_t, el_t> *)std::malloc(
-      (*edge_count ? *edge_count : 1) * sizeof(std::tuple<el_t, el_t>));
+  auto *out = (E *)std::malloc((*edge_count ? *edge_count : 1) * sizeof(E));
   if (!out) {
     sstg_host_free(s);
     sstg_host_free(d);
+    sstg_host_free(w);
     throw sstg_error(SSTG_E_NOMEM, "get_edges_from_file");
   }
-  for (uint64_t i = 0; i < *edge_count; i++)
-    new (&out[i]) std::tuple<el_t, el_t>(s[i], d[i]);
+  for (uint64_t i = 0; i < *edge_count; i++) {
+    if constexpr (valued)
+      new (&out[i]) E(s[i], d[i], w[i]);
+    else
+      new (&out[i]) E(s[i], d[i]);
+  }
   sstg_host_free(s);
   sstg_host_free(d);
+  sstg_host_free(w);
   return out;
 }
 
@@ -430,7 +510,9 @@ template <class G> uint64_t TC(G &g) {
   return out;
 }
 
-// BF(g, src) (test.hpp:618) on a binary graph: hop distances.
+// BF(g, src) (test.hpp:618, 811): shortest distances, INT32_MAX/2 where
+// unreached; edge weights on SparseMatrixV<true, uint32_t>, 1 per edge on a
+// binary graph.
 template <class G> int32_t *BF(G &g, uint32_t src) {
   int32_t *out = detail::alloc<int32_t>(g.num_nodes());
   int rc = sstg_bf(g.handle(), src, out, 0);

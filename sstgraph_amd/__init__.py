@@ -40,6 +40,7 @@ SSTG_SERVE_SNAPSHOTS = 0x2
 SSTG_SERVE_ONE_HEAVY = 0x4
 SSTG_SEMI_SORTED = 0x2
 SSTG_KEEP_SELF_LOOPS = 0x4
+SSTG_LOAD_WEIGHTS = 0x8
 
 UNREACHED = 0xFFFFFFFF
 
@@ -118,6 +119,8 @@ def lib():
         "sstg_destroy": (i32, [vp]),
         "sstg_insert_batch": (i32, [vp, vp, vp, u64, u32]),
         "sstg_remove_batch": (i32, [vp, vp, vp, u64, u32]),
+        "sstg_insert_batch_weighted": (i32, [vp, vp, vp, vp, u64, u32]),
+        "sstg_edge_weights": (i32, [vp, vp, vp, u64, vp, u32]),
         "sstg_insert": (i32, [vp, u32, u32]),
         "sstg_remove": (i32, [vp, u32, u32]),
         "sstg_has": (i32, [vp, u32, u32, P(i32)]),
@@ -377,10 +380,13 @@ class SparseMatrixV:
         self._h = h
 
     @classmethod
-    def from_file(cls, path):
-        """Reads a .adj / .mtx file into a new graph (sstg_load_file)."""
+    def from_file(cls, path, weights=False):
+        """Reads a .adj / .mtx file into a new graph (sstg_load_file); with
+        weights=True the file's weights are set (1 for an unweighted file)."""
         h = ctypes.c_void_p()
-        _check(lib().sstg_load_file(os.fsencode(path), 0, ctypes.byref(h)),
+        _check(lib().sstg_load_file(os.fsencode(path),
+                                    SSTG_LOAD_WEIGHTS if weights else 0,
+                                    ctypes.byref(h)),
                f"load {path}")
         n = ctypes.c_uint32(0)
         _check(lib().sstg_num_nodes(h, ctypes.byref(n)), "num_nodes")
@@ -470,6 +476,41 @@ class SparseMatrixV:
     def remove_batch(self, src, dst=None, semi_sorted=False):
         self._batch(lib().sstg_remove_batch, src, dst,
                     SSTG_SEMI_SORTED if semi_sorted else 0, "remove_batch")
+
+    def insert_batch_weighted(self, src, dst=None, weights=None):
+        """insert_batch of (row, col, value) tuples on a SparseMatrixV<true,
+        uint32_t> (SparseMatrix.hpp:556-643): the edges are inserted and each
+        one's weight set, the last occurrence in the batch winning.  Takes
+        three arrays, or one array of (row, col, value) rows."""
+        if isinstance(src, DeviceArray):
+            n = min(src.count, dst.count, weights.count)
+            _check(lib().sstg_insert_batch_weighted(
+                self._h, src.ptr, dst.ptr, weights.ptr, n, SSTG_DEVICE_PTRS),
+                "insert_batch_weighted")
+            return
+        if dst is None:
+            e = np.asarray(src)
+            src, dst, weights = e[:, 0], e[:, 1], e[:, 2]
+        s, d, w = _u32(src), _u32(dst), _u32(weights)
+        if s.shape != d.shape or s.shape != w.shape:
+            raise ValueError("src, dst and weights differ in length")
+        _check(lib().sstg_insert_batch_weighted(self._h, _ptr(s), _ptr(d), _ptr(w),
+                                                s.size, 0),
+               "insert_batch_weighted")
+
+    def value(self, row, col):
+        """SparseMatrix.hpp:727-735: the edge's weight (1 if it was inserted
+        without one), 0 if the edge is absent."""
+        return int(self.values([row], [col])[0])
+
+    def values(self, rows, cols):
+        r, c = _u32(rows), _u32(cols)
+        if r.shape != c.shape:
+            raise ValueError("rows and cols differ in length")
+        out = np.empty(r.size, np.uint32)
+        _check(lib().sstg_edge_weights(self._h, _ptr(r), _ptr(c), r.size, _ptr(out), 0),
+               "edge_weights")
+        return out
 
     def last_update_stats(self):
         st = UpdateStats()
@@ -607,6 +648,8 @@ class EdgeMapVertexMap:
 
     @staticmethod
     def BF(g, src):
+        """Bellman-Ford distances (weights from insert_batch_weighted, else
+        1 per edge), INT32_MAX // 2 where unreached."""
         out = np.empty(g.n_rows, np.int32)
         _check(lib().sstg_bf(g.handle, src, _ptr(out), 0), "BF")
         return out

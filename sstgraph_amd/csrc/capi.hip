@@ -256,12 +256,86 @@ int sstg_insert_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                       (flags & SSTG_DEVICE_PTRS) != 0);
 }
 
+// Runs fn(src, dst, w, count) on device copies of chunks of a host or
+// device batch (w may be null).
+extern "C++" template <class F>
+static int device_chunks(sstg_graph *g, const uint32_t *src,
+                         const uint32_t *dst, const uint32_t *w, uint64_t n,
+                         bool device_ptrs, F fn) {
+  const uint64_t kChunkPairs = 1ull << 28;
+  for (uint64_t b = 0; b < n; b += kChunkPairs) {
+    const uint64_t c = std::min(kChunkPairs, n - b);
+    if (device_ptrs) {
+      SSTG_TRY(fn(src + b, dst + b, w ? w + b : nullptr, c));
+      continue;
+    }
+    uint32_t *ds = nullptr, *dd = nullptr, *dw = nullptr;
+    SSTG_TRY(g->ws.get(kWsWtSrc, c, &ds));
+    SSTG_TRY(g->ws.get(kWsWtDst, c, &dd));
+    SSTG_HIP(hipMemcpyAsync(ds, src + b, c * 4, hipMemcpyHostToDevice,
+                            g->stream));
+    SSTG_HIP(hipMemcpyAsync(dd, dst + b, c * 4, hipMemcpyHostToDevice,
+                            g->stream));
+    if (w) {
+      SSTG_TRY(g->ws.get(kWsWtVal, c, &dw));
+      SSTG_HIP(hipMemcpyAsync(dw, w + b, c * 4, hipMemcpyHostToDevice,
+                              g->stream));
+    }
+    SSTG_TRY(fn(ds, dd, dw, c));
+  }
+  return SSTG_OK;
+}
+
+int sstg_insert_batch_weighted(sstg_graph *g, const uint32_t *src,
+                               const uint32_t *dst, const uint32_t *w,
+                               uint64_t n, uint32_t flags) {
+  if (!g || (n && (!src || !dst || !w))) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  const bool dev = (flags & SSTG_DEVICE_PTRS) != 0;
+  const bool keep_self = (flags & SSTG_KEEP_SELF_LOOPS) != 0;
+  // the edges first: an id out of range leaves graph and weights unchanged
+  SSTG_TRY(graph_update(g, src, dst, n, true, !keep_self, dev));
+  return device_chunks(g, src, dst, w, n, dev,
+                       [&](const uint32_t *s, const uint32_t *d,
+                           const uint32_t *ww, uint64_t c) {
+                         return weights_upsert(g, s, d, ww, c, keep_self);
+                       });
+}
+
 int sstg_remove_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                       uint64_t n, uint32_t flags) {
   if (!g || (n && (!src || !dst))) return SSTG_E_INVAL;
   hipSetDevice(g->device);
-  return graph_update(g, src, dst, n, false, false,
-                      (flags & SSTG_DEVICE_PTRS) != 0);
+  const bool dev = (flags & SSTG_DEVICE_PTRS) != 0;
+  SSTG_TRY(graph_update(g, src, dst, n, false, false, dev));
+  if (!g->wt_keys) return SSTG_OK;
+  return device_chunks(g, src, dst, nullptr, n, dev,
+                       [&](const uint32_t *s, const uint32_t *d,
+                           const uint32_t *, uint64_t c) {
+                         return weights_erase(g, s, d, c);
+                       });
+}
+
+int sstg_edge_weights(sstg_graph *g, const uint32_t *rows,
+                      const uint32_t *cols, uint64_t n, uint32_t *out,
+                      uint32_t flags) {
+  if (!g || (n && (!rows || !cols || !out))) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  if (flags & SSTG_DEVICE_PTRS) {
+    SSTG_TRY(weights_get(g, rows, cols, n, out));
+    SSTG_HIP(hipStreamSynchronize(g->stream));
+    return SSTG_OK;
+  }
+  uint32_t *dr = nullptr, *dc = nullptr, *dout = nullptr;
+  SSTG_TRY(g->ws.get(kWsWtSrc, n, &dr));
+  SSTG_TRY(g->ws.get(kWsWtDst, n, &dc));
+  SSTG_TRY(g->ws.get(kWsWtVal, n, &dout));
+  SSTG_HIP(hipMemcpyAsync(dr, rows, n * 4, hipMemcpyHostToDevice, g->stream));
+  SSTG_HIP(hipMemcpyAsync(dc, cols, n * 4, hipMemcpyHostToDevice, g->stream));
+  SSTG_TRY(weights_get(g, dr, dc, n, dout));
+  SSTG_HIP(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, g->stream));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  return SSTG_OK;
 }
 
 int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col) {
@@ -271,9 +345,7 @@ int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col) {
 }
 
 int sstg_remove(sstg_graph *g, uint32_t row, uint32_t col) {
-  if (!g) return SSTG_E_INVAL;
-  hipSetDevice(g->device);
-  return graph_update(g, &row, &col, 1, false, false, false);
+  return sstg_remove_batch(g, &row, &col, 1, 0);
 }
 
 int sstg_last_update_stats(sstg_graph *g, sstg_update_stats *out) {
@@ -469,7 +541,10 @@ int sstg_bf(sstg_graph *g, uint32_t src, int32_t *out, uint32_t flags) {
   if (src >= g->n_rows) return SSTG_E_RANGE;
   hipSetDevice(g->device);
   return with_output<int32_t>(g, out, g->n_rows, flags, kWsOut,
-                              [&](int32_t *d) { return algo_bf(g, src, d); });
+                              [&](int32_t *d) {
+                                return g->wt_keys ? algo_bf_weighted(g, src, d)
+                                                  : algo_bf(g, src, d);
+                              });
 }
 
 int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,

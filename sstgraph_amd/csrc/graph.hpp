@@ -34,6 +34,13 @@ struct sstg_graph {
   // cached: does every stored edge have its reverse?
   uint64_t sym_version = ~0ull;
   bool symmetric = false;
+  // edge weights (weights.hip): open-addressing table keyed by
+  // (row << 32 | col), made by the first weighted insert
+  uint64_t *wt_keys = nullptr;
+  uint64_t *wt_vals = nullptr;  // (tag << 32) | weight
+  uint64_t wt_cap = 0, wt_used = 0, wt_live = 0;
+  uint32_t wt_tag = 1;
+  uint64_t wtflat_version = ~0ull;
 
   sstg::DevGraph view() const {
     return sstg::DevGraph{n_rows, off, deg, cap, pool};
@@ -56,6 +63,15 @@ int graph_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
 int graph_touch_sum(sstg_graph *g, uint64_t *out);
 // algo.hip
 int graph_is_symmetric(sstg_graph *g, bool *out);
+// weights.hip (device arrays)
+int weights_upsert(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                   const uint32_t *w, uint64_t n, bool keep_self);
+int weights_erase(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                  uint64_t n);
+int weights_get(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
+                uint64_t n, uint32_t *out);
+int algo_bf_weighted(sstg_graph *g, uint32_t src, int32_t *dist);
+void weights_free(sstg_graph *g);
 // relabel.hip
 int graph_degree_order_map(sstg_graph *g, uint32_t *map);
 int graph_rewrite(sstg_graph *g, const uint32_t *map, sstg_graph *out);
@@ -111,6 +127,11 @@ enum : int {
   kWsRelSrc = 104,
   kWsRelDst = 105,
   kWsTouch = 106,
+  kWsWeights = 107,
+  kWsWeightFlat = 108,
+  kWsWtSrc = 109,
+  kWsWtDst = 110,
+  kWsWtVal = 111,
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range

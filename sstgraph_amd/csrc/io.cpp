@@ -302,18 +302,21 @@ int sstg_edges_from_file(const char *path, uint32_t *n, uint64_t *m,
 void sstg_host_free(void *p) { std::free(p); }
 
 int sstg_load_file(const char *path, uint32_t flags, sstg_graph **out) {
-  (void)flags;
-  if (!out) return SSTG_E_INVAL;
+  if (!out || (flags & ~SSTG_LOAD_WEIGHTS)) return SSTG_E_INVAL;
   *out = nullptr;
-  uint32_t n = 0, *s = nullptr, *d = nullptr;
+  const bool weighted = (flags & SSTG_LOAD_WEIGHTS) != 0;
+  uint32_t n = 0, *s = nullptr, *d = nullptr, *w = nullptr;
   uint64_t m = 0;
-  int rc = sstg_edges_from_file(path, &n, &m, &s, &d, nullptr);
+  int rc = sstg_edges_from_file(path, &n, &m, &s, &d, weighted ? &w : nullptr);
   if (rc != SSTG_OK) return rc;
   sstg_graph *g = nullptr;
   rc = sstg_create(n, n, 0, &g);
-  if (rc == SSTG_OK) rc = sstg_insert_batch(g, s, d, m, 0);
+  if (rc == SSTG_OK)
+    rc = weighted ? sstg_insert_batch_weighted(g, s, d, w, m, 0)
+                  : sstg_insert_batch(g, s, d, m, 0);
   std::free(s);
   std::free(d);
+  std::free(w);
   if (rc != SSTG_OK) {
     if (g) sstg_destroy(g);
     return rc;

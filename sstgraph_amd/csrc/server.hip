@@ -628,6 +628,8 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
                 SSTG_SERVE_ONE_HEAVY))
     return SSTG_E_INVAL;
   *out = nullptr;
+  // store copies and update batches carry no weight table
+  if (g->wt_keys) return SSTG_E_UNSUPPORTED;
   hipSetDevice(g->device);
   bool sym = false;
   SSTG_TRY(graph_is_symmetric(g, &sym));  // known once, then tracked

@@ -733,6 +733,7 @@ int graph_alloc(sstg_graph *g) {
 
 void graph_free(sstg_graph *g) {
   if (g->stream) hipStreamSynchronize(g->stream);
+  weights_free(g);
   hipFree(g->off);
   hipFree(g->deg);
   hipFree(g->cap);

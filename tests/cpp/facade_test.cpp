@@ -170,6 +170,42 @@ int main(int argc, char **argv) {
   orc_remove_batch(og, bs.data(), bd.data(), 5000);
   CHECK(g.M() == orc_num_edges(og));
 
+  // the weighted form BF reads (test.hpp:699-846): SparseMatrixV<true,
+  // uint32_t> built from (row, col, value) tuples, weights a function of the
+  // unordered pair so duplicates agree
+  {
+    auto wf = [](uint32_t a, uint32_t b) {
+      const uint32_t lo = std::min(a, b), hi = std::max(a, b);
+      return (uint32_t)((lo * 31u + hi) % 97u + 1u);
+    };
+    std::vector<std::tuple<el_t, el_t, uint32_t>> wedges;
+    for (const auto &e : edges)
+      wedges.emplace_back(std::get<0>(e), std::get<1>(e),
+                          wf(std::get<0>(e), std::get<1>(e)));
+    SSTGraph::SparseMatrixV<true, uint32_t> wg(n, n);
+    static_assert(!SSTGraph::SparseMatrixV<true, uint32_t>::is_binary());
+    wg.insert_batch(wedges.data(), wedges.size());
+    CHECK(wg.M() == off[n]);
+    std::vector<int32_t> aw(off[n]);
+    for (uint32_t r = 0; r < n; r++)
+      for (uint64_t e = off[r]; e < off[r + 1]; e++)
+        aw[e] = (int32_t)wf(r, cols[e]);
+    const uint32_t r0 = std::get<0>(edges[5]), c0 = std::get<1>(edges[5]);
+    if (r0 != c0) CHECK(wg.value(r0, c0) == wf(r0, c0));
+    CHECK(wg.value(n - 1, n - 1) == 0);
+    wg.insert(r0, c0, 1000u);  // a present key's value is overwritten
+    if (r0 != c0) CHECK(wg.value(r0, c0) == 1000u);
+    wg.insert(r0, c0, wf(r0, c0));
+    int32_t *wbf = EdgeMapVertexMap::BF(wg, src);
+    std::vector<int32_t> owbf(n);
+    orc_bf(n, off.data(), cols.data(), aw.data(), src, owbf.data());
+    for (uint32_t j = 0; j < n; j++) CHECK(wbf[j] == owbf[j]);
+    std::free(wbf);
+    // copies carry the weights
+    SSTGraph::SparseMatrixV<true, uint32_t> wc(wg);
+    if (r0 != c0) CHECK(wc.value(r0, c0) == wf(r0, c0));
+  }
+
   bool threw = false;
   try {
     g.insert(n, 0);
@@ -190,6 +226,17 @@ int main(int argc, char **argv) {
     CHECK(fn == 28 && fm == 420 && fg.M() == 420);
     CHECK(EdgeMapVertexMap::TC(fg) == 420);
     fg.print_statistics();
+    // the same file read weighted: a pattern file gives weight 1 throughout
+    auto *fw = SSTGraph::get_edges_from_file<uint32_t>(argv[1], &fm, &fn);
+    SSTGraph::SparseMatrixV<true, uint32_t> fwg(fn, fn);
+    fwg.insert_batch(fw, fm);
+    for (uint64_t k = 0; k < fm; k++) CHECK(std::get<2>(fw[k]) == 1u);
+    std::free(fw);
+    int32_t *d1 = EdgeMapVertexMap::BF(fwg, 0);
+    int32_t *d2 = EdgeMapVertexMap::BF(fg, 0);
+    for (uint32_t j = 0; j < fn; j++) CHECK(d1[j] == d2[j]);
+    std::free(d1);
+    std::free(d2);
   }
   std::printf("facade ok: n=%u m=%llu tc=%llu\n", n,
               (unsigned long long)off[n], (unsigned long long)tc);

@@ -326,6 +326,50 @@ def test_bf_unit_weights_equals_bfs_levels():
     assert np.all(dist[~unreached] == lev[~unreached])
 
 
+def test_bf_weighted_matches_dijkstra():
+    """Positive weights: Bellman-Ford distances are Dijkstra's
+    (scipy.sparse.csgraph), a routine independent of the oracle."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    n = 400
+    off, cols = random_graph(n, 1500, 61, symmetric=False)
+    rng = np.random.default_rng(5)
+    w = rng.integers(1, 50, cols.size).astype(np.int32)
+    a = csr_matrix((w.astype(np.float64), cols.astype(np.int64),
+                    off.astype(np.int64)), shape=(n, n))
+    for src in (0, 7, 123):
+        got = O.bf(off, cols, src, w)
+        ref = dijkstra(a, indices=src)
+        inf = ~np.isfinite(ref)
+        assert inf.any() and (~inf).sum() > 100
+        assert np.all(got[inf] == (2**31 - 1) // 2)
+        assert np.array_equal(got[~inf], ref[~inf].astype(np.int64))
+
+
+def test_bf_negative_weights_on_dag():
+    """Negative weights without a negative cycle (edges only from lower to
+    higher ids): the distances are the DAG's exact shortest paths, computed
+    by relaxing vertices in id order."""
+    n = 300
+    rng = np.random.default_rng(9)
+    s = rng.integers(0, n, 2000)
+    d = rng.integers(0, n, 2000)
+    lo, hi = np.minimum(s, d), np.maximum(s, d)
+    off, cols = csr_from_edges(n, lo, hi)
+    w = rng.integers(-20, 30, cols.size).astype(np.int32)
+    got = O.bf(off, cols, 0, w)
+    inf = (2**31 - 1) // 2
+    dist = [inf] * n
+    dist[0] = 0
+    for u in range(n):
+        if dist[u] == inf:
+            continue
+        for e in range(int(off[u]), int(off[u + 1])):
+            v = int(cols[e])
+            dist[v] = min(dist[v], dist[u] + int(w[e]))
+    assert got.tolist() == dist
+
+
 def test_er_generator_is_uniform_and_seeded():
     n = 1000
     s, d = O.er(n, 11, 200000)
