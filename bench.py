@@ -168,6 +168,12 @@ def run_batched(args):
         f"edges) in {time.perf_counter() - t0:.1f}s")
     sources = [int(np.argmax(pg.degrees())) for pg in plain]
     iters = 10
+    tune = None
+    if not args.no_tune:
+        # degree thresholds of the union store, tuned on PR iterations
+        tune = M.autotune()
+        log(f"autotuned thresholds {tune['thresholds']}: PR x2 "
+            f"{tune['start_ms']:.2f} -> {tune['best_ms']:.2f} ms")
     # batched: one hipGraph replay per step
     for _ in range(args.warmup):
         M.run(sources, pr_iters=iters)
@@ -212,6 +218,7 @@ def run_batched(args):
                   "speedup_vs_plain": t_p / t_b,
                   "edges_per_s_batched": m_total * (iters + 2) / t_b,
                   "bfs_levels": res64["levels_used"],
+                  "autotune": tune,
                   "pr_f32_kahan_max_rel_err_vs_f64": float(rel.max()),
                   "pr_f32_kahan_mean_rel_err_vs_f64": float(rel.mean())},
     }
@@ -436,6 +443,8 @@ def main():
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-algos", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the degree-threshold autotuner")
     ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs"],
                     default="updates")
     ap.add_argument("--graphs", type=int, default=512)
@@ -514,6 +523,26 @@ def main():
                                  gbs_model=(4 * m + 24 * n) / per_it / 1e9)
         log(f"PR x{args.pr_iters}: {1e3 * t:.2f} ms ({m * args.pr_iters / t / 1e9:.2f} GTEPS, "
             f"{(4 * m + 24 * n) / per_it / 1e9:.0f} GB/s model)")
+        if not args.no_tune:
+            # degree thresholds picked by the autotuner on this graph (PR
+            # iterations + BFS from the hub), then both measured again
+            rep = g.autotune(pr=True, bfs_source=hub)
+            EMV.BFS(g, hub, out=(d_par, d_lev))
+            EMV.PR_S(g, args.pr_iters, out=d_pr)
+            S.synchronize()
+            tb = float(np.median([timed(lambda: EMV.BFS(g, hub, out=(d_par, d_lev)))
+                                  for _ in range(5)]))
+            tp = float(np.median([timed(lambda: EMV.PR_S(g, args.pr_iters, out=d_pr))
+                                  for _ in range(3)]))
+            m_r = extra["bfs_hub"]["m_reached"]
+            rep.update(bfs_hub_ms=1e3 * tb, bfs_hub_gteps=m_r / tb / 1e9,
+                       pagerank_ms=1e3 * tp,
+                       pagerank_gteps=m * args.pr_iters / tp / 1e9,
+                       pagerank_gbs_model=(4 * m + 24 * n) / (tp / args.pr_iters) / 1e9)
+            extra["autotune"] = rep
+            log(f"autotuned thresholds {rep['thresholds']} ({rep['evaluations']} settings): "
+                f"BFS hub {1e3 * tb:.3f} ms, PR x{args.pr_iters} {1e3 * tp:.2f} ms")
+            g.set_degree_thresholds([8, 64, 2048])
         for d in (d_par, d_lev, d_pr):
             d.free()
 

@@ -282,6 +282,29 @@ typedef struct sstg_stats {
   uint64_t dest_bits[33];  /* stored edges whose destination needs x bits */
 } sstg_stats;
 int sstg_statistics(sstg_graph *g, sstg_stats *out);
+
+/* ---- degree thresholds and their autotuner ------------------------------
+ * No reference counterpart (the reference's tinyset thresholds are fixed
+ * by its PMA layout, TinySet_small.hpp:216-293).  The vertex-pull kernels
+ * give a vertex 1 lane if its degree is <= t[0], 8 lanes if <= t[1], a wave
+ * if <= t[2], else a 256-thread block (defaults 8, 64, 2048; t strictly
+ * increasing).  Per graph; a mutating call. */
+int sstg_set_degree_thresholds(sstg_graph *g, const uint32_t t[3]);
+int sstg_get_degree_thresholds(sstg_graph *g, uint32_t t[3]);
+#define SSTG_TUNE_PR 1u  /* objective: pr_iters FP64 PageRank iterations */
+#define SSTG_TUNE_BFS 2u /* objective: one BFS from src */
+typedef struct sstg_tune_report {
+  uint32_t thresholds[3];       /* chosen (now set on the graph) */
+  uint32_t start_thresholds[3]; /* in effect before tuning */
+  double start_ms, best_ms;     /* objective time, start and chosen */
+  uint32_t evaluations;         /* candidate settings timed */
+} sstg_tune_report;
+/* Times the objective (what = SSTG_TUNE_PR and/or SSTG_TUNE_BFS; pr_iters 0
+ * means 2) on this graph over a grid of thresholds by coordinate descent
+ * and leaves the fastest set on the graph.  Results of the algorithms are
+ * unchanged except for the summation order of PageRank. */
+int sstg_autotune(sstg_graph *g, uint32_t what, uint32_t src,
+                  uint32_t pr_iters, sstg_tune_report *out);
 /* get_edges_from_file (helpers.hpp:604-631): .adj / .adj.shuf
  * (AdjacencyGraph or WeightedAdjacencyGraph; helpers.hpp:533-601) or .mtx
  * (MatrixMarket coordinate, 1-indexed, every entry returned in both

@@ -41,6 +41,8 @@ SSTG_SERVE_ONE_HEAVY = 0x4
 SSTG_SEMI_SORTED = 0x2
 SSTG_KEEP_SELF_LOOPS = 0x4
 SSTG_LOAD_WEIGHTS = 0x8
+SSTG_TUNE_PR = 0x1
+SSTG_TUNE_BFS = 0x2
 
 UNREACHED = 0xFFFFFFFF
 
@@ -68,6 +70,28 @@ class Stats(ctypes.Structure):
                 ("pool_capacity", ctypes.c_uint64), ("pool_used", ctypes.c_uint64),
                 ("pool_garbage", ctypes.c_uint64), ("rebuilds", ctypes.c_uint64),
                 ("dest_bits", ctypes.c_uint64 * 33)]
+
+
+class TuneReport(ctypes.Structure):
+    """sstg_tune_report (the degree-threshold autotuner's outcome)."""
+    _fields_ = [("thresholds", ctypes.c_uint32 * 3),
+                ("start_thresholds", ctypes.c_uint32 * 3),
+                ("start_ms", ctypes.c_double), ("best_ms", ctypes.c_double),
+                ("evaluations", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {"thresholds": list(self.thresholds),
+                "start_thresholds": list(self.start_thresholds),
+                "start_ms": self.start_ms, "best_ms": self.best_ms,
+                "evaluations": int(self.evaluations)}
+
+
+def _autotune(handle, pr, bfs_source, pr_iters):
+    what = (SSTG_TUNE_PR if pr else 0) | (SSTG_TUNE_BFS if bfs_source is not None else 0)
+    rep = TuneReport()
+    _check(lib().sstg_autotune(handle, what, 0 if bfs_source is None else bfs_source,
+                               pr_iters, ctypes.byref(rep)), "autotune")
+    return rep.as_dict()
 
 
 class UpdateStats(ctypes.Structure):
@@ -160,6 +184,9 @@ def lib():
         "sstg_degree_order_map": (i32, [vp, vp, u32]),
         "sstg_rewrite_graph": (i32, [vp, vp, u32, P(vp)]),
         "sstg_statistics": (i32, [vp, P(Stats)]),
+        "sstg_set_degree_thresholds": (i32, [vp, vp]),
+        "sstg_get_degree_thresholds": (i32, [vp, vp]),
+        "sstg_autotune": (i32, [vp, u32, u32, u32, P(TuneReport)]),
         "sstg_edges_from_file": (i32, [ctypes.c_char_p, P(u32), P(u64), P(vp),
                                        P(vp), P(vp)]),
         "sstg_host_free": (None, [vp]),
@@ -320,6 +347,13 @@ class MultiGraph:
         _check(lib().sstg_multi_insert_batch(self._h, graph, _ptr(s), _ptr(d),
                                              s.size, 0), "multi insert")
 
+    def autotune(self, pr_iters=2):
+        """Tunes the degree thresholds of the union store on PageRank
+        iterations over all graphs; the next run re-captures with them."""
+        h = ctypes.c_void_p()
+        _check(lib().sstg_multi_graph(self._h, ctypes.byref(h)), "multi graph")
+        return _autotune(h, True, None, pr_iters)
+
     def run(self, sources, max_levels=64, pr_iters=10, damping=0.85, f32=False):
         src = _u32(sources)
         _check(lib().sstg_multi_run(self._h, _ptr(src), max_levels, pr_iters,
@@ -412,6 +446,24 @@ class SparseMatrixV:
         _check(lib().sstg_rewrite_graph(self._h, m.ctypes.data, 0, ctypes.byref(h)),
                "rewrite_graph")
         return SparseMatrixV(self.n_rows, self.n_cols, _handle=h)
+
+    def degree_thresholds(self):
+        """Degree boundaries of the 1-lane / 8-lane / wave / block groups."""
+        t = np.zeros(3, np.uint32)
+        _check(lib().sstg_get_degree_thresholds(self._h, _ptr(t)), "thresholds")
+        return t.tolist()
+
+    def set_degree_thresholds(self, t):
+        t = _u32(t)
+        if t.size != 3:
+            raise ValueError("three thresholds")
+        _check(lib().sstg_set_degree_thresholds(self._h, _ptr(t)), "thresholds")
+
+    def autotune(self, pr=True, bfs_source=None, pr_iters=2):
+        """Picks the degree thresholds by timing PageRank iterations and/or a
+        BFS from bfs_source on this graph (sstg_autotune); returns the
+        report and leaves the chosen thresholds set."""
+        return _autotune(self._h, pr, bfs_source, pr_iters)
 
     def statistics(self):
         st = Stats()

@@ -338,6 +338,24 @@ int sstg_edge_weights(sstg_graph *g, const uint32_t *rows,
   return SSTG_OK;
 }
 
+int sstg_set_degree_thresholds(sstg_graph *g, const uint32_t t[3]) {
+  if (!g || !t) return SSTG_E_INVAL;
+  return graph_set_thresholds(g, t);
+}
+
+int sstg_get_degree_thresholds(sstg_graph *g, uint32_t t[3]) {
+  if (!g || !t) return SSTG_E_INVAL;
+  for (int i = 0; i < 3; i++) t[i] = g->bin_hi[i];
+  return SSTG_OK;
+}
+
+int sstg_autotune(sstg_graph *g, uint32_t what, uint32_t src,
+                  uint32_t pr_iters, sstg_tune_report *out) {
+  if (!g) return SSTG_E_INVAL;
+  hipSetDevice(g->device);
+  return graph_autotune(g, what, src, pr_iters, out);
+}
+
 int sstg_insert(sstg_graph *g, uint32_t row, uint32_t col) {
   if (!g) return SSTG_E_INVAL;
   hipSetDevice(g->device);

@@ -463,8 +463,9 @@ int edge_map_sparse(sstg_graph *g, EdgeMapCtx &ctx, const uint32_t *front,
 
 // ----------------------------------------------------------- degree bins
 // Vertices split by degree for the vertex-pull kernels (cached per graph
-// version): bin 0 deg <= 8 (one lane), bin 1 <= 64 (8 lanes), bin 2 <= 2048
-// (a wave), bin 3 larger (a block).
+// version): bin 0 deg <= bin_hi[0] (one lane), bin 1 <= bin_hi[1] (8 lanes),
+// bin 2 <= bin_hi[2] (a wave), bin 3 larger (a block); the boundaries
+// default to 8 / 64 / 2048 and are set per graph by the autotuner.
 struct DegreeBins {
   uint32_t *list[4] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t count[4] = {0, 0, 0, 0};
@@ -484,8 +485,10 @@ __global__ void k_bin_flags(const uint32_t *deg, uint32_t n, uint32_t lo,
 }  // namespace em
 
 inline int degree_bins(sstg_graph *g, DegreeBins *out) {
-  static const uint32_t lo[4] = {0, 9, 65, 2049};
-  static const uint32_t hi[4] = {8, 64, 2048, 0xFFFFFFFFu};
+  const uint32_t lo[4] = {0, g->bin_hi[0] + 1, g->bin_hi[1] + 1,
+                          g->bin_hi[2] + 1};
+  const uint32_t hi[4] = {g->bin_hi[0], g->bin_hi[1], g->bin_hi[2],
+                          0xFFFFFFFFu};
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
   for (int b = 0; b < 4; b++)

@@ -28,9 +28,13 @@ struct sstg_graph {
   // cached derived data: exclusive prefix of deg (n_rows + 1 entries)
   uint64_t *deg_prefix = nullptr;
   uint64_t deg_prefix_version = ~0ull;
-  // cached degree bins of the vertex-pull kernels (lists live in workspace)
+  // cached degree bins of the vertex-pull kernels (lists live in workspace):
+  // bin b holds degrees in (bin_hi[b-1], bin_hi[b]], the last bin the rest;
+  // tune.hip sets the boundaries, tune_epoch counts the changes
   uint64_t bins_version = ~0ull;
   uint64_t bins_count[4] = {0, 0, 0, 0};
+  uint32_t bin_hi[3] = {8, 64, 2048};
+  uint64_t tune_epoch = 0;
   // cached: does every stored edge have its reverse?
   uint64_t sym_version = ~0ull;
   bool symmetric = false;
@@ -63,6 +67,10 @@ int graph_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
 int graph_touch_sum(sstg_graph *g, uint64_t *out);
 // algo.hip
 int graph_is_symmetric(sstg_graph *g, bool *out);
+// tune.hip
+int graph_set_thresholds(sstg_graph *g, const uint32_t t[3]);
+int graph_autotune(sstg_graph *g, uint32_t what, uint32_t src,
+                   uint32_t pr_iters, sstg_tune_report *rep);
 // weights.hip (device arrays)
 int weights_upsert(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                    const uint32_t *w, uint64_t n, bool keep_self);

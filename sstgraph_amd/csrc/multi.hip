@@ -28,7 +28,7 @@ struct sstg_multi {
   // captured execution
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint64_t exec_version = ~0ull;
+  uint64_t exec_version = ~0ull, exec_epoch = ~0ull;
   uint32_t exec_iters = 0, exec_levels = 0, levels_run = 0;
   double exec_damping = 0;
   int exec_f32 = -1;
@@ -327,6 +327,7 @@ static int capture(sstg_multi *M, uint32_t max_levels, uint32_t iters,
   SSTG_HIP(hipGraphInstantiate(&M->exec, graph, nullptr, nullptr, 0));
   M->graph = graph;
   M->exec_version = g->version;
+  M->exec_epoch = g->tune_epoch;
   M->exec_iters = iters;
   M->exec_levels = max_levels;
   M->exec_damping = damping;
@@ -452,7 +453,8 @@ int sstg_multi_run(sstg_multi *M, const uint32_t *sources, uint32_t max_levels,
     if (sources[i] >= M->vbase[i + 1] - M->vbase[i]) return SSTG_E_RANGE;
   SSTG_HIP(hipMemcpyAsync(M->sources, sources, M->k * sizeof(uint32_t),
                           hipMemcpyHostToDevice, g->stream));
-  if (!M->exec || M->exec_version != g->version || M->exec_iters != pr_iters ||
+  if (!M->exec || M->exec_version != g->version ||
+      M->exec_epoch != g->tune_epoch || M->exec_iters != pr_iters ||
       M->exec_levels != max_levels || M->exec_damping != damping ||
       M->exec_f32 != (pr_f32 ? 1 : 0))
     SSTG_TRY(capture(M, max_levels, pr_iters, damping, pr_f32 != 0));
