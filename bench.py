@@ -52,13 +52,18 @@ def hash64_np(u):
     return v
 
 
+RMAT_FLAGS = 0  # SSTG_RMAT_NORMALIZED with --rmat-normalized
+RMAT_NOTE = ("hashDouble in [0,2): effective 0.25/0.05/0.05/0.65 (reference generator)",
+             "normalised draws in [0,1): 0.5/0.1/0.1/0.3 (--rmat-normalized)")
+
+
 def device_symmetric_rmat(S, n, seed, count):
     """2*count pairs on the device: rMat edges (u,v) followed by (v,u)."""
     L = S.lib()
     src = S.DeviceArray(2 * count, np.uint32)
     dst = S.DeviceArray(2 * count, np.uint32)
     S._check(L.sstg_rmat(n, seed, 0.5, 0.1, 0.1, 0, count, src.ptr, dst.ptr,
-                         S.SSTG_DEVICE_PTRS), "rmat")
+                         S.SSTG_DEVICE_PTRS | RMAT_FLAGS), "rmat")
     half = ctypes.c_void_p(src.ptr.value + 4 * count)
     halfd = ctypes.c_void_p(dst.ptr.value + 4 * count)
     S._check(L.sstg_memcpy_d2d(half, dst.ptr, 4 * count), "d2d")
@@ -70,7 +75,7 @@ def make_batch(S, n, off, cols, size, seed):
     """80% fresh rMat(n, seed) edges + 20% edges present in the base graph
     (edge index hash64(seed, j) mod m), as device arrays."""
     fresh = int(size * 0.8)
-    fs, fd = S.rmat(n, seed, fresh)
+    fs, fd = S.rmat(n, seed, fresh, normalized=RMAT_FLAGS != 0)
     j = np.arange(size - fresh, dtype=np.uint64) + np.uint64(seed) * np.uint64(1 << 32)
     pick = (hash64_np(j) % np.uint64(len(cols))).astype(np.int64)
     rows = (np.searchsorted(off, pick, side="right") - 1).astype(np.uint32)
@@ -101,10 +106,36 @@ def cpu_baseline(off, cols, n, bs, bd, budget_s=30.0):
         og.remove_batch(bs, bd)
         times.append(ti)
     med = float(np.median(times))
+    # the same on one core
+    O.set_threads(1)
+    t1 = []
+    t_start = time.perf_counter()
+    while len(t1) < 3 and time.perf_counter() - t_start < budget_s / 2:
+        t1.append(timed(lambda: og.insert_batch(bs, bd)))
+        og.remove_batch(bs, bd)
+    O.set_threads(threads)
+    med1 = float(np.median(t1))
     return dict(value=len(bs) / med, unit="edges/s", cores=threads,
                 kind="port",
                 sample=f"oracle insert_batch of one {len(bs)}-edge batch into "
-                       f"the scale-24 base graph, median of {len(times)} runs")
+                       f"the scale-24 base graph, median of {len(times)} runs",
+                single_core=dict(value=len(bs) / med1, unit="edges/s", cores=1,
+                                 runs=len(t1)))
+
+
+def cpu_algos(off, cols, hub, iters=2):
+    """The oracle's BFS (one thread, queue order) from the hub and FP64
+    PageRank iterations (OpenMP) on the same base graph, for the BFS/PR
+    figures of the line."""
+    from oracle import oracle as O  # CPU baseline leg only
+    t_bfs = timed(lambda: O.bfs(off, cols, hub))
+    t_pr = timed(lambda: O.pagerank(off, cols, iters))
+    m = int(off[-1])
+    return dict(bfs_hub_ms=1e3 * t_bfs, bfs_threads=1,
+                bfs_gteps=m / t_bfs / 1e9,
+                pagerank_ms_per_iter=1e3 * t_pr / iters,
+                pagerank_threads=O.threads(),
+                pagerank_gteps=m * iters / t_pr / 1e9)
 
 
 def c4_specs(k=512, seed=3):
@@ -133,7 +164,7 @@ def c4_edges(S, spec):
     dst = S.DeviceArray(2 * count, np.uint32)
     if kind == "rmat":
         S._check(L.sstg_rmat(n, seed, 0.5, 0.1, 0.1, 0, count, src.ptr, dst.ptr,
-                             S.SSTG_DEVICE_PTRS), "rmat")
+                             S.SSTG_DEVICE_PTRS | RMAT_FLAGS), "rmat")
     else:
         S._check(L.sstg_er(n, seed, 0, count, src.ptr, dst.ptr,
                            S.SSTG_DEVICE_PTRS), "er")
@@ -210,7 +241,7 @@ def run_batched(args):
         "value": 1.0 / t_b, "unit": "passes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t_b, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 (PR), u32",
-        "data": "synthetic (rMat + seeded Erdos-Renyi)",
+        "data": "synthetic (rMat + seeded Erdos-Renyi; " + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
         "config": {"workload": f"{len(specs)} graphs, scale 8..18, half rMat (edge "
                                "factor 8..32), half ER (avg degree 4..64), seed 3",
                    "vertices": int(sum(sizes)), "directed_edges": int(m_total)},
@@ -343,7 +374,7 @@ def run_serve(args):
     batches = {}
     for kind, arg in reqs:
         if kind == "insert":
-            bs, bd = S.rmat(n, arg[1], arg[0])
+            bs, bd = S.rmat(n, arg[1], arg[0], normalized=RMAT_FLAGS != 0)
             batches[arg] = (np.concatenate([bs, bd]), np.concatenate([bd, bs]))
     log(f"serve: scale {scale}, m={m0}, {len(reqs)} requests")
     res, rate, t_arr = {}, None, None
@@ -368,7 +399,7 @@ def run_serve(args):
         "value": sv["poisson"]["p99_ms"], "unit": "ms", "n_gpus": 1,
         "steps": len(reqs), "warmup": 0, "ms_per_step": sv["poisson"]["mean_ms"],
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u32 / f64 (PR)", "data": "synthetic (rMat, seeded trace)",
+        "dtype": "u32 / f64 (PR)", "data": "synthetic (rMat, seeded trace; " + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
         "config": {"workload": f"served stream on rMat scale {scale} (seed 4, edge "
                                f"factor {args.edge_factor}, symmetrised), Poisson "
                                f"arrivals at {rate:.1f}/s ({args.load:.0%} of the "
@@ -426,7 +457,7 @@ def run_algo(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if args.workload == "pagerank" else "u32",
-        "data": "synthetic (rMat generator of the reference, seeded)",
+        "data": "synthetic (rMat generator of the reference, seeded; " + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
         "config": {"workload": f"{name} on rMat scale {args.scale} (seed 2, edge "
                                f"factor {args.edge_factor}, symmetrised)",
                    "directed_edges": int(m)}}))
@@ -445,6 +476,9 @@ def main():
     ap.add_argument("--no-algos", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the degree-threshold autotuner")
+    ap.add_argument("--rmat-normalized", action="store_true",
+                    help="rMat draws in [0,1) (probabilities 0.5/0.1/0.1/0.3) "
+                         "instead of the reference's [0,2)")
     ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs"],
                     default="updates")
     ap.add_argument("--graphs", type=int, default=512)
@@ -453,6 +487,9 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="Poisson rate as a fraction of the serialised capacity")
     args = ap.parse_args()
+    global RMAT_FLAGS
+    if args.rmat_normalized:
+        RMAT_FLAGS = 0x10  # SSTG_RMAT_NORMALIZED
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -482,8 +519,7 @@ def main():
     dst.free()
     off, cols = g.export_csr()
     extra = {"build_seconds": t_build, "build_pairs_per_s": 2 * count / t_build,
-             "m_directed": m, "n": n, "rmat_probabilities": "hashDouble in [0,2): "
-             "effective 0.25/0.05/0.05/0.65 (reference generator)"}
+             "m_directed": m, "n": n, "rmat_probabilities": RMAT_NOTE[RMAT_FLAGS != 0]}
 
     # ------------------------------------------------ BFS / PR on the graph
     # Outputs stay in HBM (DeviceArray); the reference generator leaves vertex
@@ -638,7 +674,13 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(off, cols, n, hs, hd)
-        log(f"cpu baseline {cpu['value']:.3e} edges/s on {cpu['cores']} threads")
+        log(f"cpu baseline {cpu['value']:.3e} edges/s on {cpu['cores']} threads, "
+            f"{cpu['single_core']['value']:.3e} on one")
+        if not args.no_algos:
+            hub = int(np.argmax(np.diff(off)))
+            extra["cpu_algos"] = cpu_algos(off, cols, hub)
+            log(f"cpu BFS from hub {extra['cpu_algos']['bfs_hub_ms']:.0f} ms, PR "
+                f"{extra['cpu_algos']['pagerank_ms_per_iter']:.0f} ms/iter")
 
     ins = float(np.mean(ti))
     out = {
@@ -653,7 +695,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic (rMat generator of the reference, seeded)",
+        "data": "synthetic (rMat generator of the reference, seeded; " + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
         "config": {"workload": f"dynamic updates: rMat scale {args.scale} base graph "
                                f"(seed 2, edge factor {args.edge_factor}, symmetrised), "
                                f"insert+remove batches of {B} edges (seed 104, 20% existing)",

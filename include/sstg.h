@@ -195,7 +195,11 @@ int sstg_multi_results(sstg_multi *m, uint32_t *level, uint32_t *label,
 
 /* ---- workload generator (rmat_util.h:82-131 rMat<uint32_t>) ------------ */
 /* Edges i in [first, first+count) of rMat(n, seed, a, b, c) into src/dst
- * (host or device arrays per flags).  n must be a power of two. */
+ * (host or device arrays per flags).  n must be a power of two.  The draw
+ * is the reference's hash32 / INT32_MAX in [0, 2), so the effective quadrant
+ * probabilities are a/2, b/2, c/2 and the rest; with SSTG_RMAT_NORMALIZED
+ * (no reference counterpart) it is hash32 / 2^32 and they are a, b, c. */
+#define SSTG_RMAT_NORMALIZED 0x10u
 int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
               uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
               uint32_t flags);

@@ -43,6 +43,8 @@ def lib():
         "orc_aspen_ith_rand": (u64, [u64, u64]),
         "orc_log2_up": (u32, [u64]),
         "orc_rmat": (ctypes.c_int, [u32, u32, f64, f64, f64, u64, u64, _u32p, _u32p]),
+        "orc_rmat_normalized": (ctypes.c_int,
+                                [u32, u32, f64, f64, f64, u64, u64, _u32p, _u32p]),
         "orc_er": (ctypes.c_int, [u32, u32, u64, u64, _u32p, _u32p]),
         "orc_graph_create": (vp, [u32, u32]),
         "orc_graph_destroy": (None, [vp]),
@@ -83,10 +85,11 @@ def hash64(x):
     return int(lib().orc_hash64(x))
 
 
-def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1):
+def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1, normalized=False):
     src = np.empty(count, dtype=np.uint32)
     dst = np.empty(count, dtype=np.uint32)
-    rc = lib().orc_rmat(n, seed, a, b, c, first, count, src, dst)
+    fn = lib().orc_rmat_normalized if normalized else lib().orc_rmat
+    rc = fn(n, seed, a, b, c, first, count, src, dst)
     if rc != 0:
         raise ValueError("rMat: n must be a power of two and a+b+c <= 1")
     return src, dst

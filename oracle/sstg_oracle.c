@@ -89,11 +89,19 @@ static double rmat_hash_double(uint32_t i) {
   return (double)orc_hash32(i) / (double)INT32_MAX;
 }
 
+/* The normalised draw r = hash32(i) / 2^32, in [0, 1): quadrant
+ * probabilities are a, b, c, 1-a-b-c as stated, not halved.  No reference
+ * counterpart; the measurement plan asks for it beside the reference draw. */
+static double rmat_hash_unit(uint32_t i) {
+  return (double)orc_hash32(i) / 4294967296.0;
+}
+
 /* rmat_util.h:82-131.  The recursion rMatRec(nn, start, stride) decides the
  * bit nn/2 with r = hashDouble(start) after recursing on (nn/2, start+stride);
  * unrolled here from the top level (k = 0, bit n/2) downwards. */
-int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
-             uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst) {
+static int rmat_impl(uint32_t n, uint32_t seed, double a, double b, double c,
+                     uint64_t first, uint64_t count, int normalized,
+                     uint32_t *src, uint32_t *dst) {
   double ab = a + b, abc = a + b + c;
   if (abc > 1) return -1;
   if (n == 0 || (n & (n - 1)) != 0) return -1;
@@ -107,7 +115,8 @@ int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
     uint32_t r_out = 0, c_out = 0;
     uint32_t nn = n;
     for (uint32_t k = 0; k < lg; k++) {
-      double r = rmat_hash_double(start + k * stride);
+      double r = normalized ? rmat_hash_unit(start + k * stride)
+                            : rmat_hash_double(start + k * stride);
       uint32_t half = nn >> 1;
       if (r < a) {
       } else if (r < ab) {
@@ -124,6 +133,17 @@ int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
     dst[t] = c_out;
   }
   return 0;
+}
+
+int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
+             uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst) {
+  return rmat_impl(n, seed, a, b, c, first, count, 0, src, dst);
+}
+
+int orc_rmat_normalized(uint32_t n, uint32_t seed, double a, double b,
+                        double c, uint64_t first, uint64_t count,
+                        uint32_t *src, uint32_t *dst) {
+  return rmat_impl(n, seed, a, b, c, first, count, 1, src, dst);
 }
 
 /* Seeded Erdos-Renyi edge stream (no reference counterpart; the reference's

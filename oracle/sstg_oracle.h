@@ -27,6 +27,11 @@ uint32_t orc_log2_up(uint64_t i);                /* rmat_util.h:72-80 */
 /* rMat<uint32_t>(n, seed, a, b, c)(i) for i in [first, first+count).
  * Returns 0, or -1 when n is not a power of two or a+b+c > 1
  * (rmat_util.h:92-99 abort()s there). */
+/* The same recursion with r = hash32 / 2^32 (quadrant probabilities as
+ * given); no reference counterpart. */
+int orc_rmat_normalized(uint32_t n, uint32_t seed, double a, double b,
+                        double c, uint64_t first, uint64_t count,
+                        uint32_t *src, uint32_t *dst);
 int orc_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
              uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst);
 

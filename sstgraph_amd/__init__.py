@@ -41,6 +41,7 @@ SSTG_SERVE_ONE_HEAVY = 0x4
 SSTG_SEMI_SORTED = 0x2
 SSTG_KEEP_SELF_LOOPS = 0x4
 SSTG_LOAD_WEIGHTS = 0x8
+SSTG_RMAT_NORMALIZED = 0x10
 SSTG_TUNE_PR = 0x1
 SSTG_TUNE_BFS = 0x2
 
@@ -284,18 +285,21 @@ class DeviceArray:
             pass
 
 
-def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1, device=False):
+def rmat(n, seed, count, first=0, a=0.5, b=0.1, c=0.1, device=False,
+         normalized=False):
     """Edges [first, first+count) of rMat<uint32_t>(n, seed, a, b, c)
     (rmat_util.h:82-131), generated on the GPU.  device=True returns a pair
-    of DeviceArray."""
+    of DeviceArray; normalized=True draws in [0, 1) (quadrant probabilities
+    a, b, c as given) instead of the reference's [0, 2)."""
+    nf = SSTG_RMAT_NORMALIZED if normalized else 0
     if device:
         s, d = DeviceArray(count, np.uint32), DeviceArray(count, np.uint32)
         _check(lib().sstg_rmat(n, seed, a, b, c, first, count, s.ptr, d.ptr,
-                               SSTG_DEVICE_PTRS), "sstg_rmat")
+                               SSTG_DEVICE_PTRS | nf), "sstg_rmat")
         return s, d
     s = np.empty(count, np.uint32)
     d = np.empty(count, np.uint32)
-    _check(lib().sstg_rmat(n, seed, a, b, c, first, count, _ptr(s), _ptr(d), 0),
+    _check(lib().sstg_rmat(n, seed, a, b, c, first, count, _ptr(s), _ptr(d), nf),
            "sstg_rmat")
     return s, d
 

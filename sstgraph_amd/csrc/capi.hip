@@ -569,18 +569,20 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
               uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
               uint32_t flags) {
   if (count && (!src || !dst)) return SSTG_E_INVAL;
+  if (flags & ~(SSTG_DEVICE_PTRS | SSTG_RMAT_NORMALIZED)) return SSTG_E_INVAL;
+  const bool norm = (flags & SSTG_RMAT_NORMALIZED) != 0;
   int dev = 0;
   if (!device_ok(&dev)) return SSTG_E_NODEVICE;
   hipStream_t s = nullptr;
   if (flags & SSTG_DEVICE_PTRS) {
-    SSTG_TRY(rmat_generate(n, seed, a, b, c, first, count, src, dst, s));
+    SSTG_TRY(rmat_generate(n, seed, a, b, c, first, count, src, dst, s, norm));
     SSTG_HIP(hipStreamSynchronize(s));
     return SSTG_OK;
   }
   uint32_t *ds = nullptr, *dd = nullptr;
   SSTG_HIP(hipMalloc(&ds, (count ? count : 1) * sizeof(uint32_t)));
   SSTG_HIP(hipMalloc(&dd, (count ? count : 1) * sizeof(uint32_t)));
-  int rc = rmat_generate(n, seed, a, b, c, first, count, ds, dd, s);
+  int rc = rmat_generate(n, seed, a, b, c, first, count, ds, dd, s, norm);
   if (rc == SSTG_OK && count) {
     hipMemcpy(src, ds, count * sizeof(uint32_t), hipMemcpyDeviceToHost);
     hipMemcpy(dst, dd, count * sizeof(uint32_t), hipMemcpyDeviceToHost);

@@ -91,7 +91,7 @@ int er_generate(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
                 uint32_t *src, uint32_t *dst, hipStream_t s);
 int rmat_generate(uint32_t n, uint32_t seed, double a, double b, double c,
                   uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
-                  hipStream_t s);
+                  hipStream_t s, bool normalized = false);
 
 // Workspace slot map of one graph (each user owns a range of slots).
 enum : int {

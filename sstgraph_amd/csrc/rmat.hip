@@ -1,5 +1,7 @@
 // Device RMAT edge generator, bit-exact with rMat<uint32_t>
-// (include/SSTGraph/internal/rmat_util.h:82-131).
+// (include/SSTGraph/internal/rmat_util.h:82-131).  A normalised variant
+// draws r = hash32(x) / 2^32 instead, so the quadrant probabilities are the
+// a, b, c given rather than half of them.
 //
 // The reference draws r = hash32(x) / INT32_MAX in double and compares it
 // with a, a+b, a+b+c.  The quotient is monotone in hash32(x), so each
@@ -32,13 +34,15 @@ uint32_t hash32_host(uint32_t a) {
   return a;
 }
 
-// smallest h in [0, 2^32] with (double)h / INT32_MAX >= p (2^32: none);
-// mid < 2^32 throughout the search
-uint64_t threshold(double p) {
+// smallest h in [0, 2^32] with (double)h / D >= p (2^32: none), where D is
+// INT32_MAX (the reference's draw) or 2^32 (the normalised one); mid < 2^32
+// throughout the search
+uint64_t threshold(double p, bool normalized) {
+  const double D = normalized ? 4294967296.0 : (double)2147483647;
   uint64_t lo = 0, hi = 1ull << 32;
   while (lo < hi) {
     uint64_t mid = (lo + hi) >> 1;
-    if ((double)(uint32_t)mid / (double)2147483647 >= p)
+    if ((double)(uint32_t)mid / D >= p)
       hi = mid;
     else
       lo = mid + 1;
@@ -109,7 +113,7 @@ int er_generate(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
 
 int rmat_generate(uint32_t n, uint32_t seed, double a, double b, double c,
                   uint64_t first, uint64_t count, uint32_t *src, uint32_t *dst,
-                  hipStream_t s) {
+                  hipStream_t s, bool normalized) {
   double ab = a + b, abc = a + b + c;
   if (abc > 1) return SSTG_E_INVAL;                      // rmat_util.h:92-95
   if (n == 0 || (n & (n - 1)) != 0) return SSTG_E_INVAL;  // rmat_util.h:96-99
@@ -117,8 +121,9 @@ int rmat_generate(uint32_t n, uint32_t seed, double a, double b, double c,
   uint32_t lg = 0;
   while ((1u << lg) < n) lg++;
   uint32_t h = hash32_host(seed);
-  SSTG_LAUNCH(k_rmat, ceil_div(count, 256), 256, 0, s, n, lg, h, threshold(a),
-              threshold(ab), threshold(abc), first, count, src, dst);
+  SSTG_LAUNCH(k_rmat, ceil_div(count, 256), 256, 0, s, n, lg, h,
+              threshold(a, normalized), threshold(ab, normalized),
+              threshold(abc, normalized), first, count, src, dst);
   return SSTG_OK;
 }
 

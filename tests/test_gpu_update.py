@@ -30,6 +30,14 @@ def test_device_rmat_matches_reference_vectors():
     s, d = S.rmat(1 << 20, 12, 1 << 20)
     os_, od = O.rmat(1 << 20, 12, 1 << 20)
     assert np.array_equal(s, os_) and np.array_equal(d, od)
+    # the normalised draw (no reference counterpart) against the oracle's
+    s, d = S.rmat(1 << 20, 12, 1 << 20, first=77, normalized=True)
+    os_, od = O.rmat(1 << 20, 12, 1 << 20, first=77, normalized=True)
+    assert np.array_equal(s, os_) and np.array_equal(d, od)
+    for a, b, c in ((0.57, 0.19, 0.19), (0.25, 0.25, 0.25), (1.0, 0.0, 0.0)):
+        s, d = S.rmat(1 << 12, 3, 50000, a=a, b=b, c=c, normalized=True)
+        os_, od = O.rmat(1 << 12, 3, 50000, a=a, b=b, c=c, normalized=True)
+        assert np.array_equal(s, os_) and np.array_equal(d, od), (a, b, c)
 
 
 @pytest.mark.parametrize("n,sizes", [(1000, [1, 7, 100, 101, 1000, 20000]),

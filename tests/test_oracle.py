@@ -53,6 +53,39 @@ def test_rmat_effective_quadrants():
     assert s.max() < n and d.max() < n
 
 
+def test_rmat_normalized_quadrants_and_draws():
+    """The normalised draw r = hash32 / 2^32 gives the stated probabilities;
+    a few edges recomputed bit by bit from hash32 (itself pinned by the
+    reference build's vectors) fix the recursion order and the scaling."""
+    n = 1 << 20
+    s, d = O.rmat(n, 9, 400_000, normalized=True)
+    top_r, top_c = s >= n // 2, d >= n // 2
+    q = [np.mean(~top_r & ~top_c), np.mean(~top_r & top_c),
+         np.mean(top_r & ~top_c), np.mean(top_r & top_c)]
+    np.testing.assert_allclose(q, [0.5, 0.1, 0.1, 0.3], atol=0.005)
+    m32 = 0xFFFFFFFF
+    h = O.hash32(9)
+    for i in (0, 1, 12345, 399_999):
+        start = O.hash32((2 * i * h) & m32)
+        stride = O.hash32(((2 * i + 1) * h) & m32)
+        r = c = 0
+        for k in range(20):
+            half = n >> (k + 1)
+            x = O.hash32((start + k * stride) & m32) / 2.0**32
+            if x < 0.5:
+                pass
+            elif x < 0.6:
+                c += half
+            elif x < 0.7:
+                r += half
+            else:
+                r += half
+                c += half
+        assert (int(s[i]), int(d[i])) == (r, c), i
+    s2, d2 = O.rmat(n, 9, 1000)
+    assert not np.array_equal(s2, s[:1000])
+
+
 def test_rmat_rejects_non_power_of_two():
     with pytest.raises(ValueError):
         O.rmat(1000, 0, 4)
