@@ -576,9 +576,11 @@ def main():
                        pagerank_gteps=m * args.pr_iters / tp / 1e9,
                        pagerank_gbs_model=(4 * m + 24 * n) / (tp / args.pr_iters) / 1e9)
             extra["autotune"] = rep
-            log(f"autotuned thresholds {rep['thresholds']} ({rep['evaluations']} settings): "
+            log(f"autotuned thresholds {rep['thresholds']}, BFS switch alpha "
+                f"{rep['alpha']:g} beta {rep['beta']:g} ({rep['evaluations']} settings): "
                 f"BFS hub {1e3 * tb:.3f} ms, PR x{args.pr_iters} {1e3 * tp:.2f} ms")
             g.set_degree_thresholds([8, 64, 2048])
+            g.set_bfs_switch(15, 18)
         for d in (d_par, d_lev, d_pr):
             d.free()
 

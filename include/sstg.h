@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define SSTG_ABI_VERSION 1
+#define SSTG_ABI_VERSION 2
 
 /* status codes */
 #define SSTG_OK 0
@@ -295,18 +295,27 @@ int sstg_statistics(sstg_graph *g, sstg_stats *out);
  * increasing).  Per graph; a mutating call. */
 int sstg_set_degree_thresholds(sstg_graph *g, const uint32_t t[3]);
 int sstg_get_degree_thresholds(sstg_graph *g, uint32_t t[3]);
+/* BFS direction switch (Beamer): top-down -> bottom-up when the frontier
+ * grows and its edges exceed unexplored / alpha, back when it holds fewer
+ * than n / beta vertices (defaults 15 and 18; both > 0).  Per graph. */
+int sstg_set_bfs_switch(sstg_graph *g, double alpha, double beta);
+int sstg_get_bfs_switch(sstg_graph *g, double *alpha, double *beta);
 #define SSTG_TUNE_PR 1u  /* objective: pr_iters FP64 PageRank iterations */
 #define SSTG_TUNE_BFS 2u /* objective: one BFS from src */
 typedef struct sstg_tune_report {
   uint32_t thresholds[3];       /* chosen (now set on the graph) */
   uint32_t start_thresholds[3]; /* in effect before tuning */
+  double alpha, beta;           /* chosen BFS switch */
+  double start_alpha, start_beta;
   double start_ms, best_ms;     /* objective time, start and chosen */
   uint32_t evaluations;         /* candidate settings timed */
 } sstg_tune_report;
 /* Times the objective (what = SSTG_TUNE_PR and/or SSTG_TUNE_BFS; pr_iters 0
- * means 2) on this graph over a grid of thresholds by coordinate descent
- * and leaves the fastest set on the graph.  Results of the algorithms are
- * unchanged except for the summation order of PageRank. */
+ * means 2) on this graph over a grid of degree thresholds and, with a BFS
+ * objective, of the BFS switch, by coordinate descent, and leaves the
+ * fastest setting on the graph.  Results of the algorithms are unchanged
+ * except for the summation order of PageRank and the choice among valid BFS
+ * parents. */
 int sstg_autotune(sstg_graph *g, uint32_t what, uint32_t src,
                   uint32_t pr_iters, sstg_tune_report *out);
 /* get_edges_from_file (helpers.hpp:604-631): .adj / .adj.shuf

@@ -77,12 +77,16 @@ class TuneReport(ctypes.Structure):
     """sstg_tune_report (the degree-threshold autotuner's outcome)."""
     _fields_ = [("thresholds", ctypes.c_uint32 * 3),
                 ("start_thresholds", ctypes.c_uint32 * 3),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("start_alpha", ctypes.c_double), ("start_beta", ctypes.c_double),
                 ("start_ms", ctypes.c_double), ("best_ms", ctypes.c_double),
                 ("evaluations", ctypes.c_uint32)]
 
     def as_dict(self):
         return {"thresholds": list(self.thresholds),
                 "start_thresholds": list(self.start_thresholds),
+                "alpha": self.alpha, "beta": self.beta,
+                "start_alpha": self.start_alpha, "start_beta": self.start_beta,
                 "start_ms": self.start_ms, "best_ms": self.best_ms,
                 "evaluations": int(self.evaluations)}
 
@@ -188,6 +192,8 @@ def lib():
         "sstg_set_degree_thresholds": (i32, [vp, vp]),
         "sstg_get_degree_thresholds": (i32, [vp, vp]),
         "sstg_autotune": (i32, [vp, u32, u32, u32, P(TuneReport)]),
+        "sstg_set_bfs_switch": (i32, [vp, ctypes.c_double, ctypes.c_double]),
+        "sstg_get_bfs_switch": (i32, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
         "sstg_edges_from_file": (i32, [ctypes.c_char_p, P(u32), P(u64), P(vp),
                                        P(vp), P(vp)]),
         "sstg_host_free": (None, [vp]),
@@ -462,6 +468,16 @@ class SparseMatrixV:
         if t.size != 3:
             raise ValueError("three thresholds")
         _check(lib().sstg_set_degree_thresholds(self._h, _ptr(t)), "thresholds")
+
+    def bfs_switch(self):
+        """(alpha, beta) of the BFS direction switch."""
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        _check(lib().sstg_get_bfs_switch(self._h, ctypes.byref(a), ctypes.byref(b)),
+               "bfs_switch")
+        return a.value, b.value
+
+    def set_bfs_switch(self, alpha, beta):
+        _check(lib().sstg_set_bfs_switch(self._h, alpha, beta), "bfs_switch")
 
     def autotune(self, pr=True, bfs_source=None, pr_iters=2):
         """Picks the degree thresholds by timing PageRank iterations and/or a

@@ -289,8 +289,9 @@ __global__ void k_bc_init(uint32_t n, double *sigma, double *delta,
 // ====================================================================== BFS
 // Direction-optimising BFS (Beamer): top-down push over a compacted queue
 // while the frontier is small, bottom-up pull over a frontier bitmap once the
-// frontier's edges exceed 1/alpha of the unexplored edges, back to top-down
-// when the frontier shrinks below n/beta.  Records the BFS order per level
+// frontier grows and its edges exceed 1/alpha of the unexplored edges, back
+// to top-down when the frontier shrinks below n/beta (alpha and beta are
+// per-graph settings, defaults 15 and 18).  Records the BFS order per level
 // in `order` (level starts in level_start) when requested (BC uses it).
 int graph_is_symmetric(sstg_graph *g, bool *out) {
   if (g->sym_version == g->version) {
@@ -350,7 +351,8 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   uint64_t qs = 0, nf = 1, mf = src_deg;
   uint64_t explored = src_deg;  // edges of visited vertices
   bool bottom_up = false;
-  const double alpha = 15.0, beta = 18.0;
+  const double alpha = g->bfs_alpha, beta = g->bfs_beta;
+  uint64_t prev_nf = 0;
   if (run) {  // level 0 is queue[0, 1); each step appends the end of the next
     run->level_start.clear();
     run->level_start.push_back(0);
@@ -367,9 +369,12 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   auto t_level = std::chrono::steady_clock::now();
   while (nf > 0) {
     uint64_t unexplored = g->m > explored ? g->m - explored : 0;
+    // the unexplored-edge estimate runs out in the tail, so a shrinking
+    // frontier never switches to bottom-up
     bool want_bu = !top_down_only &&
                    (bottom_up ? (double)nf >= (double)n / beta
-                              : (double)mf > (double)unexplored / alpha);
+                              : nf > prev_nf &&
+                                    (double)mf > (double)unexplored / alpha);
     EdgeMapResult res;
     if (want_bu) {
       if (!bottom_up) SSTG_TRY(frontier_to_bitmap(ctx, queue + qs, nf, s));
@@ -406,6 +411,7 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
       t_level = t1;
     }
     qs += nf;
+    prev_nf = nf;
     nf = res.count;
     mf = res.degree_sum;
     explored += mf;

@@ -349,6 +349,18 @@ int sstg_get_degree_thresholds(sstg_graph *g, uint32_t t[3]) {
   return SSTG_OK;
 }
 
+int sstg_set_bfs_switch(sstg_graph *g, double alpha, double beta) {
+  if (!g) return SSTG_E_INVAL;
+  return graph_set_bfs_switch(g, alpha, beta);
+}
+
+int sstg_get_bfs_switch(sstg_graph *g, double *alpha, double *beta) {
+  if (!g || !alpha || !beta) return SSTG_E_INVAL;
+  *alpha = g->bfs_alpha;
+  *beta = g->bfs_beta;
+  return SSTG_OK;
+}
+
 int sstg_autotune(sstg_graph *g, uint32_t what, uint32_t src,
                   uint32_t pr_iters, sstg_tune_report *out) {
   if (!g) return SSTG_E_INVAL;

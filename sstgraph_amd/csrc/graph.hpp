@@ -35,6 +35,10 @@ struct sstg_graph {
   uint64_t bins_count[4] = {0, 0, 0, 0};
   uint32_t bin_hi[3] = {8, 64, 2048};
   uint64_t tune_epoch = 0;
+  // BFS direction switch (Beamer): top-down -> bottom-up when the frontier
+  // grows and its edges exceed unexplored / alpha; back when it holds fewer
+  // than n / beta vertices
+  double bfs_alpha = 15.0, bfs_beta = 18.0;
   // cached: does every stored edge have its reverse?
   uint64_t sym_version = ~0ull;
   bool symmetric = false;
@@ -69,6 +73,7 @@ int graph_touch_sum(sstg_graph *g, uint64_t *out);
 int graph_is_symmetric(sstg_graph *g, bool *out);
 // tune.hip
 int graph_set_thresholds(sstg_graph *g, const uint32_t t[3]);
+int graph_set_bfs_switch(sstg_graph *g, double alpha, double beta);
 int graph_autotune(sstg_graph *g, uint32_t what, uint32_t src,
                    uint32_t pr_iters, sstg_tune_report *rep);
 // weights.hip (device arrays)

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_strerror():
-    assert S.lib().sstg_abi_version() == 1
+    assert S.lib().sstg_abi_version() == 2
     for code in [0, -1, -2, -3, -5, -6]:
         assert S.strerror(code)
     assert S.strerror(-2) == "vertex id out of range"

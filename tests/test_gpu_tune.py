@@ -55,6 +55,31 @@ def test_threshold_settings_keep_results_exact(g14):
         _check_all(g, off, cols, str(t))
 
 
+def test_bfs_switch_settings_keep_levels_exact(g14):
+    g, off, cols = g14
+    assert g.bfs_switch() == (15.0, 18.0)
+    hub = int(np.argmax(np.diff(off)))
+    olev, _ = O.bfs(off, cols, hub)
+    for a, b in ((1e-3, 1e-3), (1e6, 1e6), (4, 6), (60, 36), (15, 18)):
+        g.set_bfs_switch(a, b)
+        assert g.bfs_switch() == (a, b)
+        parent, lev = EMV.BFS(g, hub, levels=True)
+        assert np.array_equal(lev, olev), (a, b)
+        # parents are one level up and adjacent
+        reached = np.nonzero(lev != S.UNREACHED)[0]
+        for v in reached[:: max(1, len(reached) // 500)]:
+            p = int(parent[v])
+            if v == hub:
+                assert p == hub
+                continue
+            assert lev[p] + 1 == lev[v]
+            assert v in set(cols[off[p]:off[p + 1]].tolist())
+    for a, b in ((0, 18), (15, -1), (float("nan"), 18)):
+        with pytest.raises(S.SSTGError):
+            g.set_bfs_switch(a, b)
+    assert g.bfs_switch() == (15.0, 18.0)
+
+
 def test_invalid_thresholds_rejected(g14):
     g, _, _ = g14
     for t in ([8, 8, 2048], [64, 8, 2048], [1, 2, 2]):
@@ -76,10 +101,13 @@ def test_autotune_picks_a_valid_faster_setting(g14):
     assert t[0] < t[1] < t[2]
     assert g.degree_thresholds() == t
     assert rep["start_thresholds"] == [8, 64, 2048]
+    assert (rep["start_alpha"], rep["start_beta"]) == (15.0, 18.0)
+    assert g.bfs_switch() == (rep["alpha"], rep["beta"])
     assert rep["evaluations"] >= 10
     assert rep["best_ms"] <= rep["start_ms"]
     _check_all(g, off, cols, f"tuned {t}")
     g.set_degree_thresholds([8, 64, 2048])
+    g.set_bfs_switch(15, 18)
 
 
 def test_multigraph_recaptures_after_tuning():
