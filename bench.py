@@ -204,7 +204,7 @@ def run_batched(args):
         # degree thresholds of the union store, tuned on PR iterations
         tune = M.autotune()
         log(f"autotuned thresholds {tune['thresholds']}: PR x2 "
-            f"{tune['start_ms']:.2f} -> {tune['best_ms']:.2f} ms")
+            f"{tune['pr_start_ms']:.2f} -> {tune['pr_best_ms']:.2f} ms")
     # batched: one hipGraph replay per step
     for _ in range(args.warmup):
         M.run(sources, pr_iters=iters)
@@ -576,10 +576,12 @@ def main():
                        pagerank_gteps=m * args.pr_iters / tp / 1e9,
                        pagerank_gbs_model=(4 * m + 24 * n) / (tp / args.pr_iters) / 1e9)
             extra["autotune"] = rep
-            log(f"autotuned thresholds {rep['thresholds']}, BFS switch alpha "
-                f"{rep['alpha']:g} beta {rep['beta']:g} ({rep['evaluations']} settings): "
-                f"BFS hub {1e3 * tb:.3f} ms, PR x{args.pr_iters} {1e3 * tp:.2f} ms")
+            log(f"autotuned thresholds {rep['thresholds']}, BFS {rep['bfs_thresholds']} "
+                f"alpha {rep['alpha']:g} beta {rep['beta']:g} ({rep['evaluations']} "
+                f"settings): BFS hub {1e3 * tb:.3f} ms, PR x{args.pr_iters} "
+                f"{1e3 * tp:.2f} ms")
             g.set_degree_thresholds([8, 64, 2048])
+            g.set_degree_thresholds([8, 64, 2048], bfs=True)
             g.set_bfs_switch(15, 18)
         for d in (d_par, d_lev, d_pr):
             d.free()

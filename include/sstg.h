@@ -292,9 +292,13 @@ int sstg_statistics(sstg_graph *g, sstg_stats *out);
  * by its PMA layout, TinySet_small.hpp:216-293).  The vertex-pull kernels
  * give a vertex 1 lane if its degree is <= t[0], 8 lanes if <= t[1], a wave
  * if <= t[2], else a 256-thread block (defaults 8, 64, 2048; t strictly
- * increasing).  Per graph; a mutating call. */
+ * increasing).  Two sets per graph: the reductions (PageRank, BC, TC's
+ * orientation, the batched store's BFS levels) and BFS (bottom-up steps).  Mutating
+ * calls. */
 int sstg_set_degree_thresholds(sstg_graph *g, const uint32_t t[3]);
 int sstg_get_degree_thresholds(sstg_graph *g, uint32_t t[3]);
+int sstg_set_bfs_degree_thresholds(sstg_graph *g, const uint32_t t[3]);
+int sstg_get_bfs_degree_thresholds(sstg_graph *g, uint32_t t[3]);
 /* BFS direction switch (Beamer): top-down -> bottom-up when the frontier
  * grows and its edges exceed unexplored / alpha, back when it holds fewer
  * than n / beta vertices (defaults 15 and 18; both > 0).  Per graph. */
@@ -303,19 +307,22 @@ int sstg_get_bfs_switch(sstg_graph *g, double *alpha, double *beta);
 #define SSTG_TUNE_PR 1u  /* objective: pr_iters FP64 PageRank iterations */
 #define SSTG_TUNE_BFS 2u /* objective: one BFS from src */
 typedef struct sstg_tune_report {
-  uint32_t thresholds[3];       /* chosen (now set on the graph) */
-  uint32_t start_thresholds[3]; /* in effect before tuning */
-  double alpha, beta;           /* chosen BFS switch */
+  uint32_t thresholds[3];           /* reductions set, chosen (now set) */
+  uint32_t start_thresholds[3];     /* reductions set before tuning */
+  uint32_t bfs_thresholds[3];       /* BFS set, chosen */
+  uint32_t start_bfs_thresholds[3]; /* BFS set before tuning */
+  double alpha, beta;               /* BFS switch, chosen */
   double start_alpha, start_beta;
-  double start_ms, best_ms;     /* objective time, start and chosen */
-  uint32_t evaluations;         /* candidate settings timed */
+  double pr_start_ms, pr_best_ms;   /* PageRank objective (0 if not tuned) */
+  double bfs_start_ms, bfs_best_ms; /* BFS objective (0 if not tuned) */
+  uint32_t evaluations;             /* settings timed */
 } sstg_tune_report;
-/* Times the objective (what = SSTG_TUNE_PR and/or SSTG_TUNE_BFS; pr_iters 0
- * means 2) on this graph over a grid of degree thresholds and, with a BFS
- * objective, of the BFS switch, by coordinate descent, and leaves the
- * fastest setting on the graph.  Results of the algorithms are unchanged
- * except for the summation order of PageRank and the choice among valid BFS
- * parents. */
+/* With SSTG_TUNE_PR, times pr_iters PageRank iterations (0 means 2) over a
+ * grid of the reductions' thresholds; with SSTG_TUNE_BFS, one BFS from src
+ * over a grid of the BFS thresholds and switch; coordinate descent each, and
+ * the fastest settings are left on the graph.  Results of the algorithms are
+ * unchanged except for the summation order of PageRank and the choice among
+ * valid BFS parents. */
 int sstg_autotune(sstg_graph *g, uint32_t what, uint32_t src,
                   uint32_t pr_iters, sstg_tune_report *out);
 /* get_edges_from_file (helpers.hpp:604-631): .adj / .adj.shuf

@@ -77,18 +77,21 @@ class TuneReport(ctypes.Structure):
     """sstg_tune_report (the degree-threshold autotuner's outcome)."""
     _fields_ = [("thresholds", ctypes.c_uint32 * 3),
                 ("start_thresholds", ctypes.c_uint32 * 3),
+                ("bfs_thresholds", ctypes.c_uint32 * 3),
+                ("start_bfs_thresholds", ctypes.c_uint32 * 3),
                 ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
                 ("start_alpha", ctypes.c_double), ("start_beta", ctypes.c_double),
-                ("start_ms", ctypes.c_double), ("best_ms", ctypes.c_double),
+                ("pr_start_ms", ctypes.c_double), ("pr_best_ms", ctypes.c_double),
+                ("bfs_start_ms", ctypes.c_double), ("bfs_best_ms", ctypes.c_double),
                 ("evaluations", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {"thresholds": list(self.thresholds),
-                "start_thresholds": list(self.start_thresholds),
-                "alpha": self.alpha, "beta": self.beta,
-                "start_alpha": self.start_alpha, "start_beta": self.start_beta,
-                "start_ms": self.start_ms, "best_ms": self.best_ms,
-                "evaluations": int(self.evaluations)}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        for k in ("thresholds", "start_thresholds", "bfs_thresholds",
+                  "start_bfs_thresholds"):
+            d[k] = list(d[k])
+        d["evaluations"] = int(d["evaluations"])
+        return d
 
 
 def _autotune(handle, pr, bfs_source, pr_iters):
@@ -193,6 +196,8 @@ def lib():
         "sstg_get_degree_thresholds": (i32, [vp, vp]),
         "sstg_autotune": (i32, [vp, u32, u32, u32, P(TuneReport)]),
         "sstg_set_bfs_switch": (i32, [vp, ctypes.c_double, ctypes.c_double]),
+        "sstg_set_bfs_degree_thresholds": (i32, [vp, vp]),
+        "sstg_get_bfs_degree_thresholds": (i32, [vp, vp]),
         "sstg_get_bfs_switch": (i32, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
         "sstg_edges_from_file": (i32, [ctypes.c_char_p, P(u32), P(u64), P(vp),
                                        P(vp), P(vp)]),
@@ -457,17 +462,22 @@ class SparseMatrixV:
                "rewrite_graph")
         return SparseMatrixV(self.n_rows, self.n_cols, _handle=h)
 
-    def degree_thresholds(self):
-        """Degree boundaries of the 1-lane / 8-lane / wave / block groups."""
+    def degree_thresholds(self, bfs=False):
+        """Degree boundaries of the 1-lane / 8-lane / wave / block groups of
+        the reductions (PageRank, BC, TC), or of BFS with bfs=True."""
         t = np.zeros(3, np.uint32)
-        _check(lib().sstg_get_degree_thresholds(self._h, _ptr(t)), "thresholds")
+        fn = (lib().sstg_get_bfs_degree_thresholds if bfs
+              else lib().sstg_get_degree_thresholds)
+        _check(fn(self._h, _ptr(t)), "thresholds")
         return t.tolist()
 
-    def set_degree_thresholds(self, t):
+    def set_degree_thresholds(self, t, bfs=False):
         t = _u32(t)
         if t.size != 3:
             raise ValueError("three thresholds")
-        _check(lib().sstg_set_degree_thresholds(self._h, _ptr(t)), "thresholds")
+        fn = (lib().sstg_set_bfs_degree_thresholds if bfs
+              else lib().sstg_set_degree_thresholds)
+        _check(fn(self._h, _ptr(t)), "thresholds")
 
     def bfs_switch(self):
         """(alpha, beta) of the BFS direction switch."""

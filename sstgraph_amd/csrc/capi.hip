@@ -340,12 +340,23 @@ int sstg_edge_weights(sstg_graph *g, const uint32_t *rows,
 
 int sstg_set_degree_thresholds(sstg_graph *g, const uint32_t t[3]) {
   if (!g || !t) return SSTG_E_INVAL;
-  return graph_set_thresholds(g, t);
+  return graph_set_thresholds(g, kBinsPull, t);
 }
 
 int sstg_get_degree_thresholds(sstg_graph *g, uint32_t t[3]) {
   if (!g || !t) return SSTG_E_INVAL;
-  for (int i = 0; i < 3; i++) t[i] = g->bin_hi[i];
+  for (int i = 0; i < 3; i++) t[i] = g->bin_hi[kBinsPull][i];
+  return SSTG_OK;
+}
+
+int sstg_set_bfs_degree_thresholds(sstg_graph *g, const uint32_t t[3]) {
+  if (!g || !t) return SSTG_E_INVAL;
+  return graph_set_thresholds(g, kBinsBfs, t);
+}
+
+int sstg_get_bfs_degree_thresholds(sstg_graph *g, uint32_t t[3]) {
+  if (!g || !t) return SSTG_E_INVAL;
+  for (int i = 0; i < 3; i++) t[i] = g->bin_hi[kBinsBfs][i];
   return SSTG_OK;
 }
 

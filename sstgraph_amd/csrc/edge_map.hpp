@@ -462,10 +462,10 @@ int edge_map_sparse(sstg_graph *g, EdgeMapCtx &ctx, const uint32_t *front,
 }
 
 // ----------------------------------------------------------- degree bins
-// Vertices split by degree for the vertex-pull kernels (cached per graph
-// version): bin 0 deg <= bin_hi[0] (one lane), bin 1 <= bin_hi[1] (8 lanes),
-// bin 2 <= bin_hi[2] (a wave), bin 3 larger (a block); the boundaries
-// default to 8 / 64 / 2048 and are set per graph by the autotuner.
+// Vertices split by degree (cached per graph version and threshold set):
+// bin 0 deg <= t[0] (one lane), bin 1 <= t[1] (8 lanes), bin 2 <= t[2] (a
+// wave), bin 3 larger (a block), t = bin_hi[set]; the boundaries default to
+// 8 / 64 / 2048 and are set per graph by the autotuner.
 struct DegreeBins {
   uint32_t *list[4] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t count[4] = {0, 0, 0, 0};
@@ -484,17 +484,17 @@ __global__ void k_bin_flags(const uint32_t *deg, uint32_t n, uint32_t lo,
 }  // namespace
 }  // namespace em
 
-inline int degree_bins(sstg_graph *g, DegreeBins *out) {
-  const uint32_t lo[4] = {0, g->bin_hi[0] + 1, g->bin_hi[1] + 1,
-                          g->bin_hi[2] + 1};
-  const uint32_t hi[4] = {g->bin_hi[0], g->bin_hi[1], g->bin_hi[2],
-                          0xFFFFFFFFu};
+inline int degree_bins(sstg_graph *g, DegreeBins *out, int set = kBinsPull) {
+  const uint32_t *t = g->bin_hi[set];
+  const uint32_t lo[4] = {0, t[0] + 1, t[1] + 1, t[2] + 1};
+  const uint32_t hi[4] = {t[0], t[1], t[2], 0xFFFFFFFFu};
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
+  const int slot0 = set == kBinsPull ? kWsAlgo + 26 : kWsBfsBins;
   for (int b = 0; b < 4; b++)
-    SSTG_TRY(g->ws.get(kWsAlgo + 26 + b, (uint64_t)n + 1, &out->list[b]));
-  if (g->bins_version == g->version) {
-    for (int b = 0; b < 4; b++) out->count[b] = g->bins_count[b];
+    SSTG_TRY(g->ws.get(slot0 + b, (uint64_t)n + 1, &out->list[b]));
+  if (g->bins_version[set] == g->version) {
+    for (int b = 0; b < 4; b++) out->count[b] = g->bins_count[set][b];
     return SSTG_OK;
   }
   uint8_t *flags = nullptr;
@@ -512,8 +512,8 @@ inline int degree_bins(sstg_graph *g, DegreeBins *out) {
     out->count[b] = g->host_rb[16];
     SSTG_TRY(narrow_u64_to_u32(tmp, out->count[b], out->list[b], s));
   }
-  for (int b = 0; b < 4; b++) g->bins_count[b] = out->count[b];
-  g->bins_version = g->version;
+  for (int b = 0; b < 4; b++) g->bins_count[set][b] = out->count[b];
+  g->bins_version[set] = g->version;
   return SSTG_OK;
 }
 
@@ -524,7 +524,7 @@ template <class F>
 int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
   hipStream_t s = g->stream;
   DegreeBins bins;
-  SSTG_TRY(degree_bins(g, &bins));
+  SSTG_TRY(degree_bins(g, &bins, kBinsBfs));
   // the first bottom-up step reads the degree bins; later ones the vertices
   // the previous step left unvisited
   const bool first = ctx.act_src < 0;

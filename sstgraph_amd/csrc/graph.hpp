@@ -28,12 +28,14 @@ struct sstg_graph {
   // cached derived data: exclusive prefix of deg (n_rows + 1 entries)
   uint64_t *deg_prefix = nullptr;
   uint64_t deg_prefix_version = ~0ull;
-  // cached degree bins of the vertex-pull kernels (lists live in workspace):
-  // bin b holds degrees in (bin_hi[b-1], bin_hi[b]], the last bin the rest;
-  // tune.hip sets the boundaries, tune_epoch counts the changes
-  uint64_t bins_version = ~0ull;
-  uint64_t bins_count[4] = {0, 0, 0, 0};
-  uint32_t bin_hi[3] = {8, 64, 2048};
+  // cached degree bins (lists live in workspace), two sets: [kBinsPull] for
+  // the vertex-pull reductions (PageRank, BC, TC orientation, the batched
+  // store's BFS levels), [kBinsBfs] for the single-source BFS steps.  Bin b holds degrees in
+  // (bin_hi[set][b-1], bin_hi[set][b]], the last bin the rest; tune.hip sets
+  // the boundaries, tune_epoch counts the changes
+  uint64_t bins_version[2] = {~0ull, ~0ull};
+  uint64_t bins_count[2][4] = {};
+  uint32_t bin_hi[2][3] = {{8, 64, 2048}, {8, 64, 2048}};
   uint64_t tune_epoch = 0;
   // BFS direction switch (Beamer): top-down -> bottom-up when the frontier
   // grows and its edges exceed unexplored / alpha; back when it holds fewer
@@ -72,7 +74,7 @@ int graph_touch_sum(sstg_graph *g, uint64_t *out);
 // algo.hip
 int graph_is_symmetric(sstg_graph *g, bool *out);
 // tune.hip
-int graph_set_thresholds(sstg_graph *g, const uint32_t t[3]);
+int graph_set_thresholds(sstg_graph *g, int set, const uint32_t t[3]);
 int graph_set_bfs_switch(sstg_graph *g, double alpha, double beta);
 int graph_autotune(sstg_graph *g, uint32_t what, uint32_t src,
                    uint32_t pr_iters, sstg_tune_report *rep);
@@ -150,7 +152,14 @@ enum : int {
   kWsPrim2 = 64,     // 64..79 second primitive range
   kWsBottomUp = 112,  // 112..119 active lists of the bottom-up steps,
                       // 120 their keep flags, 121 per-block partial sums
-  kWsSlots = 122,
+  kWsBfsBins = 122,   // 122..125 degree bin lists of the BFS set
+  kWsSlots = 126,
 };
+
+// degree threshold sets of a graph (sstg_graph::bin_hi): the reductions
+// (PageRank, BC, TC orientation, the batched store's BFS levels) and the
+// single-source BFS steps
+constexpr int kBinsPull = 0;
+constexpr int kBinsBfs = 1;
 
 }  // namespace sstg

@@ -255,8 +255,10 @@ static int capture(sstg_multi *M, uint32_t max_levels, uint32_t iters,
   // everything that allocates or reads back happens before capture
   const uint64_t *prefix = nullptr;
   SSTG_TRY(graph_degree_prefix(g, &prefix));
+  // the level kernels and PageRank share the reductions' threshold set,
+  // which MultiGraph tuning sets on PageRank
   DegreeBins bins;
-  SSTG_TRY(degree_bins(g, &bins));
+  SSTG_TRY(degree_bins(g, &bins, kBinsPull));
   bool symmetric = false;
   SSTG_TRY(graph_is_symmetric(g, &symmetric));
   SSTG_HIP(hipStreamSynchronize(s));
@@ -469,7 +471,7 @@ int sstg_multi_run(sstg_multi *M, const uint32_t *sources, uint32_t max_levels,
     if (c == 0) break;
     if (L + 1 >= kMultiCounts) return SSTG_E_UNSUPPORTED;
     DegreeBins bins;
-    SSTG_TRY(degree_bins(g, &bins));
+    SSTG_TRY(degree_bins(g, &bins, kBinsPull));
     launch_level(M, bins, L, g->stream);
     SSTG_HIP(hipGetLastError());
     L++;

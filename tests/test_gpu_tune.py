@@ -48,11 +48,16 @@ def _check_all(g, off, cols, what):
 def test_threshold_settings_keep_results_exact(g14):
     g, off, cols = g14
     assert g.degree_thresholds() == [8, 64, 2048]
-    for t in ([1, 16, 256], [32, 512, 65536], [4, 32, 1024], [8, 64, 2048]):
-        say("thresholds", t)
+    assert g.degree_thresholds(bfs=True) == [8, 64, 2048]
+    sets = ([1, 16, 256], [32, 512, 65536], [4, 32, 1024], [8, 64, 2048])
+    for t, tb in zip(sets, sets[::-1]):
+        say("thresholds", t, "bfs", tb)
         g.set_degree_thresholds(t)
+        g.set_degree_thresholds(tb, bfs=True)
         assert g.degree_thresholds() == t
-        _check_all(g, off, cols, str(t))
+        assert g.degree_thresholds(bfs=True) == tb
+        _check_all(g, off, cols, f"{t} / bfs {tb}")
+    g.set_degree_thresholds([8, 64, 2048], bfs=True)
 
 
 def test_bfs_switch_settings_keep_levels_exact(g14):
@@ -85,7 +90,10 @@ def test_invalid_thresholds_rejected(g14):
     for t in ([8, 8, 2048], [64, 8, 2048], [1, 2, 2]):
         with pytest.raises(S.SSTGError):
             g.set_degree_thresholds(t)
+        with pytest.raises(S.SSTGError):
+            g.set_degree_thresholds(t, bfs=True)
     assert g.degree_thresholds() == [8, 64, 2048]
+    assert g.degree_thresholds(bfs=True) == [8, 64, 2048]
     with pytest.raises(S.SSTGError):
         g.autotune(pr=False, bfs_source=None)
     with pytest.raises(S.IdOutOfRange):
@@ -97,16 +105,23 @@ def test_autotune_picks_a_valid_faster_setting(g14):
     hub = int(np.argmax(np.diff(off)))
     rep = g.autotune(pr=True, bfs_source=hub)
     say("autotune", rep)
-    t = rep["thresholds"]
-    assert t[0] < t[1] < t[2]
+    t, tb = rep["thresholds"], rep["bfs_thresholds"]
+    assert t[0] < t[1] < t[2] and tb[0] < tb[1] < tb[2]
     assert g.degree_thresholds() == t
+    assert g.degree_thresholds(bfs=True) == tb
     assert rep["start_thresholds"] == [8, 64, 2048]
+    assert rep["start_bfs_thresholds"] == [8, 64, 2048]
     assert (rep["start_alpha"], rep["start_beta"]) == (15.0, 18.0)
     assert g.bfs_switch() == (rep["alpha"], rep["beta"])
-    assert rep["evaluations"] >= 10
-    assert rep["best_ms"] <= rep["start_ms"]
-    _check_all(g, off, cols, f"tuned {t}")
+    assert rep["evaluations"] >= 20
+    assert 0 < rep["pr_best_ms"] <= rep["pr_start_ms"]
+    assert 0 < rep["bfs_best_ms"] <= rep["bfs_start_ms"]
+    _check_all(g, off, cols, f"tuned {t} / bfs {tb}")
+    # a PageRank-only tuning leaves the BFS settings alone
+    rep2 = g.autotune(pr=True, bfs_source=None)
+    assert rep2["bfs_thresholds"] == tb and rep2["bfs_start_ms"] == 0
     g.set_degree_thresholds([8, 64, 2048])
+    g.set_degree_thresholds([8, 64, 2048], bfs=True)
     g.set_bfs_switch(15, 18)
 
 
