@@ -31,9 +31,11 @@ struct EdgeMapCtx {
   uint64_t fcap = 0;
   int slot = 0;
   unsigned long long *ctr = nullptr;  // [0] count, [1] degree sum
-  // bottom-up active lists per degree bin (double-buffered); act_src < 0
-  // until the first bottom-up step
-  uint32_t *act[4][2] = {};
+  // bottom-up active lists: the bins' survivors back to back in one buffer
+  // (bin b at act_off[b], act_n[b] entries), double-buffered; act_src < 0
+  // until the first bottom-up step, whose lists are the degree bins
+  uint32_t *act[2] = {};
+  uint64_t act_off[4] = {0, 0, 0, 0};
   uint64_t act_n[4] = {0, 0, 0, 0};
   int act_src = -1;
 };
@@ -484,17 +486,24 @@ __global__ void k_bin_flags(const uint32_t *deg, uint32_t n, uint32_t lo,
 }  // namespace
 }  // namespace em
 
+// The four lists lie back to back in one buffer (list[b + 1] = list[b] +
+// count[b]).
 inline int degree_bins(sstg_graph *g, DegreeBins *out, int set = kBinsPull) {
   const uint32_t *t = g->bin_hi[set];
   const uint32_t lo[4] = {0, t[0] + 1, t[1] + 1, t[2] + 1};
   const uint32_t hi[4] = {t[0], t[1], t[2], 0xFFFFFFFFu};
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
-  const int slot0 = set == kBinsPull ? kWsAlgo + 26 : kWsBfsBins;
-  for (int b = 0; b < 4; b++)
-    SSTG_TRY(g->ws.get(slot0 + b, (uint64_t)n + 1, &out->list[b]));
+  const int slot = set == kBinsPull ? kWsAlgo + 26 : kWsBfsBins;
+  uint32_t *all = nullptr;
+  SSTG_TRY(g->ws.get(slot, (uint64_t)n + 1, &all));
   if (g->bins_version[set] == g->version) {
-    for (int b = 0; b < 4; b++) out->count[b] = g->bins_count[set][b];
+    uint64_t off = 0;
+    for (int b = 0; b < 4; b++) {
+      out->count[b] = g->bins_count[set][b];
+      out->list[b] = all + off;
+      off += out->count[b];
+    }
     return SSTG_OK;
   }
   uint8_t *flags = nullptr;
@@ -510,6 +519,7 @@ inline int degree_bins(sstg_graph *g, DegreeBins *out, int set = kBinsPull) {
                             hipMemcpyDeviceToHost, s));
     SSTG_HIP(hipStreamSynchronize(s));
     out->count[b] = g->host_rb[16];
+    out->list[b] = all + (b ? (out->list[b - 1] - all) + out->count[b - 1] : 0);
     SSTG_TRY(narrow_u64_to_u32(tmp, out->count[b], out->list[b], s));
   }
   for (int b = 0; b < 4; b++) g->bins_count[set][b] = out->count[b];
@@ -517,56 +527,123 @@ inline int degree_bins(sstg_graph *g, DegreeBins *out, int set = kBinsPull) {
   return SSTG_OK;
 }
 
-// Dense pull over all vertices against ctx.bm_cur, split by degree bin
-// (1, 8, 64 lanes or a block per vertex); the output frontier is
-// ctx.bm_next, after which the two bitmaps swap roles.
+namespace em {
+namespace {
+// Kept flags of each bin's segment of a bottom-up step, summed: counts[b] =
+// sum of keep[seg[b] .. seg[b + 1]).  Grid-stride per-block partials, then
+// one block adds them (no contended atomics).
+constexpr uint32_t kSegBlocks = 1024;
+
+__global__ void __launch_bounds__(kT)
+    k_segment_partials(const uint8_t *keep, const uint64_t *seg,
+                       unsigned long long *part) {
+  const uint64_t s1 = seg[1], s2 = seg[2], s3 = seg[3], total = seg[4];
+  uint64_t c[4] = {0, 0, 0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * kT) {
+    const uint64_t k = keep[i];
+    const int b = i < s1 ? 0 : i < s2 ? 1 : i < s3 ? 2 : 3;
+    c[b] += k;
+  }
+  __shared__ unsigned long long wc[4][kT / kWave];
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const uint64_t t = wave_sum(c[b]);
+    if (lane_id() == 0) wc[b][threadIdx.x / kWave] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kT / kWave; w++) t += wc[threadIdx.x][w];
+    part[4 * (uint64_t)blockIdx.x + threadIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kT)
+    k_segment_sum(const unsigned long long *part, uint32_t nb,
+                  unsigned long long *counts) {
+  uint64_t c[4] = {0, 0, 0, 0};
+  for (uint32_t i = threadIdx.x; i < nb; i += kT)
+#pragma unroll
+    for (int b = 0; b < 4; b++) c[b] += part[4 * (uint64_t)i + b];
+  __shared__ unsigned long long wc[4][kT / kWave];
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const uint64_t t = wave_sum(c[b]);
+    if (lane_id() == 0) wc[b][threadIdx.x / kWave] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kT / kWave; w++) t += wc[threadIdx.x][w];
+    counts[threadIdx.x] = t;
+  }
+}
+}  // namespace
+}  // namespace em
+
+// Dense pull over the active vertices against ctx.bm_cur, split by degree
+// bin (1, 8, 64 lanes or a block per vertex); the output frontier is
+// ctx.bm_next, after which the two bitmaps swap roles.  The bins' active
+// lists lie back to back, so one compaction of the keep flags carries the
+// survivors of every bin to the next step.
 template <class F>
 int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
   hipStream_t s = g->stream;
   DegreeBins bins;
   SSTG_TRY(degree_bins(g, &bins, kBinsBfs));
-  // the first bottom-up step reads the degree bins; later ones the vertices
-  // the previous step left unvisited
+  const uint64_t all = bins.count[0] + bins.count[1] + bins.count[2] +
+                       bins.count[3];
   const bool first = ctx.act_src < 0;
   const int dst = first ? 0 : 1 - ctx.act_src;
+  SSTG_TRY(g->ws.get(kWsBottomUp + 0, all + 1, &ctx.act[0]));
+  SSTG_TRY(g->ws.get(kWsBottomUp + 1, all + 1, &ctx.act[1]));
+  // the first bottom-up step reads the degree bins; later ones the vertices
+  // the previous step left unvisited
+  const uint32_t *cur = first ? bins.list[0] : ctx.act[ctx.act_src];
+  uint64_t cnt[4], off[5];
+  off[0] = 0;
   for (int b = 0; b < 4; b++) {
-    const uint64_t cap = bins.count[b] ? bins.count[b] : 1;
-    SSTG_TRY(g->ws.get(kWsBottomUp + 2 * b + 0, cap, &ctx.act[b][0]));
-    SSTG_TRY(g->ws.get(kWsBottomUp + 2 * b + 1, cap, &ctx.act[b][1]));
+    cnt[b] = first ? bins.count[b] : ctx.act_n[b];
+    off[b + 1] = off[b] + cnt[b];
   }
-  // ctr[0..1]: found count and degree sum; ctr[3 + b]: active count of bin b
+  const uint64_t total = off[4];
+  // ctr[0..1]: found count and degree sum; ctr[2]: survivors;
+  // ctr[3 + b]: survivors of bin b
   SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 7 * sizeof(uint64_t), s));
   uint8_t *keep = nullptr;
-  SSTG_TRY(g->ws.get(kWsBottomUp + 8, bins.count[0] + bins.count[1] +
-                                          bins.count[2] + bins.count[3] + 4,
-                     &keep));
+  uint64_t *seg = nullptr;
+  SSTG_TRY(g->ws.get(kWsBottomUp + 8, all + 1, &keep));
+  SSTG_TRY(g->ws.get(kWsBottomUp + 2, 5, &seg));
+  for (int b = 0; b < 5; b++) g->host_rb[40 + b] = off[b];  // pinned
+  SSTG_HIP(hipMemcpyAsync(seg, g->host_rb + 40, sizeof(off),
+                          hipMemcpyHostToDevice, s));
   DevGraph v = g->view();
-  const uint32_t *cur = ctx.bm_cur;
-  const uint32_t *src[4];
-  uint64_t cnt[4];
-  for (int b = 0; b < 4; b++) {
-    src[b] = first ? bins.list[b] : ctx.act[b][ctx.act_src];
-    cnt[b] = first ? bins.count[b] : ctx.act_n[b];
-  }
-  uint8_t *kb[4];
-  kb[0] = keep;
-  for (int b = 1; b < 4; b++) kb[b] = kb[b - 1] + bins.count[b - 1] + 1;
+  const uint32_t *bm = ctx.bm_cur;
   if (cnt[0])
     SSTG_LAUNCH((em::k_dense_pull<F, 1, 8>), ceil_div(cnt[0], em::kT), em::kT,
-                0, s, v, src[0], cnt[0], f, cur, kb[0]);
+                0, s, v, cur + off[0], cnt[0], f, bm, keep + off[0]);
   if (cnt[1])
     SSTG_LAUNCH((em::k_dense_pull<F, 8, 8>), ceil_div(cnt[1] * 8, em::kT),
-                em::kT, 0, s, v, src[1], cnt[1], f, cur, kb[1]);
+                em::kT, 0, s, v, cur + off[1], cnt[1], f, bm, keep + off[1]);
   if (cnt[2])
     SSTG_LAUNCH((em::k_dense_pull<F, 64, 4>), ceil_div(cnt[2] * 64, em::kT),
-                em::kT, 0, s, v, src[2], cnt[2], f, cur, kb[2]);
+                em::kT, 0, s, v, cur + off[2], cnt[2], f, bm, keep + off[2]);
   if (cnt[3])
     SSTG_LAUNCH((em::k_dense_pull_block<F, 4>), cnt[3], em::kT, 0, s, v,
-                src[3], cnt[3], f, cur, kb[3]);
-  for (int b = 0; b < 4; b++)
-    SSTG_TRY(select_flagged_u32(kb[b], src[b], cnt[b], ctx.act[b][dst],
-                                (uint64_t *)(ctx.ctr + 3 + b), g->ws, kWsPrim,
-                                s));
+                cur + off[3], cnt[3], f, bm, keep + off[3]);
+  if (total) {
+    SSTG_TRY(select_flagged_u32(keep, cur, total, ctx.act[dst],
+                                (uint64_t *)(ctx.ctr + 2), g->ws, kWsPrim, s));
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(
+        ceil_div(total, (uint64_t)em::kT * 16), em::kSegBlocks);
+    unsigned long long *spart = nullptr;
+    SSTG_TRY(g->ws.get(kWsBottomUp + 3, 4 * (uint64_t)em::kSegBlocks, &spart));
+    SSTG_LAUNCH(em::k_segment_partials, nb, em::kT, 0, s, (const uint8_t *)keep,
+                (const uint64_t *)seg, spart);
+    SSTG_LAUNCH(em::k_segment_sum, 1, em::kT, 0, s,
+                (const unsigned long long *)spart, nb, ctx.ctr + 3);
+  }
   const uint64_t bm_waves = ceil_div(ctx.words * 32, (uint64_t)kWave);
   const uint64_t bm_blocks =
       ceil_div(bm_waves, (uint64_t)(em::kT / kWave) * em::kBitmapRounds);

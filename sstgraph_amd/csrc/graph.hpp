@@ -150,9 +150,11 @@ enum : int {
   kWsAlgo = 16,      // 16..47 algorithms
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range
-  kWsBottomUp = 112,  // 112..119 active lists of the bottom-up steps,
-                      // 120 their keep flags, 121 per-block partial sums
-  kWsBfsBins = 122,   // 122..125 degree bin lists of the BFS set
+  kWsBottomUp = 112,  // 112, 113 active lists of the bottom-up steps,
+                      // 114 their bin offsets, 115 per-bin partial
+                      // counts, 120 keep flags, 121
+                      // per-block partial sums
+  kWsBfsBins = 122,   // degree bin lists of the BFS set, back to back
   kWsSlots = 126,
 };
 
