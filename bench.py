@@ -17,6 +17,8 @@ reported as GTEPS under "extra".
                     [--no-cpu] [--gpus 1]
     python bench.py --workload batched   # 512 graphs, one hipGraph replay
     python bench.py --workload serve     # served request stream, p50/p99
+    python bench.py --workload algos     # BFS, PR, CC, BC, TC at scale 22
+    python bench.py --workload pagerank  # (or bfs) one algorithm, for profiles
 """
 import argparse
 import ctypes
@@ -463,12 +465,110 @@ def run_algo(args):
                    "directed_edges": int(m)}}))
 
 
+def run_c2(args):
+    """The plain-method GPU configuration: rMat scale --scale (22 by default
+    here), seed 1, edge factor 16, symmetrised; BFS (from vertex 0 and from
+    the largest hub), PageRank x --pr-iters in FP64, CC, BC from vertex 0 and
+    from the hub, and TC, outputs kept in HBM.  The line's value is
+    PageRank's traversed edges/s; every algorithm's time and rate is in
+    extra, beside the oracle's on the same graph (bounded CPU sample)."""
+    import sstgraph_amd as S
+    EMV = S.EdgeMapVertexMap
+    L = S.lib()
+    scale = args.scale
+    n = 1 << scale
+    src, dst = device_symmetric_rmat(S, n, 1, args.edge_factor * n)
+    g = S.SparseMatrixV(n)
+    g.insert_batch(src, dst)
+    src.free()
+    dst.free()
+    m = g.M()
+    off, cols = g.export_csr()
+    deg = np.diff(off).astype(np.int64)
+    hub = int(np.argmax(deg))
+    if not args.no_tune:
+        tune = g.autotune(pr=True, bfs_source=hub)
+        log(f"autotuned thresholds {tune['thresholds']}, BFS {tune['bfs_thresholds']}")
+    else:
+        tune = None
+    d_i32 = S.DeviceArray(n, np.int32)
+    d_u32 = S.DeviceArray(n, np.uint32)
+    d_f64 = S.DeviceArray(n, np.float64)
+    reps = max(3, min(args.steps, 10))
+
+    def best(fn):
+        for _ in range(args.warmup):
+            fn()
+        S.synchronize()
+        return float(np.median([timed(lambda: (fn(), S.synchronize()))
+                                for _ in range(reps)]))
+
+    out = {}
+    for name, sv in (("bfs_src0", 0), ("bfs_hub", hub)):
+        t = best(lambda: EMV.BFS(g, sv, out=(d_i32, d_u32)))
+        lev = d_u32.to_host()
+        m_r = int(deg[lev != S.UNREACHED].sum())
+        out[name] = dict(src=sv, ms=1e3 * t, reached=int((lev != S.UNREACHED).sum()),
+                         gteps=m_r / t / 1e9, gbs_model=(4 * m_r + 12 * n) / t / 1e9)
+    t = best(lambda: EMV.PR_S(g, args.pr_iters, out=d_f64))
+    out["pagerank"] = dict(iters=args.pr_iters, ms=1e3 * t,
+                           gteps=m * args.pr_iters / t / 1e9,
+                           gbs_model=(4 * m + 24 * n) * args.pr_iters / t / 1e9)
+    pr_value = m * args.pr_iters / t
+    t = best(lambda: S._check(L.sstg_cc(g.handle, d_u32.ptr, S.SSTG_DEVICE_PTRS), "cc"))
+    labels = d_u32.to_host()
+    out["cc"] = dict(ms=1e3 * t, components=int(np.unique(labels).size),
+                     gteps=m / t / 1e9)
+    for name, sv in (("bc_src0", 0), ("bc_hub", hub)):
+        t = best(lambda: S._check(L.sstg_bc(g.handle, sv, d_f64.ptr,
+                                            S.SSTG_DEVICE_PTRS), "bc"))
+        out[name] = dict(src=sv, ms=1e3 * t)
+    t = best(lambda: EMV.TC(g))
+    out["tc"] = dict(ms=1e3 * t, triangles=int(EMV.TC(g)))
+    for k, v in out.items():
+        log(f"{k}: " + ", ".join(f"{a}={b:.4g}" if isinstance(b, float) else f"{a}={b}"
+                                 for a, b in v.items()))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O  # CPU baseline leg only
+        if not os.path.exists(O.LIB_PATH):
+            O.build()
+        t_pr = timed(lambda: O.pagerank(off, cols, 2)) / 2
+        t_bfs = timed(lambda: O.bfs(off, cols, hub))
+        t_cc = timed(lambda: O.cc(off, cols))
+        t_bc = timed(lambda: O.bc(off, cols, hub))
+        cpu = dict(value=m / t_pr, unit="edges/s", cores=O.threads(), kind="port",
+                   sample=f"oracle PageRank (2 iterations, OpenMP) on the same scale-"
+                          f"{scale} graph; BFS and BC from the hub (one thread) and CC "
+                          f"(OpenMP) beside it",
+                   pagerank_ms_per_iter=1e3 * t_pr, bfs_hub_ms=1e3 * t_bfs,
+                   cc_ms=1e3 * t_cc, bc_hub_ms=1e3 * t_bc)
+        log(f"cpu: PR {1e3 * t_pr:.1f} ms/iter, BFS {1e3 * t_bfs:.0f} ms, "
+            f"CC {1e3 * t_cc:.0f} ms, BC {1e3 * t_bc:.0f} ms")
+    print(json.dumps({
+        "metric": f"PageRank FP64 traversed edges/s on rMat scale {scale} (with BFS, CC, "
+                  "BC, TC in extra)",
+        "value": pr_value, "unit": "edges/s", "n_gpus": 1, "steps": reps,
+        "warmup": args.warmup, "ms_per_step": out["pagerank"]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (PR, BC), u32",
+        "data": "synthetic (rMat generator of the reference, seeded; "
+                + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
+        "config": {"workload": f"plain graph algorithms on rMat scale {scale} (seed 1, "
+                               f"edge factor {args.edge_factor}, symmetrised): BFS, "
+                               f"PageRank x{args.pr_iters}, CC, BC, TC",
+                   "directed_edges": int(m)},
+        "cpu_baseline": cpu,
+        "extra": dict(out, autotune=tune)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--scale", type=int, default=None,
+                    help="rMat scale (24 by default; 22 for --workload algos)")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--batch", type=float, default=1e7)
     ap.add_argument("--pr-iters", type=int, default=20)
@@ -479,7 +579,8 @@ def main():
     ap.add_argument("--rmat-normalized", action="store_true",
                     help="rMat draws in [0,1) (probabilities 0.5/0.1/0.1/0.3) "
                          "instead of the reference's [0,2)")
-    ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs"],
+    ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs",
+                                           "algos"],
                     default="updates")
     ap.add_argument("--graphs", type=int, default=512)
     ap.add_argument("--serve-scale", type=int, default=25)
@@ -487,6 +588,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="Poisson rate as a fraction of the serialised capacity")
     args = ap.parse_args()
+    if args.scale is None:
+        args.scale = 22 if args.workload == "algos" else 24
     global RMAT_FLAGS
     if args.rmat_normalized:
         RMAT_FLAGS = 0x10  # SSTG_RMAT_NORMALIZED
@@ -501,6 +604,8 @@ def main():
         return run_serve(args)
     if args.workload in ("pagerank", "bfs"):
         return run_algo(args)
+    if args.workload == "algos":
+        return run_c2(args)
     import sstgraph_amd as S
     L = S.lib()
     n = 1 << args.scale
