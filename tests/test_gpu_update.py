@@ -169,3 +169,37 @@ def test_c1_build_then_update_batches_bit_exact():
         g.remove_batch(bs, bd)
         og.remove_batch(bs, bd)
         assert_same_csr(g, og, f"C1 remove {bsz}")
+
+
+def test_scale22_update_batches_bit_exact():
+    """The update workload's shape at scale 22: rMat(2^22, seed 2, 16n)
+    symmetrised, then insert+remove batches of 1e5, 1e6 and 1e7 edges (80%
+    fresh rMat edges, 20% present ones), contents compared with the oracle
+    after every batch.  The large batches take the relocation and big-row
+    paths, which the statistics confirm."""
+    n, s, d = O.rmat_graph_edges(22, 16, 2)
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    say("scale 22 build", len(s))
+    g.insert_batch(s, d)
+    og.insert_batch(s, d)
+    assert_same_csr(g, og, "scale 22 build")
+    off, cols = og.csr()
+    seen = {"big_rows": 0, "allocated": 0, "tile_rows": 0}
+    for k, bsz in enumerate([100_000, 1_000_000, 10_000_000]):
+        say("scale 22 batch", bsz)
+        fs, fd = O.rmat(n, 100 + k, int(bsz * 0.8))
+        pick = np.random.default_rng(100 + k).integers(0, len(cols), bsz - len(fs))
+        rows = np.searchsorted(off, pick, side="right") - 1
+        bs = np.concatenate([fs, rows.astype(np.uint32)])
+        bd = np.concatenate([fd, cols[pick]])
+        g.insert_batch(bs, bd)
+        og.insert_batch(bs, bd)
+        st = g.last_update_stats()
+        for key in seen:
+            seen[key] += st[key]
+        assert_same_csr(g, og, f"scale 22 insert {bsz}")
+        g.remove_batch(bs, bd)
+        og.remove_batch(bs, bd)
+        assert_same_csr(g, og, f"scale 22 remove {bsz}")
+    say("paths taken", seen)
+    assert seen["big_rows"] > 0 and seen["allocated"] > 0 and seen["tile_rows"] > 0
