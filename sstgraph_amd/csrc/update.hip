@@ -51,11 +51,12 @@ enum {
 };
 
 // Rows whose merge inputs (suffix + batch run) fit in kTileMax elements are
-// merged in place by one block through LDS; larger rows are staged.
-constexpr int kTile = 2048;
+// merged in place by one block through LDS; larger rows are staged.  A block
+// owns the rows whose inputs start in its kTile range, so its tile holds at
+// most kTile + kTileMax inputs.
+constexpr int kTile = 1024;
 constexpr int kTileMax = 2048;
 constexpr int kTileLds = kTile + kTileMax;
-constexpr int kTileItems = kTileLds / kT;
 
 __device__ __forceinline__ uint64_t grow_cap(uint64_t d, uint32_t n_cols) {
   uint64_t c = 4;
@@ -433,12 +434,14 @@ __device__ __forceinline__ void tile_row_info(const SegArgs &a,
   }
 }
 
-constexpr int kTileRows = 256;  // rows per block described in LDS
-// k_merge_tile runs 256 threads over one LDS tile: the tile and a 256-row
-// table take 23 KB, so six tiles are in flight per CU (tiles with more rows
-// read their row descriptors from HBM).  Measured at scale 24 with a 1e7
-// batch: 1.29 ms per launch, against 1.40 ms with 1024 threads and a
-// 1024-row table (two tiles per CU) and 1.36 ms with 512 and 512.
+constexpr int kTileRows = 128;  // rows per block described in LDS
+// k_merge_tile runs 256 threads over one LDS tile: the 12 KB tile and a
+// 128-row table take 15.6 KB, so eight tiles (the wave limit) are in flight
+// per CU; tiles with more rows read their row descriptors from HBM.  The
+// block shape was chosen by timing the 1e7-edge insert at scale 24 (merge
+// launch 1.13 ms; 1.29 ms with a 2048-input range and a 256-row table, 1.40
+// ms with 1024-thread blocks, 1.17 ms with 1024 for both tile bounds, whose
+// longer rows go to the slower big-row path).
 constexpr int kMergeThreads = 256;
 constexpr int kMergeItems = kTileLds / kMergeThreads;
 
