@@ -6,9 +6,12 @@ namespace sstg {
 
 namespace {
 
+// Tile sizes chosen by timing the batch update at scale 24 (batches of 1e5,
+// 1e6 and 1e7 edges): smaller tiles keep more blocks in flight for the
+// short scans and sorts of small batches and cost nothing at 1e7.
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+constexpr int kScanItems = 2;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 512
 
 // Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
 // Returns the exclusive prefix; *total gets the block sum.
@@ -145,8 +148,8 @@ __global__ void __launch_bounds__(kScanThreads)
 // --------------------------------------------------------- radix sort
 constexpr int kRsThreads = 256;
 constexpr int kRsWaves = kRsThreads / kWave;
-constexpr int kRsItems = 16;
-constexpr int kRsTile = kRsThreads * kRsItems;  // 4096 keys
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kRsThreads * kRsItems;  // 2048 keys
 constexpr int kRsBins = 256;
 
 // Digit histogram of each tile; hist is digit-major: hist[d * tiles + t].
