@@ -324,6 +324,113 @@ struct BfsRun {
   std::vector<uint64_t> level_start;  // into order, levels + 1 entries
 };
 
+// The tail of a BFS in one block.  Once the frontier and its edges are small
+// the per-level launches and host read-backs cost more than the work, so the
+// remaining top-down levels run in a single 1024-thread block: per level it
+// scans the frontier's degrees in LDS, walks the frontier's edges flat (one
+// edge per thread per step), claims unvisited neighbours as BfsPush does and
+// appends them to the queue through an LDS counter.  It stops when the
+// frontier empties, grows past the bounds (the host takes over again), or
+// after kTailLevels levels.  state: [0] start of the current frontier in the
+// queue, [1] its size, [2] its level, [3] levels run, [4] degree sum of every
+// frontier found, [5] size of the frontier before the current one, [6]
+// degree sum of the current frontier; ends[k] = end of the frontier found by
+// the k-th level run.
+constexpr uint32_t kTailThreads = 1024;
+constexpr uint32_t kTailFrontier = 16384;
+constexpr uint32_t kTailEdges = 1u << 17;
+constexpr uint32_t kTailLevels = 1024;
+constexpr uint32_t kTailPer = kTailFrontier / kTailThreads;
+
+__global__ void __launch_bounds__(kTailThreads)
+    k_bfs_tail(DevGraph g, uint32_t *level, int32_t *parent, uint32_t *queue,
+               uint64_t *state, uint64_t *ends) {
+  __shared__ uint32_t pre[kTailFrontier + 1];
+  __shared__ uint32_t wsum[kTailThreads / kWave];
+  __shared__ uint32_t s_next;
+  __shared__ unsigned long long s_edges;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  uint64_t head = state[0];
+  uint32_t cnt = (uint32_t)state[1];
+  uint32_t L = (uint32_t)state[2];
+  uint64_t prev = state[5], found_edges = 0, cur_edges = state[6];
+  uint32_t k = 0;
+  while (cnt > 0 && k < kTailLevels) {  // cnt and k are block-uniform
+    // exclusive prefix of the frontier's degrees (kTailPer per thread)
+    uint32_t dl[kTailPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kTailPer; j++) {
+      const uint32_t idx = tid * kTailPer + j;
+      dl[j] = idx < cnt ? g.deg[queue[head + idx]] : 0u;
+      sum += dl[j];
+    }
+    const uint32_t incl = wave_inclusive_sum(sum);
+    if (lane == kWave - 1) wsum[wave] = incl;
+    if (tid == 0) {
+      s_next = 0;
+      s_edges = 0;
+    }
+    __syncthreads();
+    uint32_t run = incl - sum, total = 0;
+    for (uint32_t w = 0; w < kTailThreads / kWave; w++) {
+      const uint32_t x = wsum[w];
+      if (w < wave) run += x;
+      total += x;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kTailPer; j++) {
+      const uint32_t idx = tid * kTailPer + j;
+      if (idx < cnt) pre[idx] = run;
+      run += dl[j];
+    }
+    if (tid == 0) pre[cnt] = total;
+    __syncthreads();
+    // the frontier's edges, flat
+    const uint64_t tail = head + cnt;
+    for (uint32_t e = tid; e < total; e += kTailThreads) {
+      uint32_t lo = 0, hi = cnt;  // largest j with pre[j] <= e
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= e)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const uint32_t u = queue[head + lo];
+      const uint32_t w = g.pool[g.off[u] + (e - pre[lo])];
+      if (level[w] == kInf && atomicCAS(&level[w], kInf, L + 1) == kInf) {
+        if (parent) parent[w] = (int32_t)u;
+        const uint32_t pos = atomicAdd(&s_next, 1u);
+        queue[tail + pos] = w;
+        atomicAdd(&s_edges, (unsigned long long)g.deg[w]);
+      }
+    }
+    __syncthreads();
+    const uint32_t nxt = s_next;
+    const uint64_t ned = s_edges;
+    if (tid == 0) ends[k] = tail + nxt;
+    prev = cnt;
+    head = tail;
+    cnt = nxt;
+    cur_edges = ned;
+    found_edges += ned;
+    L++;
+    k++;
+    __syncthreads();  // s_next and s_edges are reset by the next level
+    if (cnt > kTailFrontier || ned > kTailEdges) break;
+  }
+  if (tid == 0) {
+    state[0] = head;
+    state[1] = cnt;
+    state[2] = L;
+    state[3] = k;
+    state[4] = found_edges;
+    state[5] = prev;
+    state[6] = cur_edges;
+  }
+}
+
 static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                     int32_t *parent, uint32_t *order, bool top_down_only,
                     BfsRun *run) {
@@ -375,6 +482,51 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                    (bottom_up ? (double)nf >= (double)n / beta
                               : nf > prev_nf &&
                                     (double)mf > (double)unexplored / alpha);
+    if (!want_bu && nf <= kTailFrontier && mf <= kTailEdges) {
+      // the remaining top-down levels in one block (k_bfs_tail)
+      uint64_t *st = nullptr, *ends = nullptr;
+      SSTG_TRY(ws.get(kWsAlgo + 8, 8, &st));
+      SSTG_TRY(ws.get(kWsAlgo + 9, kTailLevels, &ends));
+      uint64_t *h = g->host_rb + 48;  // pinned staging
+      h[0] = qs;
+      h[1] = nf;
+      h[2] = L;
+      h[3] = 0;
+      h[4] = 0;
+      h[5] = prev_nf;
+      h[6] = mf;
+      SSTG_HIP(hipMemcpyAsync(st, h, 7 * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              s));
+      SSTG_LAUNCH(k_bfs_tail, 1, kTailThreads, 0, s, g->view(), level, parent,
+                  queue, st, ends);
+      SSTG_HIP(hipMemcpyAsync(h, st, 7 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              s));
+      SSTG_HIP(hipStreamSynchronize(s));
+      const uint64_t levels_run = h[3];
+      if (run && levels_run) {
+        std::vector<uint64_t> e(levels_run);
+        SSTG_HIP(hipMemcpy(e.data(), ends, levels_run * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost));
+        for (uint64_t x : e) run->level_start.push_back(x);
+      }
+      if (verbose) {
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr,
+                     "bfs levels %u..%u in one block, frontier %llu  %.3f ms\n", L,
+                     (uint32_t)(L + levels_run - 1), (unsigned long long)h[1],
+                     std::chrono::duration<double, std::milli>(t1 - t_level)
+                         .count());
+        t_level = t1;
+      }
+      qs = h[0];
+      nf = h[1];
+      L = (uint32_t)h[2];
+      explored += h[4];
+      prev_nf = h[5];
+      mf = h[6];
+      bottom_up = false;
+      continue;
+    }
     EdgeMapResult res;
     if (want_bu) {
       if (!bottom_up) SSTG_TRY(frontier_to_bitmap(ctx, queue + qs, nf, s));
