@@ -734,7 +734,7 @@ int algo_bc(sstg_graph *g, uint32_t src, double *out) {
       uint64_t b = run.level_start[L], e = run.level_start[L + 1];
       if (e <= b) continue;
       SigmaPull f{level, sigma, L};
-      SSTG_TRY(vertex_pull_list(g, order + b, e - b, f));
+      SSTG_TRY(vertex_pull_level(g, order + b, e - b, level, L, f));
     } else {
       uint64_t b = run.level_start[L - 1], e = run.level_start[L];
       if (e <= b) continue;
@@ -748,7 +748,7 @@ int algo_bc(sstg_graph *g, uint32_t src, double *out) {
     uint64_t b = run.level_start[L], e = run.level_start[L + 1];
     if (e <= b) continue;
     DeltaPull f{level, sigma, out, (uint32_t)L};
-    SSTG_TRY(vertex_pull_list(g, order + b, e - b, f));
+    SSTG_TRY(vertex_pull_level(g, order + b, e - b, level, (uint32_t)L, f));
   }
   SSTG_HIP(hipStreamSynchronize(s));
   return SSTG_OK;

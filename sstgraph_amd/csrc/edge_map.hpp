@@ -362,9 +362,19 @@ __global__ void __launch_bounds__(kT)
 // Reduction over all neighbours of each vertex with G lanes per vertex
 // (G <= 64 via wave shuffles; G == 256 uses the whole block).  The lane
 // stride and the shuffle tree are fixed, so results do not depend on timing.
-template <class F, int G>
-__global__ void __launch_bounds__(kT)
-    k_vertex_pull(DevGraph g, const uint32_t *list, uint64_t count, F f) {
+// Only vertices for which pred(v, degree) holds are reduced; pred must be the
+// same for a whole group (it depends on v and its degree only).
+struct AllVertices {
+  __device__ __forceinline__ bool operator()(uint32_t, uint32_t) const {
+    return true;
+  }
+};
+
+template <class F, int G, class P>
+__device__ __forceinline__ void vertex_pull_body(const DevGraph &g,
+                                                 const uint32_t *list,
+                                                 uint64_t count, const F &f,
+                                                 const P &pred) {
   using Acc = typename F::Acc;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t gid = (uint64_t)blk * kT + threadIdx.x;
@@ -373,6 +383,7 @@ __global__ void __launch_bounds__(kT)
   if (grp >= count) return;  // whole groups leave together
   const uint32_t v = list ? list[grp] : (uint32_t)grp;
   const uint32_t d = g.deg[v];
+  if (!pred(v, d)) return;  // whole groups (whole blocks for G == 256)
   const uint32_t *row = g.pool + g.off[v];
   Acc acc = f.zero();
   uint32_t k = lig;
@@ -403,6 +414,34 @@ __global__ void __launch_bounds__(kT)
   }
   if (lig == 0) f.store(v, acc);
 }
+
+template <class F, int G>
+__global__ void __launch_bounds__(kT)
+    k_vertex_pull(DevGraph g, const uint32_t *list, uint64_t count, F f) {
+  vertex_pull_body<F, G>(g, list, count, f, AllVertices{});
+}
+
+template <class F, int G, class P>
+__global__ void __launch_bounds__(kT)
+    k_vertex_pull_if(DevGraph g, const uint32_t *list, uint64_t count, F f,
+                     P pred) {
+  vertex_pull_body<F, G>(g, list, count, f, pred);
+}
+
+// predicates of vertex_pull_level
+struct UpToDegree {
+  uint32_t max_deg;
+  __device__ __forceinline__ bool operator()(uint32_t, uint32_t d) const {
+    return d <= max_deg;
+  }
+};
+struct OnLevel {
+  const uint32_t *level;
+  uint32_t want;
+  __device__ __forceinline__ bool operator()(uint32_t v, uint32_t) const {
+    return level[v] == want;
+  }
+};
 
 }  // namespace
 }  // namespace em
@@ -694,6 +733,37 @@ int vertex_pull_list(sstg_graph *g, const uint32_t *list, uint64_t count, F f) {
   if (!count) return SSTG_OK;
   SSTG_LAUNCH((em::k_vertex_pull<F, 64>), ceil_div(count * 64, em::kT), em::kT,
               0, g->stream, g->view(), list, count, f);
+  return SSTG_OK;
+}
+
+// Vertex pull over the vertices of one BFS level (list: that level's
+// vertices, level[v] == L for each).  Large levels give their low-degree
+// vertices (up to the reductions' 8-lane bound) 8 lanes each and reach the
+// rest through the wave and block degree bins filtered by level, instead of
+// a wave per vertex; small levels take a wave per vertex.
+constexpr uint64_t kLevelSplitMin = 1u << 16;
+
+template <class F>
+int vertex_pull_level(sstg_graph *g, const uint32_t *list, uint64_t count,
+                      const uint32_t *level, uint32_t L, F f) {
+  if (!count) return SSTG_OK;
+  if (count < kLevelSplitMin) return vertex_pull_list(g, list, count, f);
+  DegreeBins bins;
+  SSTG_TRY(degree_bins(g, &bins, kBinsPull));
+  hipStream_t s = g->stream;
+  const DevGraph v = g->view();
+  const em::UpToDegree low{g->bin_hi[kBinsPull][1]};
+  const em::OnLevel on{level, L};
+  SSTG_LAUNCH((em::k_vertex_pull_if<F, 8, em::UpToDegree>),
+              ceil_div(count * 8, em::kT), em::kT, 0, s, v, list, count, f, low);
+  if (bins.count[2])
+    SSTG_LAUNCH((em::k_vertex_pull_if<F, 64, em::OnLevel>),
+                ceil_div(bins.count[2] * 64, em::kT), em::kT, 0, s, v,
+                (const uint32_t *)bins.list[2], bins.count[2], f, on);
+  if (bins.count[3])
+    SSTG_LAUNCH((em::k_vertex_pull_if<F, 256, em::OnLevel>), bins.count[3],
+                em::kT, 0, s, v, (const uint32_t *)bins.list[3], bins.count[3],
+                f, on);
   return SSTG_OK;
 }
 
