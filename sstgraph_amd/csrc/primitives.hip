@@ -6,12 +6,31 @@ namespace sstg {
 
 namespace {
 
-// Tile sizes chosen by timing the batch update at scale 24 (batches of 1e5,
-// 1e6 and 1e7 edges): smaller tiles keep more blocks in flight for the
-// short scans and sorts of small batches and cost nothing at 1e7.
+// Scans and selects run in at most three launches whatever their length:
+// each block of a fixed grid (at most kScanBlocks) reduces one contiguous
+// range of whole tiles, one block scans those per-block partials, and each
+// block then scans its range tile by tile from its partial's offset.  Up to
+// kScanOneBlock tiles one block does the whole job in one launch.  A tile is
+// kScanItems values per thread (512), small so that short scans still spread
+// over many blocks; the cap of 2048 blocks (8 per CU) keeps the partials to
+// one block's worth and the machine full.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 2;
 constexpr int kScanTile = kScanThreads * kScanItems;  // 512
+constexpr uint64_t kScanBlocks = 2048;
+constexpr int kPartItems = (int)(kScanBlocks / kScanThreads);  // 8
+constexpr uint64_t kScanOneBlock = 8;
+
+struct Ranges {
+  uint64_t per;     // values per block, a multiple of kScanTile
+  uint32_t blocks;  // ceil(n / per) <= kScanBlocks
+};
+
+Ranges ranges_of(uint64_t n) {
+  const uint64_t tiles = ceil_div(n, kScanTile);
+  const uint64_t tpb = ceil_div(tiles, kScanBlocks);
+  return Ranges{tpb * kScanTile, (uint32_t)ceil_div(tiles, tpb)};
+}
 
 // Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
 // Returns the exclusive prefix; *total gets the block sum.
@@ -35,48 +54,91 @@ __device__ __forceinline__ T block_exclusive(T v, T *wtot, T *total) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kScanThreads)
-    k_tile_reduce(const T *in, uint64_t n, T *partials) {
-  __shared__ T wtot[kScanThreads / kWave];
-  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  T acc = 0;
-#pragma unroll
-  for (int j = 0; j < kScanItems; j++) {
-    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
-    if (i < n) acc += in[i];
-  }
-  acc = wave_sum(acc);
-  if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = acc;
+__device__ __forceinline__ T block_sum(T v, T *wtot) {
+  v = wave_sum(v);
+  if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    T t = 0;
-    for (int i = 0; i < kScanThreads / kWave; i++) t += wtot[i];
-    partials[blockIdx.x] = t;
+  T t = 0;
+#pragma unroll
+  for (int i = 0; i < kScanThreads / kWave; i++) t += wtot[i];
+  return t;
+}
+
+// Values the scans read: a plain array, or a byte flag counted as 0 / 1.
+template <class T> struct ArrayIn {
+  const T *p;
+  __device__ T operator()(uint64_t i) const { return p[i]; }
+};
+struct FlagIn {
+  const uint8_t *p;
+  __device__ uint64_t operator()(uint64_t i) const { return p[i] ? 1 : 0; }
+};
+
+template <class T, class In>
+__global__ void __launch_bounds__(kScanThreads)
+    k_range_reduce(In in, uint64_t n, uint64_t per, T *partials) {
+  __shared__ T wtot[kScanThreads / kWave];
+  const uint64_t lo = (uint64_t)blockIdx.x * per;
+  const uint64_t hi = lo + per < n ? lo + per : n;
+  T acc = 0;
+  for (uint64_t c = lo; c < hi; c += kScanTile) {
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+      uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
+      if (i < hi) acc += in(i);
+    }
+  }
+  T t = block_sum(acc, wtot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// Exclusive scan, in place, of nb <= kScanBlocks per-block partials (one
+// block, kPartItems consecutive values per thread).
+template <class T>
+__global__ void __launch_bounds__(kScanThreads)
+    k_scan_partials(T *p, uint32_t nb) {
+  __shared__ T wtot[kScanThreads / kWave];
+  const uint32_t base = threadIdx.x * kPartItems;
+  T v[kPartItems], sum = 0;
+#pragma unroll
+  for (int j = 0; j < kPartItems; j++) {
+    v[j] = base + j < nb ? p[base + j] : (T)0;
+    sum += v[j];
+  }
+  T tot;
+  T run = block_exclusive(sum, wtot, &tot);
+#pragma unroll
+  for (int j = 0; j < kPartItems; j++) {
+    if (base + j < nb) p[base + j] = run;
+    run += v[j];
   }
 }
 
-// Exclusive scan of each tile plus its offset; the block owning the last
-// tile also writes the grand total.
-template <class T>
+// Exclusive scan of each block's range from its offset (offsets null: one
+// block, offset 0); the last block writes the grand total.
+template <class T, class In>
 __global__ void __launch_bounds__(kScanThreads)
-    k_tile_scan(const T *in, T *out, uint64_t n, const T *offsets,
-                T *d_total) {
+    k_range_scan(In in, T *out, uint64_t n, uint64_t per, const T *offsets,
+                 T *d_total) {
   __shared__ T wtot[kScanThreads / kWave];
-  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  const uint64_t lo = (uint64_t)blockIdx.x * per;
+  const uint64_t hi = lo + per < n ? lo + per : n;
   T carry = offsets ? offsets[blockIdx.x] : (T)0;
-  T vals[kScanItems];
+  for (uint64_t c = lo; c < hi; c += kScanTile) {
+    T vals[kScanItems];
 #pragma unroll
-  for (int j = 0; j < kScanItems; j++) {
-    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
-    vals[j] = (i < n) ? in[i] : (T)0;
-  }
+    for (int j = 0; j < kScanItems; j++) {
+      uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
+      vals[j] = (i < hi) ? in(i) : (T)0;
+    }
 #pragma unroll
-  for (int j = 0; j < kScanItems; j++) {
-    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
-    T tot;
-    T ex = block_exclusive(vals[j], wtot, &tot);
-    if (i < n) out[i] = carry + ex;
-    carry += tot;
+    for (int j = 0; j < kScanItems; j++) {
+      uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
+      T tot;
+      T ex = block_exclusive(vals[j], wtot, &tot);
+      if (i < hi) out[i] = carry + ex;
+      carry += tot;
+    }
   }
   if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
     *d_total = carry;
@@ -89,60 +151,53 @@ int scan_impl(const T *in, T *out, uint64_t n, T *d_total, Workspace &ws,
     if (d_total) SSTG_HIP(hipMemsetAsync(d_total, 0, sizeof(T), s));
     return SSTG_OK;
   }
-  uint64_t tiles = ceil_div(n, kScanTile);
-  if (tiles == 1) {
-    SSTG_LAUNCH(k_tile_scan<T>, 1, kScanThreads, 0, s, in, out, n,
-                (const T *)nullptr, d_total);
+  const ArrayIn<T> src{in};
+  if (ceil_div(n, kScanTile) <= kScanOneBlock) {
+    SSTG_LAUNCH((k_range_scan<T, ArrayIn<T>>), 1, kScanThreads, 0, s, src, out,
+                n, n, (const T *)nullptr, d_total);
     return SSTG_OK;
   }
-  if (slot >= kWsSlots + 16) return SSTG_E_INVAL;  // recursion depth guard
+  const Ranges r = ranges_of(n);
   T *partials = nullptr;
-  SSTG_TRY(ws.get(slot, tiles, &partials));
-  SSTG_LAUNCH(k_tile_reduce<T>, tiles, kScanThreads, 0, s, in, n, partials);
-  SSTG_TRY(scan_impl<T>(partials, partials, tiles, (T *)nullptr, ws, slot + 1,
-                        s));
-  SSTG_LAUNCH(k_tile_scan<T>, tiles, kScanThreads, 0, s, in, out, n,
-              (const T *)partials, d_total);
+  SSTG_TRY(ws.get(slot, r.blocks, &partials));
+  SSTG_LAUNCH((k_range_reduce<T, ArrayIn<T>>), r.blocks, kScanThreads, 0, s,
+              src, n, r.per, partials);
+  SSTG_LAUNCH(k_scan_partials<T>, 1, kScanThreads, 0, s, partials, r.blocks);
+  SSTG_LAUNCH((k_range_scan<T, ArrayIn<T>>), r.blocks, kScanThreads, 0, s, src,
+              out, n, r.per, (const T *)partials, d_total);
   return SSTG_OK;
 }
 
 // ------------------------------------------------------------- select
-__global__ void __launch_bounds__(kScanThreads)
-    k_count_flags(const uint8_t *flags, uint64_t n, uint64_t *counts) {
-  __shared__ uint64_t wtot[kScanThreads / kWave];
-  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  uint64_t acc = 0;
-#pragma unroll
-  for (int j = 0; j < kScanItems; j++) {
-    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
-    if (i < n) acc += flags[i] ? 1 : 0;
-  }
-  acc = wave_sum(acc);
-  if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    for (int i = 0; i < kScanThreads / kWave; i++) t += wtot[i];
-    counts[blockIdx.x] = t;
-  }
-}
-
+// Writes the flagged values of each block's range in order from its offset
+// (offsets null: one block, offset 0); the last block writes the count.
 template <class V>
 __global__ void __launch_bounds__(kScanThreads)
-    k_select(const uint8_t *flags, const V *values, uint64_t n,
-             const uint64_t *offsets, V *out) {
+    k_select_range(const uint8_t *flags, const V *values, uint64_t n,
+                   uint64_t per, const uint64_t *offsets, V *out,
+                   uint64_t *d_count) {
   __shared__ uint64_t wtot[kScanThreads / kWave];
-  uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-  uint64_t carry = offsets[blockIdx.x];
+  const uint64_t lo = (uint64_t)blockIdx.x * per;
+  const uint64_t hi = lo + per < n ? lo + per : n;
+  uint64_t carry = offsets ? offsets[blockIdx.x] : 0;
+  for (uint64_t c = lo; c < hi; c += kScanTile) {
+    uint8_t f[kScanItems];
 #pragma unroll
-  for (int j = 0; j < kScanItems; j++) {
-    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
-    uint64_t f = (i < n && flags[i]) ? 1 : 0;
-    uint64_t tot;
-    uint64_t ex = block_exclusive(f, wtot, &tot);
-    if (f) out[carry + ex] = values ? values[i] : (V)i;
-    carry += tot;
+    for (int j = 0; j < kScanItems; j++) {
+      uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
+      f[j] = (i < hi && flags[i]) ? 1 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+      uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
+      uint64_t tot;
+      uint64_t ex = block_exclusive((uint64_t)f[j], wtot, &tot);
+      if (f[j]) out[carry + ex] = values ? values[i] : (V)i;
+      carry += tot;
+    }
   }
+  if (d_count && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    *d_count = carry;
 }
 
 // --------------------------------------------------------- radix sort
@@ -287,14 +342,20 @@ static int select_flagged(const uint8_t *flags, const V *values, uint64_t n,
     SSTG_HIP(hipMemsetAsync(d_count, 0, sizeof(uint64_t), s));
     return SSTG_OK;
   }
-  uint64_t tiles = ceil_div(n, kScanTile);
+  if (ceil_div(n, kScanTile) <= kScanOneBlock) {
+    SSTG_LAUNCH(k_select_range<V>, 1, kScanThreads, 0, s, flags, values, n, n,
+                (const uint64_t *)nullptr, out, d_count);
+    return SSTG_OK;
+  }
+  const Ranges r = ranges_of(n);
   uint64_t *counts = nullptr;
-  SSTG_TRY(ws.get(ws_base, tiles, &counts));
-  SSTG_LAUNCH(k_count_flags, tiles, kScanThreads, 0, s, flags, n, counts);
-  SSTG_TRY(scan_exclusive_u64(counts, counts, tiles, d_count, ws, ws_base + 1,
-                              s));
-  SSTG_LAUNCH(k_select<V>, tiles, kScanThreads, 0, s, flags, values, n,
-              (const uint64_t *)counts, out);
+  SSTG_TRY(ws.get(ws_base, r.blocks, &counts));
+  SSTG_LAUNCH((k_range_reduce<uint64_t, FlagIn>), r.blocks, kScanThreads, 0, s,
+              FlagIn{flags}, n, r.per, counts);
+  SSTG_LAUNCH(k_scan_partials<uint64_t>, 1, kScanThreads, 0, s, counts,
+              r.blocks);
+  SSTG_LAUNCH(k_select_range<V>, r.blocks, kScanThreads, 0, s, flags, values,
+              n, r.per, (const uint64_t *)counts, out, d_count);
   return SSTG_OK;
 }
 

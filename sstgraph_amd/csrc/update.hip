@@ -15,7 +15,8 @@
 //   5. heads     row boundaries in B -> one segment per touched row
 //   6. segprep   per row: new degree, in place or relocate, first moved
 //                position p0, staged suffix length, output work length
-//   7. scans     stage / alloc / work offsets (+ totals read back once)
+//   7. scans     alloc / stage / work / tile offsets, the big and tile row
+//                classes compacted (totals read back once)
 //   8. stage     copy each in-place row's suffix [p0, deg) to scratch
 //   9. merge     insert: merge-path merge of (suffix or whole old row) with
 //                the row's batch run into the row slot (or its new slot);
@@ -124,10 +125,12 @@ __global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
   uint64_t s = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (s > nmax) return;
   uint64_t t = ctr[kCtrT], b = ctr[kCtrB];
-  if (s >= t) {
-    seg_alloc[s] = 0;
-    is_big[s] = 0;
-    is_tile[s] = 0;
+  if (s >= t) {  // slot t is read as zero by the scans; later slots are not
+    if (s == t) {
+      seg_alloc[s] = 0;
+      is_big[s] = 0;
+      is_tile[s] = 0;
+    }
     return;
   }
   uint64_t bs = seg_idx[s], be = (s + 1 < t) ? seg_idx[s + 1] : b;
@@ -167,27 +170,182 @@ __global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
   is_tile[s] = tile;
 }
 
-// Lengths of the compacted classes: big rows (stage, work) and tile rows
-// (inputs); entries past a class count are zero.
-__global__ void k_class_lengths(const uint64_t *ctr, uint64_t nmax,
-                                const uint64_t *big_idx,
-                                const uint64_t *tile_idx,
-                                const uint32_t *seg_stage,
-                                const uint64_t *seg_work,
-                                const uint32_t *seg_tile, uint64_t *big_stage,
-                                uint64_t *big_work, uint64_t *tile_len) {
-  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (i > nmax) return;
-  uint64_t nb = ctr[kCtrBig], nt = ctr[kCtrTiles];
-  if (i < nb) {
-    uint64_t s = big_idx[i];
-    big_stage[i] = seg_stage[s];
-    big_work[i] = seg_work[s];
-  } else {
-    big_stage[i] = 0;
-    big_work[i] = 0;
+// Step 7 in three launches: the six running sums of the segment slots
+// (slots at or past t are zero) are scanned together -- pool elements
+// allocated, big rows, tile rows, and the staged elements and merge work of
+// big rows and the inputs of tile rows.  The same pass compacts the two
+// classes: big row k gets big_idx[k] = its segment and its stage and work
+// offsets; tile row k gets tile_idx[k] and its input offset; each offset
+// array ends with its total (entry nb or nt).  Each block of a fixed grid
+// (at most kSegBlocks) owns a contiguous range of slots: a reduce writes its
+// six partials, one block scans them, and each block then scans its range.
+constexpr int kSegSums = 6;
+constexpr int kSegItems = 4;
+constexpr int kSegTile = kT * kSegItems;  // slots per step of a block
+constexpr uint64_t kSegBlocks = 2048;
+constexpr uint64_t kSegOneBlock = 8;  // up to this many tiles: one block
+
+struct SegScan {
+  uint64_t *alloc;  // in: per slot; out: exclusive offsets
+  const uint8_t *is_big, *is_tile;
+  const uint32_t *stage, *tile;
+  const uint64_t *work;
+  uint64_t n, per;     // slots (nseg + 1), slots per block
+  uint64_t *partials;  // kSegSums x blocks, sum-major; null: one block
+  uint32_t blocks;
+  uint64_t *big_idx, *big_stage, *big_work, *tile_idx, *tile_len;
+  uint64_t *ctr;
+};
+
+__device__ __forceinline__ void seg_sums(const SegScan &a, uint64_t s,
+                                         uint64_t v[kSegSums]) {
+  const bool big = a.is_big[s], til = a.is_tile[s];
+  v[0] = a.alloc[s];
+  v[1] = big;
+  v[2] = til;
+  v[3] = big ? a.stage[s] : 0;
+  v[4] = big ? a.work[s] : 0;
+  v[5] = til ? a.tile[s] : 0;
+}
+
+// End of a block's range.  Slots past t (ctr[kCtrT]) are zero and only slot
+// t's alloc offset (the total) is read after the scan, so ranges stop at t.
+__device__ __forceinline__ uint64_t seg_end(const SegScan &a, uint64_t lo) {
+  uint64_t hi = lo + a.per < a.n ? lo + a.per : a.n;
+  const uint64_t t1 = a.ctr[kCtrT] + 1;
+  return hi < t1 ? hi : t1;
+}
+
+__global__ void __launch_bounds__(kT) k_seg_reduce(SegScan a) {
+  __shared__ uint64_t wtot[kT / kWave][kSegSums];
+  const uint64_t lo = (uint64_t)blockIdx.x * a.per;
+  const uint64_t hi = seg_end(a, lo);
+  uint64_t acc[kSegSums] = {0, 0, 0, 0, 0, 0};
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kT) {
+    uint64_t v[kSegSums];
+    seg_sums(a, i, v);
+#pragma unroll
+    for (int k = 0; k < kSegSums; k++) acc[k] += v[k];
   }
-  tile_len[i] = i < nt ? seg_tile[tile_idx[i]] : 0;
+#pragma unroll
+  for (int k = 0; k < kSegSums; k++) {
+    const uint64_t w = wave_sum(acc[k]);
+    if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave][k] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < kSegSums) {
+    uint64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kT / kWave; w++) t += wtot[w][threadIdx.x];
+    a.partials[(uint64_t)threadIdx.x * a.blocks + blockIdx.x] = t;
+  }
+}
+
+// One block: exclusive scan in place of each of the six partial arrays.
+__global__ void __launch_bounds__(kT) k_seg_scan_partials(SegScan a) {
+  constexpr int kPer = (int)(kSegBlocks / kT);
+  __shared__ uint64_t wtot[kT / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint32_t base = threadIdx.x * kPer;
+  for (int k = 0; k < kSegSums; k++) {
+    uint64_t *p = a.partials + (uint64_t)k * a.blocks;
+    uint64_t v[kPer], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      v[j] = base + j < a.blocks ? p[base + j] : 0;
+      sum += v[j];
+    }
+    const uint64_t incl = wave_inclusive_sum(sum);
+    if (lane == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    uint64_t run = incl - sum;
+    for (int i = 0; i < w; i++) run += wtot[i];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (base + j < a.blocks) p[base + j] = run;
+      run += v[j];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kT) k_seg_scan(SegScan a) {
+  __shared__ uint64_t wtot[kT / kWave][kSegSums];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint64_t lo = (uint64_t)blockIdx.x * a.per;
+  const uint64_t hi = seg_end(a, lo);
+  uint64_t carry[kSegSums];
+#pragma unroll
+  for (int k = 0; k < kSegSums; k++)
+    carry[k] = a.partials ? a.partials[(uint64_t)k * a.blocks + blockIdx.x]
+                          : 0;
+  for (uint64_t c = lo; c < hi; c += kSegTile) {
+    // kSegItems consecutive slots per thread: one block scan per step
+    const uint64_t i0 = c + (uint64_t)threadIdx.x * kSegItems;
+    uint64_t v[kSegItems][kSegSums], ex[kSegSums];
+#pragma unroll
+    for (int k = 0; k < kSegSums; k++) ex[k] = 0;
+#pragma unroll
+    for (int j = 0; j < kSegItems; j++) {
+      if (i0 + j < hi) {
+        seg_sums(a, i0 + j, v[j]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kSegSums; k++) v[j][k] = 0;
+      }
+#pragma unroll
+      for (int k = 0; k < kSegSums; k++) ex[k] += v[j][k];
+    }
+#pragma unroll
+    for (int k = 0; k < kSegSums; k++) {
+      const uint64_t incl = wave_inclusive_sum(ex[k]);
+      if (lane == kWave - 1) wtot[w][k] = incl;
+      ex[k] = incl - ex[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSegSums; k++) {
+      uint64_t woff = 0, tot = 0;
+#pragma unroll
+      for (int ww = 0; ww < kT / kWave; ww++) {
+        const uint64_t x = wtot[ww][k];
+        if (ww < w) woff += x;
+        tot += x;
+      }
+      ex[k] += carry[k] + woff;
+      carry[k] += tot;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSegItems; j++) {
+      const uint64_t i = i0 + j;
+      if (i < hi) {
+        a.alloc[i] = ex[0];
+        if (v[j][1]) {
+          a.big_idx[ex[1]] = i;
+          a.big_stage[ex[1]] = ex[3];
+          a.big_work[ex[1]] = ex[4];
+        }
+        if (v[j][2]) {
+          a.tile_idx[ex[2]] = i;
+          a.tile_len[ex[2]] = ex[5];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kSegSums; k++) ex[k] += v[j][k];
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    a.ctr[kCtrAlloc] = carry[0];
+    a.ctr[kCtrBig] = carry[1];
+    a.ctr[kCtrTiles] = carry[2];
+    a.ctr[kCtrStage] = carry[3];
+    a.ctr[kCtrWork] = carry[4];
+    a.ctr[kCtrTile] = carry[5];
+    a.big_stage[carry[1]] = carry[3];
+    a.big_work[carry[1]] = carry[4];
+    a.tile_len[carry[2]] = carry[5];
+  }
 }
 
 struct SegArgs {
@@ -877,23 +1035,36 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
               insert ? 1 : 0, g->n_cols, nseg, seg_v, seg_p0, seg_od,
               seg_alloc, seg_stage, seg_work, seg_tile, is_big, is_tile,
               ctr + kCtrGarbage);
-  SSTG_TRY(scan_exclusive_u64(seg_alloc, seg_alloc, nseg + 1, ctr + kCtrAlloc,
-                              ws, kWsPrim, s));
-  SSTG_TRY(select_flagged_u64(is_big, nullptr, nseg + 1, big_idx,
-                              ctr + kCtrBig, ws, kWsPrim, s));
-  SSTG_TRY(select_flagged_u64(is_tile, nullptr, nseg + 1, tile_idx,
-                              ctr + kCtrTiles, ws, kWsPrim, s));
-  SSTG_LAUNCH(k_class_lengths, blocks(nseg + 1), kT, 0, s,
-              (const uint64_t *)ctr, nseg, (const uint64_t *)big_idx,
-              (const uint64_t *)tile_idx, (const uint32_t *)seg_stage,
-              (const uint64_t *)seg_work, (const uint32_t *)seg_tile,
-              big_stage, big_work, tile_len);
-  SSTG_TRY(scan_exclusive_u64(big_stage, big_stage, nseg + 1, ctr + kCtrStage,
-                              ws, kWsPrim, s));
-  SSTG_TRY(scan_exclusive_u64(big_work, big_work, nseg + 1, ctr + kCtrWork, ws,
-                              kWsPrim, s));
-  SSTG_TRY(scan_exclusive_u64(tile_len, tile_len, nseg + 1, ctr + kCtrTile, ws,
-                              kWsPrim, s));
+  {
+    SegScan sc{};
+    sc.alloc = seg_alloc;
+    sc.is_big = is_big;
+    sc.is_tile = is_tile;
+    sc.stage = seg_stage;
+    sc.tile = seg_tile;
+    sc.work = seg_work;
+    sc.n = nseg + 1;
+    sc.big_idx = big_idx;
+    sc.big_stage = big_stage;
+    sc.big_work = big_work;
+    sc.tile_idx = tile_idx;
+    sc.tile_len = tile_len;
+    sc.ctr = ctr;
+    const uint64_t tiles = ceil_div(sc.n, kSegTile);
+    if (tiles <= kSegOneBlock) {
+      sc.per = sc.n;
+      sc.blocks = 1;
+      SSTG_LAUNCH(k_seg_scan, 1, kT, 0, s, sc);
+    } else {
+      const uint64_t tpb = ceil_div(tiles, kSegBlocks);
+      sc.per = tpb * kSegTile;
+      sc.blocks = (uint32_t)ceil_div(tiles, tpb);
+      SSTG_TRY(ws.get(kWsPrim, (uint64_t)kSegSums * sc.blocks, &sc.partials));
+      SSTG_LAUNCH(k_seg_reduce, sc.blocks, kT, 0, s, sc);
+      SSTG_LAUNCH(k_seg_scan_partials, 1, kT, 0, s, sc);
+      SSTG_LAUNCH(k_seg_scan, sc.blocks, kT, 0, s, sc);
+    }
+  }
   SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, kCtrCount * sizeof(uint64_t),
                           hipMemcpyDeviceToHost, s));
   SSTG_HIP(hipStreamSynchronize(s));
