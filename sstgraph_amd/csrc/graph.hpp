@@ -155,6 +155,8 @@ enum : int {
                       // counts, 120 keep flags, 121
                       // per-block partial sums
   kWsBfsBins = 122,   // degree bin lists of the BFS set, back to back
+  kWsMarkPos = 123,   // update: row position of each sorted batch key
+  kWsBPos = 124,      // update: the same for the kept keys
   kWsSlots = 126,
 };
 

@@ -170,12 +170,14 @@ int scan_impl(const T *in, T *out, uint64_t n, T *d_total, Workspace &ws,
 
 // ------------------------------------------------------------- select
 // Writes the flagged values of each block's range in order from its offset
-// (offsets null: one block, offset 0); the last block writes the count.
+// (offsets null: one block, offset 0), and values2 beside them into out2
+// when given; the last block writes the count.
 template <class V>
 __global__ void __launch_bounds__(kScanThreads)
     k_select_range(const uint8_t *flags, const V *values, uint64_t n,
                    uint64_t per, const uint64_t *offsets, V *out,
-                   uint64_t *d_count) {
+                   uint64_t *d_count, const uint32_t *values2,
+                   uint32_t *out2) {
   __shared__ uint64_t wtot[kScanThreads / kWave];
   const uint64_t lo = (uint64_t)blockIdx.x * per;
   const uint64_t hi = lo + per < n ? lo + per : n;
@@ -192,7 +194,10 @@ __global__ void __launch_bounds__(kScanThreads)
       uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
       uint64_t tot;
       uint64_t ex = block_exclusive((uint64_t)f[j], wtot, &tot);
-      if (f[j]) out[carry + ex] = values ? values[i] : (V)i;
+      if (f[j]) {
+        out[carry + ex] = values ? values[i] : (V)i;
+        if (out2) out2[carry + ex] = values2[i];
+      }
       carry += tot;
     }
   }
@@ -337,14 +342,16 @@ int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
 template <class V>
 static int select_flagged(const uint8_t *flags, const V *values, uint64_t n,
                           V *out, uint64_t *d_count, Workspace &ws,
-                          int ws_base, hipStream_t s) {
+                          int ws_base, hipStream_t s,
+                          const uint32_t *values2 = nullptr,
+                          uint32_t *out2 = nullptr) {
   if (n == 0) {
     SSTG_HIP(hipMemsetAsync(d_count, 0, sizeof(uint64_t), s));
     return SSTG_OK;
   }
   if (ceil_div(n, kScanTile) <= kScanOneBlock) {
     SSTG_LAUNCH(k_select_range<V>, 1, kScanThreads, 0, s, flags, values, n, n,
-                (const uint64_t *)nullptr, out, d_count);
+                (const uint64_t *)nullptr, out, d_count, values2, out2);
     return SSTG_OK;
   }
   const Ranges r = ranges_of(n);
@@ -355,7 +362,7 @@ static int select_flagged(const uint8_t *flags, const V *values, uint64_t n,
   SSTG_LAUNCH(k_scan_partials<uint64_t>, 1, kScanThreads, 0, s, counts,
               r.blocks);
   SSTG_LAUNCH(k_select_range<V>, r.blocks, kScanThreads, 0, s, flags, values,
-              n, r.per, (const uint64_t *)counts, out, d_count);
+              n, r.per, (const uint64_t *)counts, out, d_count, values2, out2);
   return SSTG_OK;
 }
 
@@ -364,6 +371,14 @@ int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
                        Workspace &ws, int ws_base, hipStream_t s) {
   return select_flagged<uint64_t>(flags, values, n, out, d_count, ws, ws_base,
                                   s);
+}
+
+int select_flagged_u64_u32(const uint8_t *flags, const uint64_t *values,
+                           const uint32_t *values2, uint64_t n, uint64_t *out,
+                           uint32_t *out2, uint64_t *d_count, Workspace &ws,
+                           int ws_base, hipStream_t s) {
+  return select_flagged<uint64_t>(flags, values, n, out, d_count, ws, ws_base,
+                                  s, values2, out2);
 }
 
 int select_flagged_u32(const uint8_t *flags, const uint32_t *values,

@@ -21,6 +21,11 @@ int scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint64_t n,
 int select_flagged_u64(const uint8_t *flags, const uint64_t *values,
                        uint64_t n, uint64_t *out, uint64_t *d_count,
                        Workspace &ws, int ws_base, hipStream_t s);
+// select_flagged_u64 that also compacts values2 into out2 in the same order.
+int select_flagged_u64_u32(const uint8_t *flags, const uint64_t *values,
+                           const uint32_t *values2, uint64_t n, uint64_t *out,
+                           uint32_t *out2, uint64_t *d_count, Workspace &ws,
+                           int ws_base, hipStream_t s);
 // The same for 32-bit values (values null: the indices, which must fit).
 int select_flagged_u32(const uint8_t *flags, const uint32_t *values,
                        uint64_t n, uint32_t *out, uint64_t *d_count,

@@ -81,25 +81,31 @@ __global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n,
   keys[i] = ((uint64_t)s << 32) | d;
 }
 
+// pos[i] = the row position where key i sits or would go (its lower bound
+// in the old row); the segment step takes each row's first moved position
+// from the row's first kept key.
 __global__ void k_mark(const uint64_t *keys, uint64_t n, DevGraph g,
-                       int insert, int skip_self, uint8_t *flags) {
+                       int insert, int skip_self, uint8_t *flags,
+                       uint32_t *pos) {
   uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
   uint64_t k = keys[i];
   bool keep = (i == 0) || (keys[i - 1] != k);
   uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)k;
   if (insert && skip_self && s == d) keep = false;
+  uint32_t p = 0;
   if (keep) {
     uint32_t dg = g.deg[s];
     bool exists = false;
     if (dg) {
       const uint32_t *row = g.pool + g.off[s];
-      uint64_t p = lower_bound_dev(row, (uint64_t)dg, d);
+      p = (uint32_t)lower_bound_dev(row, (uint64_t)dg, d);
       exists = p < dg && row[p] == d;
     }
     keep = insert ? !exists : exists;
   }
   flags[i] = keep ? 1 : 0;
+  pos[i] = p;
 }
 
 __global__ void k_heads(const uint64_t *B, const uint64_t *ctr, uint64_t nmax,
@@ -110,11 +116,13 @@ __global__ void k_heads(const uint64_t *B, const uint64_t *ctr, uint64_t nmax,
   flags[i] = (i < b) && (i == 0 || (B[i] >> 32) != (B[i - 1] >> 32));
 }
 
-// One thread per touched row (s < t) and one per unused slot s in [t, nmax]
-// (zero lengths, so scans over nmax + 1 entries give exact totals).  Each
-// row is classed as a tile row (merge inputs <= kTileMax, merged in place
-// through LDS) or a big row (suffix staged, merged over a flat index space).
-__global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
+// One thread per touched row (s < t); slot t gets zero lengths, which the
+// scans read as the end of the rows.  Each row is classed as a tile row
+// (merge inputs <= kTileMax, merged in place through LDS) or a big row
+// (suffix staged, merged over a flat index space).  p0 is the row position
+// k_mark found for the row's first kept key.
+__global__ void k_segprep(const uint64_t *B, const uint32_t *B_pos,
+                          const uint64_t *ctr,
                           const uint64_t *seg_idx, DevGraph g, int insert,
                           uint32_t n_cols, uint64_t nmax, uint32_t *seg_v,
                           uint32_t *seg_p0, uint32_t *seg_olddeg,
@@ -136,7 +144,6 @@ __global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
   uint64_t bs = seg_idx[s], be = (s + 1 < t) ? seg_idx[s + 1] : b;
   uint64_t blen = be - bs;
   uint32_t v = (uint32_t)(B[bs] >> 32);
-  uint32_t first = (uint32_t)B[bs];
   uint32_t od = g.deg[v], cp = g.cap[v];
   uint64_t p0 = 0, stage = 0, alloc = 0, work = 0;
   if (insert) {
@@ -146,12 +153,12 @@ __global__ void k_segprep(const uint64_t *B, const uint64_t *ctr,
       work = nd;
       if (cp) atomicAdd((unsigned long long *)garbage, (unsigned long long)cp);
     } else {
-      p0 = od ? lower_bound_dev(g.pool + g.off[v], (uint64_t)od, first) : 0;
+      p0 = od ? B_pos[bs] : 0;
       stage = od - p0;
       work = stage + blen;
     }
   } else {
-    p0 = lower_bound_dev(g.pool + g.off[v], (uint64_t)od, first);
+    p0 = B_pos[bs];
     stage = od - p0;
     work = stage;
   }
@@ -997,12 +1004,15 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
   uint64_t *sorted = nullptr;
   SSTG_TRY(radix_sort_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  uint32_t *pos = nullptr, *B_pos = nullptr;
+  SSTG_TRY(ws.get(kWsMarkPos, n, &pos));
+  SSTG_TRY(ws.get(kWsBPos, n, &B_pos));
   SSTG_LAUNCH(k_mark, blocks(n), kT, 0, s, (const uint64_t *)sorted, n,
-              g->view(), insert ? 1 : 0, skip_self ? 1 : 0, flags);
+              g->view(), insert ? 1 : 0, skip_self ? 1 : 0, flags, pos);
   // B reuses the buffer that does not hold the sorted keys
   B = (sorted == keys) ? tmp : keys;
-  SSTG_TRY(select_flagged_u64(flags, sorted, n, B, ctr + kCtrB, ws, kWsPrim,
-                              s));
+  SSTG_TRY(select_flagged_u64_u32(flags, sorted, pos, n, B, B_pos, ctr + kCtrB,
+                                  ws, kWsPrim, s));
   // at most one segment per row: segment arrays hold nseg + 1 entries
   const uint64_t nseg = n < (uint64_t)g->n_rows ? n : (uint64_t)g->n_rows;
   SSTG_TRY(ws.get(kWsSegIdx, nseg + 1, &seg_idx));
@@ -1031,7 +1041,7 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   SSTG_TRY(ws.get(kWsBigWork, nseg + 1, &big_work));
   SSTG_TRY(ws.get(kWsTileLen, nseg + 1, &tile_len));
   SSTG_LAUNCH(k_segprep, blocks(nseg + 1), kT, 0, s, (const uint64_t *)B,
-              (const uint64_t *)ctr, (const uint64_t *)seg_idx, g->view(),
+              (const uint32_t *)B_pos, (const uint64_t *)ctr, (const uint64_t *)seg_idx, g->view(),
               insert ? 1 : 0, g->n_cols, nseg, seg_v, seg_p0, seg_od,
               seg_alloc, seg_stage, seg_work, seg_tile, is_big, is_tile,
               ctr + kCtrGarbage);
