@@ -325,6 +325,95 @@ __global__ void __launch_bounds__(kRsThreads)
   }
 }
 
+// One-block LSD radix sort of up to kSsMax keys.  The keys stay in LDS
+// across all passes (each pass ranks, scans and scatters in place), so a
+// short batch costs one launch instead of three per pass.  Ranking is the
+// same wave-striped stable ballot ranking as k_rs_scatter, with 16 waves.
+constexpr int kSsThreads = 1024;
+constexpr int kSsWaves = kSsThreads / kWave;
+constexpr int kSsItems = 16;
+constexpr uint32_t kSsMax = kSsThreads * kSsItems;  // 16384 keys, 128 KB
+
+struct SortShifts {
+  int shift[16];
+  int passes;
+};
+
+__global__ void __launch_bounds__(kSsThreads)
+    k_sort_small(uint64_t *keys, uint32_t n, SortShifts sh) {
+  __shared__ uint64_t stage[kSsMax];
+  __shared__ uint32_t wcnt[kSsWaves][kRsBins];
+  __shared__ uint32_t tstart[kRsBins];
+  __shared__ uint32_t wtot[kSsWaves];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const uint32_t base = (uint32_t)w * (kWave * kSsItems);
+  for (uint32_t i = threadIdx.x; i < n; i += kSsThreads) stage[i] = keys[i];
+  for (int p = 0; p < sh.passes; p++) {
+    const int shift = sh.shift[p];
+    for (int i = threadIdx.x; i < kSsWaves * kRsBins; i += kSsThreads)
+      (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    uint64_t key[kSsItems];
+    uint32_t rank[kSsItems];
+#pragma unroll
+    for (int j = 0; j < kSsItems; j++) {
+      const uint32_t i = base + (uint32_t)j * kWave + lane;
+      const bool valid = i < n;
+      key[j] = valid ? stage[i] : 0;
+      const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const uint64_t m = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? m : ~m;
+      }
+      const uint32_t before = wcnt[w][d];
+      const uint32_t r = mask_rank(peers);
+      rank[j] = before + r;
+      if (valid && r == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    uint32_t dtot = 0;
+    if (threadIdx.x < kRsBins) {
+      const int d = threadIdx.x;
+      uint32_t run = 0;
+#pragma unroll
+      for (int ww = 0; ww < kSsWaves; ww++) {
+        const uint32_t c = wcnt[ww][d];
+        wcnt[ww][d] = run;
+        run += c;
+      }
+      dtot = run;
+    }
+    {
+      const uint32_t incl = wave_inclusive_sum(dtot);
+      if (lane == kWave - 1) wtot[w] = incl;
+      __syncthreads();
+      uint32_t woff = 0;
+#pragma unroll
+      for (int ww = 0; ww < kRsBins / kWave; ww++)
+        if (ww < w) woff += wtot[ww];
+      if (threadIdx.x < kRsBins) tstart[threadIdx.x] = woff + incl - dtot;
+    }
+    __syncthreads();
+    // every key of the pass is in registers: scatter in place
+#pragma unroll
+    for (int j = 0; j < kSsItems; j++) {
+      const uint32_t i = base + (uint32_t)j * kWave + lane;
+      if (i < n) {
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+        stage[tstart[d] + wcnt[w][d] + rank[j]] = key[j];
+      }
+    }
+    // the scatter reads wcnt and tstart: no wave may clear them for the
+    // next pass before every wave has placed its keys
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < n; i += kSsThreads) keys[i] = stage[i];
+}
+
 }  // namespace
 
 int scan_exclusive_u64(const uint64_t *in, uint64_t *out, uint64_t n,
@@ -393,7 +482,14 @@ int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
                    hipStream_t s, uint64_t **result) {
   *result = keys;
   if (n <= 1 || npass == 0) return SSTG_OK;
-  if (n >= (1ull << 32)) return SSTG_E_INVAL;
+  if (n >= (1ull << 32) || npass > 16) return SSTG_E_INVAL;
+  if (n <= kSsMax) {
+    SortShifts sh{};
+    for (int p = 0; p < npass; p++) sh.shift[p] = shifts[p];
+    sh.passes = npass;
+    SSTG_LAUNCH(k_sort_small, 1, kSsThreads, 0, s, keys, (uint32_t)n, sh);
+    return SSTG_OK;
+  }
   uint32_t tiles = (uint32_t)ceil_div(n, kRsTile);
   uint32_t *hist = nullptr;
   SSTG_TRY(ws.get(ws_base, (uint64_t)tiles * kRsBins, &hist));
