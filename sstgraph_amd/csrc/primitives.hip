@@ -347,7 +347,10 @@ __global__ void __launch_bounds__(kSsThreads)
   __shared__ uint32_t wtot[kSsWaves];
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
-  const uint32_t base = (uint32_t)w * (kWave * kSsItems);
+  // each wave owns a contiguous run of `per` 64-key items (stability needs
+  // wave order = key order), sized to n so that short batches use all waves
+  const uint32_t per = (((n + kWave - 1) / kWave) + kSsWaves - 1) / kSsWaves;
+  const uint32_t base = (uint32_t)w * per * kWave;
   for (uint32_t i = threadIdx.x; i < n; i += kSsThreads) stage[i] = keys[i];
   for (int p = 0; p < sh.passes; p++) {
     const int shift = sh.shift[p];
@@ -358,6 +361,7 @@ __global__ void __launch_bounds__(kSsThreads)
     uint32_t rank[kSsItems];
 #pragma unroll
     for (int j = 0; j < kSsItems; j++) {
+      if ((uint32_t)j >= per) break;  // uniform across the block
       const uint32_t i = base + (uint32_t)j * kWave + lane;
       const bool valid = i < n;
       key[j] = valid ? stage[i] : 0;
@@ -401,6 +405,7 @@ __global__ void __launch_bounds__(kSsThreads)
     // every key of the pass is in registers: scatter in place
 #pragma unroll
     for (int j = 0; j < kSsItems; j++) {
+      if ((uint32_t)j >= per) break;
       const uint32_t i = base + (uint32_t)j * kWave + lane;
       if (i < n) {
         const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;

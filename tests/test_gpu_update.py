@@ -54,6 +54,25 @@ def test_insert_batches_match_oracle(n, sizes):
         assert_same_csr(g, og, f"insert n={n} k={k}")
 
 
+def test_batch_sizes_around_the_sort_and_scan_paths():
+    """Batches on both sides of the one-block LDS sort (16384 keys) and of
+    the one-block scans (4096 values), inserted and then half removed; ids
+    below 2^17 so every batch sorts on three column and three row digits."""
+    n = 1 << 17
+    rng = np.random.default_rng(11)
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    for k in (2, 63, 64, 65, 4095, 4096, 4097, 16383, 16384, 16385, 40000):
+        say("batch", k)
+        s = rng.integers(0, n, k).astype(np.uint32)
+        d = rng.integers(0, n, k).astype(np.uint32)
+        g.insert_batch(s, d)
+        og.insert_batch(s, d)
+        assert_same_csr(g, og, f"insert k={k}")
+        g.remove_batch(s[::2], d[::2])
+        og.remove_batch(s[::2], d[::2])
+        assert_same_csr(g, og, f"remove k={k}")
+
+
 def test_remove_batches_match_oracle():
     rng = np.random.default_rng(7)
     n = 5000
