@@ -627,17 +627,22 @@ __global__ void k_tile_rows(SegArgs a, DevGraph g, int insert,
   t_bst[i] = bst;
 }
 
-// First tile row of each block: blk_r0[b] = lower_bound(tile_off, b*kTile).
+// First tile row of each block and that row's input offset, side by side:
+// blk[2b] = r0 = lower_bound(tile_off, b*kTile), blk[2b+1] = tile_off[r0].
+// A merge block reads its range and the next one's in one 32-byte load
+// instead of two dependent ones.
 __global__ void k_tile_blocks(const uint64_t *tile_off, uint64_t nt,
-                              uint64_t nblocks, uint64_t *blk_r0) {
+                              uint64_t nblocks, uint64_t *blk) {
   uint64_t b = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (b > nblocks) return;
-  blk_r0[b] = b == nblocks ? nt
-                           : lower_bound_dev(tile_off, nt, b * (uint64_t)kTile);
+  const uint64_t r0 =
+      b == nblocks ? nt : lower_bound_dev(tile_off, nt, b * (uint64_t)kTile);
+  blk[2 * b] = r0;
+  blk[2 * b + 1] = tile_off[r0];
 }
 
 __global__ void __launch_bounds__(kMergeThreads)
-    k_merge_tile(SegArgs a, DevGraph g, int insert, const uint64_t *blk_r0,
+    k_merge_tile(SegArgs a, DevGraph g, int insert, const uint64_t *blk,
                  const uint64_t *t_src, const uint64_t *t_dst,
                  const uint32_t *t_alen, const uint32_t *t_bst) {
   __shared__ uint32_t buf[kTileLds];
@@ -646,9 +651,9 @@ __global__ void __launch_bounds__(kMergeThreads)
   __shared__ uint32_t rbstart[kTileRows];
   __shared__ uint64_t rsrc[kTileRows];
   __shared__ uint64_t rdst[kTileRows];
-  const uint64_t r0 = blk_r0[blockIdx.x], r1 = blk_r0[blockIdx.x + 1];
+  const uint64_t *bk = blk + 2 * (uint64_t)blockIdx.x;
+  const uint64_t r0 = bk[0], e0 = bk[1], r1 = bk[2], e1 = bk[3];
   if (r0 >= r1) return;
-  const uint64_t e0 = a.tile_off[r0], e1 = a.tile_off[r1];
   const uint32_t W = (uint32_t)(e1 - e0);
   const uint32_t nrows = (uint32_t)(r1 - r0);
   const bool table = nrows <= (uint32_t)kTileRows;
@@ -1119,20 +1124,20 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   a.nt = nt;
   if (tile_tot) {
     const uint64_t nblk = ceil_div(tile_tot, kTile);
-    uint64_t *t_src = nullptr, *t_dst = nullptr, *blk_r0 = nullptr;
+    uint64_t *t_src = nullptr, *t_dst = nullptr, *blk = nullptr;
     uint32_t *t_alen = nullptr, *t_bst = nullptr;
     SSTG_TRY(ws.get(kWsTileSrc, nt + 1, &t_src));
     SSTG_TRY(ws.get(kWsTileDst, nt + 1, &t_dst));
     SSTG_TRY(ws.get(kWsTileAlen, nt + 1, &t_alen));
     SSTG_TRY(ws.get(kWsTileBst, nt + 1, &t_bst));
-    SSTG_TRY(ws.get(kWsTileBlk, nblk + 1, &blk_r0));
+    SSTG_TRY(ws.get(kWsTileBlk, 2 * (nblk + 1), &blk));
     SSTG_LAUNCH(k_tile_rows, blocks(nt), kT, 0, s, a, g->view(),
                 insert ? 1 : 0, t_src, t_dst, t_alen, t_bst);
     SSTG_LAUNCH(k_tile_blocks, blocks(nblk + 1), kT, 0, s,
-                (const uint64_t *)tile_len, nt, nblk, blk_r0);
+                (const uint64_t *)tile_len, nt, nblk, blk);
     SSTG_LAUNCH(k_merge_tile, nblk, kMergeThreads, 0, s, a, g->view(),
                 insert ? 1 : 0,
-                (const uint64_t *)blk_r0, (const uint64_t *)t_src,
+                (const uint64_t *)blk, (const uint64_t *)t_src,
                 (const uint64_t *)t_dst, (const uint32_t *)t_alen,
                 (const uint32_t *)t_bst);
   }
