@@ -600,6 +600,7 @@ __device__ __forceinline__ void tile_row_info(const SegArgs &a,
 }
 
 constexpr int kTileRows = 128;  // rows per block described in LDS
+static_assert(kTileRows <= 256, "tile rows are numbered in bytes");
 // k_merge_tile runs 256 threads over one LDS tile: the 12 KB tile and a
 // 128-row table take 15.6 KB, so eight tiles (the wave limit) are in flight
 // per CU; tiles with more rows read their row descriptors from HBM.  The
@@ -651,6 +652,7 @@ __global__ void __launch_bounds__(kMergeThreads)
   __shared__ uint32_t rbstart[kTileRows];
   __shared__ uint64_t rsrc[kTileRows];
   __shared__ uint64_t rdst[kTileRows];
+  __shared__ uint8_t rmap[kTileLds];  // tile row of each input (table path)
   const uint64_t *bk = blk + 2 * (uint64_t)blockIdx.x;
   const uint64_t r0 = bk[0], e0 = bk[1], r1 = bk[2], e1 = bk[3];
   if (r0 >= r1) return;
@@ -667,6 +669,15 @@ __global__ void __launch_bounds__(kMergeThreads)
     }
     if (threadIdx.x == 0) rstart[nrows] = W;
     __syncthreads();
+    // each wave paints whole rows into the row map (coalesced LDS bytes), so
+    // that an input finds its row with one read instead of a search
+    const uint32_t ln = threadIdx.x % kWave;
+    for (uint32_t r = threadIdx.x / kWave; r < nrows;
+         r += kMergeThreads / kWave) {
+      const uint32_t b1 = rstart[r + 1];
+      for (uint32_t e = rstart[r] + ln; e < b1; e += kWave) rmap[e] = (uint8_t)r;
+    }
+    __syncthreads();
   }
   uint16_t row_of[kMergeItems];
 #pragma unroll
@@ -678,15 +689,7 @@ __global__ void __launch_bounds__(kMergeThreads)
       uint64_t src;
       uint32_t alen, bst, j;
       if (table) {
-        uint32_t l = 0, h = nrows;  // largest r with rstart[r] <= e
-        while (h - l > 1) {
-          uint32_t m = (l + h) >> 1;
-          if (rstart[m] <= e)
-            l = m;
-          else
-            h = m;
-        }
-        r = l;
+        r = rmap[e];
         src = rsrc[r];
         alen = ralen[r];
         bst = rbstart[r];
