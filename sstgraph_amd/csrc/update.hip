@@ -601,9 +601,10 @@ __device__ __forceinline__ void tile_row_info(const SegArgs &a,
 
 constexpr int kTileRows = 128;  // rows per block described in LDS
 static_assert(kTileRows <= 256, "tile rows are numbered in bytes");
-// k_merge_tile runs 256 threads over one LDS tile: the 12 KB tile and a
-// 128-row table take 15.6 KB, so eight tiles (the wave limit) are in flight
-// per CU; tiles with more rows read their row descriptors from HBM.  The
+// k_merge_tile runs 256 threads over one LDS tile: the 12 KB tile, a
+// 128-row table and the 3 KB row map take 18.5 KB, so eight tiles (the wave
+// limit) are in flight per CU; tiles with more rows read their row
+// descriptors from HBM and search them.  The
 // block shape was chosen by timing the 1e7-edge insert at scale 24 (merge
 // launch 1.13 ms; 1.29 ms with a 2048-input range and a 256-row table, 1.40
 // ms with 1024-thread blocks, 1.17 ms with 1024 for both tile bounds, whose
