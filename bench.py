@@ -122,7 +122,22 @@ def cpu_baseline(off, cols, n, bs, bd, budget_s=30.0):
                 sample=f"oracle insert_batch of one {len(bs)}-edge batch into "
                        f"the scale-24 base graph, median of {len(times)} runs",
                 single_core=dict(value=len(bs) / med1, unit="edges/s", cores=1,
-                                 runs=len(t1)))
+                                 runs=len(t1)),
+                host=host_cpu())
+
+
+def host_cpu():
+    """The host's logical CPU count and model name (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(logical_cpus=os.cpu_count(), model=model)
 
 
 def cpu_algos(off, cols, hub, iters=2):
@@ -712,12 +727,17 @@ def main():
         # step the edge count is stable at m minus those edges
         m_after = g.M()
         g.insert_batch(ds, dd)
+        new_edges = g.last_update_stats()["effective"]
         g.remove_batch(ds, dd)
         assert g.M() == m_after, "update steps must leave the graph stable"
         rec = dict(steps=steps, insert_ms=1e3 * float(np.mean(ti)),
                    remove_ms=1e3 * float(np.mean(tr)),
                    inserts_per_s=size / float(np.mean(ti)),
-                   removes_per_s=size / float(np.mean(tr)))
+                   removes_per_s=size / float(np.mean(tr)),
+                   # edges each timed insert actually added (distinct, not
+                   # self-loops, not already present)
+                   unique_new_edges=new_edges,
+                   unique_new_per_s=new_edges / float(np.mean(ti)))
         rec["m_after"] = m_after
         per_size[str(size)] = rec
         log(f"batch {size}: insert {rec['insert_ms']:.3f} ms "
