@@ -89,7 +89,9 @@ __device__ __forceinline__ uint32_t hash_has(const uint32_t *table, uint32_t x,
 // Streams the concatenation of the lists out(v) (list j starts at dag +
 // base[j], its flat range is [pre[j], pre[j+1]), pre[k] = total) through the
 // hash table, P lanes striding the flat index; four loads are issued before
-// their probes so that each lane keeps four gathers in flight.
+// their probes so that each lane keeps four gathers in flight.  A lane keeps
+// its current list's end and address shift in registers and reads the LDS
+// tables only when it crosses into another list.
 template <uint32_t P>
 __device__ __forceinline__ uint64_t probe_lists(const uint32_t *table,
                                                 uint32_t bits,
@@ -98,7 +100,9 @@ __device__ __forceinline__ uint64_t probe_lists(const uint32_t *table,
                                                 const uint32_t *dag,
                                                 uint32_t total, uint32_t me) {
   uint64_t found = 0;
-  uint32_t seg = 0;
+  if (me >= total) return 0;
+  uint32_t seg = 0, seg_start = 0, seg_end = pre[1];
+  uint64_t shift = base[0];  // dag offset of flat index t: shift + t
   for (uint32_t t0 = me; t0 < total; t0 += 4 * P) {
     uint32_t w[4];
 #pragma unroll
@@ -106,8 +110,15 @@ __device__ __forceinline__ uint64_t probe_lists(const uint32_t *table,
       const uint32_t t = t0 + r * P;
       w[r] = kEmpty;
       if (t < total) {
-        while (pre[seg + 1] <= t) seg++;
-        w[r] = dag[base[seg] + (t - pre[seg])];
+        if (seg_end <= t) {
+          do {
+            seg++;
+            seg_start = seg_end;
+            seg_end = pre[seg + 1];
+          } while (seg_end <= t);
+          shift = base[seg] - seg_start;  // wraps; shift + t is exact
+        }
+        w[r] = dag[shift + t];
       }
     }
 #pragma unroll
