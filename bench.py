@@ -192,6 +192,35 @@ def c4_edges(S, spec):
     return n, src, dst
 
 
+def c4_update_batches(S, specs, frac=100, seed_shift=7919):
+    """One fresh update batch per batched-workload graph, of uneven sizes
+    (about 1/frac of the graph's generated edges, both directions), back to
+    back in two device arrays; returns (graphs, counts, src, dst)."""
+    L = S.lib()
+    counts = []
+    for kind, scale, param, seed in specs:
+        n = 1 << scale
+        counts.append(max(8, (param * n if kind == "rmat" else param * n // 2) // frac))
+    total = 2 * sum(counts)
+    src = S.DeviceArray(total, np.uint32)
+    dst = S.DeviceArray(total, np.uint32)
+    off = 0
+    for (kind, scale, param, seed), c in zip(specs, counts):
+        n = 1 << scale
+        ps = ctypes.c_void_p(src.ptr.value + 4 * off)
+        pd = ctypes.c_void_p(dst.ptr.value + 4 * off)
+        if kind == "rmat":
+            S._check(L.sstg_rmat(n, seed + seed_shift, 0.5, 0.1, 0.1, 0, c, ps, pd,
+                                 S.SSTG_DEVICE_PTRS | RMAT_FLAGS), "rmat")
+        else:
+            S._check(L.sstg_er(n, seed + seed_shift, 0, c, ps, pd,
+                               S.SSTG_DEVICE_PTRS), "er")
+        S._check(L.sstg_memcpy_d2d(ctypes.c_void_p(ps.value + 4 * c), pd, 4 * c), "d2d")
+        S._check(L.sstg_memcpy_d2d(ctypes.c_void_p(pd.value + 4 * c), ps, 4 * c), "d2d")
+        off += 2 * c
+    return list(range(len(specs))), [2 * c for c in counts], src, dst
+
+
 def run_batched(args):
     """512 graphs: BFS (from each graph's largest hub), CC and 10 PageRank
     iterations on all of them from one hipGraph replay, against per-graph
@@ -253,6 +282,64 @@ def run_batched(args):
         pr = dr.to_host()
         assert np.allclose(M.part(res64["rank"], g), pr, rtol=1e-9, atol=0)
     t_b, t_p = float(np.median(tb)), float(np.median(tp))
+    # update batches of every graph: one insert_batches + one remove_batches
+    # call over the union store, against one insert and one remove call per
+    # graph on the same device data
+    gids, counts, us, ud = c4_update_batches(S, specs)
+    L = S.lib()
+    m_union = M.edges()
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+    def upd_batched():
+        M.insert_batches_device(gids, counts, us, ud)
+        S.synchronize()
+        ti = time.perf_counter()
+        M.remove_batches_device(gids, counts, us, ud)
+        S.synchronize()
+        return ti
+
+    def upd_plain():
+        for g in gids:
+            ps = ctypes.c_void_p(us.ptr.value + 4 * int(offs[g]))
+            pd = ctypes.c_void_p(ud.ptr.value + 4 * int(offs[g]))
+            S._check(L.sstg_multi_insert_batch(M._h, g, ps, pd, counts[g],
+                                               S.SSTG_DEVICE_PTRS), "insert")
+        S.synchronize()
+        ti = time.perf_counter()
+        for g in gids:
+            ps = ctypes.c_void_p(us.ptr.value + 4 * int(offs[g]))
+            pd = ctypes.c_void_p(ud.ptr.value + 4 * int(offs[g]))
+            gi = np.array([g], np.uint32)
+            cnt = np.array([counts[g]], np.uint64)
+            S._check(L.sstg_multi_remove_batches(M._h, 1, S._ptr(gi), S._ptr(cnt),
+                                                 ps, pd, S.SSTG_DEVICE_PTRS), "remove")
+        S.synchronize()
+        return ti
+
+    def timed_pair(fn):
+        t0 = time.perf_counter()
+        ti = fn()
+        return ti - t0, time.perf_counter() - ti
+
+    for fn in (upd_batched, upd_plain):
+        for _ in range(args.warmup):
+            fn()
+    ub = [timed_pair(upd_batched) for _ in range(args.steps)]
+    up = [timed_pair(upd_plain) for _ in range(max(1, args.steps // 2))]
+    assert M.edges() <= m_union, "insert+remove of fresh batches keeps the union"
+    us.free()
+    ud.free()
+    upd = dict(batch_edges=int(sum(counts)),
+               batched_insert_ms=1e3 * float(np.median([x[0] for x in ub])),
+               batched_remove_ms=1e3 * float(np.median([x[1] for x in ub])),
+               plain_insert_ms=1e3 * float(np.median([x[0] for x in up])),
+               plain_remove_ms=1e3 * float(np.median([x[1] for x in up])))
+    upd["speedup_insert"] = upd["plain_insert_ms"] / upd["batched_insert_ms"]
+    upd["speedup_remove"] = upd["plain_remove_ms"] / upd["batched_remove_ms"]
+    log(f"update batches of all graphs ({upd['batch_edges']} edges): insert "
+        f"{upd['batched_insert_ms']:.2f} ms in one call vs {upd['plain_insert_ms']:.2f} ms "
+        f"per graph ({upd['speedup_insert']:.1f}x); remove {upd['batched_remove_ms']:.2f} "
+        f"vs {upd['plain_remove_ms']:.2f} ms ({upd['speedup_remove']:.1f}x)")
     out = {
         "metric": "batched BFS+CC+PR(10) over 512 graphs: graph-algorithm passes/s",
         "value": 1.0 / t_b, "unit": "passes/s", "n_gpus": 1, "steps": args.steps,
@@ -268,7 +355,8 @@ def run_batched(args):
                   "bfs_levels": res64["levels_used"],
                   "autotune": tune,
                   "pr_f32_kahan_max_rel_err_vs_f64": float(rel.max()),
-                  "pr_f32_kahan_mean_rel_err_vs_f64": float(rel.mean())},
+                  "pr_f32_kahan_mean_rel_err_vs_f64": float(rel.mean()),
+                  "update_batches": upd},
     }
     log(f"batched {1e3 * t_b:.2f} ms vs plain {1e3 * t_p:.2f} ms per pass "
         f"({t_p / t_b:.1f}x); FP32+Kahan PR max rel err {rel.max():.2e}")

@@ -179,6 +179,20 @@ int sstg_multi_destroy(sstg_multi *m);
 /* insert_batch (SM:556-643) into graph `graph`; ids are local to it. */
 int sstg_multi_insert_batch(sstg_multi *m, uint32_t graph, const uint32_t *src,
                             const uint32_t *dst, uint64_t n, uint32_t flags);
+/* insert_batch (SM:556-643) / remove_batch (SM:645-718) of nb batches of
+ * several graphs in ONE update pass over the union store.  Batch j holds
+ * count[j] edges of graph graph[j] (ids local to it); src/dst are the batches
+ * concatenated in order (host or device arrays per flags).  The reference
+ * has no counterpart that takes several graphs at once: it is the batched
+ * form of calling sstg_multi_insert_batch per graph, with the same result.
+ * graph[j] >= k is SSTG_E_INVAL; a vertex id outside its batch's graph is
+ * SSTG_E_RANGE and leaves every graph unchanged. */
+int sstg_multi_insert_batches(sstg_multi *m, uint32_t nb, const uint32_t *graph,
+                              const uint64_t *count, const uint32_t *src,
+                              const uint32_t *dst, uint32_t flags);
+int sstg_multi_remove_batches(sstg_multi *m, uint32_t nb, const uint32_t *graph,
+                              const uint64_t *count, const uint32_t *src,
+                              const uint32_t *dst, uint32_t flags);
 /* The union as an ordinary graph handle (owned by m; do not destroy). */
 int sstg_multi_graph(sstg_multi *m, sstg_graph **out);
 /* BFS from sources[g] in every graph g (levels as sstg_bfs), CC labels (as

@@ -179,6 +179,8 @@ def lib():
         "sstg_multi_create": (i32, [u32, vp, P(vp)]),
         "sstg_multi_destroy": (i32, [vp]),
         "sstg_multi_insert_batch": (i32, [vp, u32, vp, vp, u64, u32]),
+        "sstg_multi_insert_batches": (i32, [vp, u32, vp, vp, vp, vp, u32]),
+        "sstg_multi_remove_batches": (i32, [vp, u32, vp, vp, vp, vp, u32]),
         "sstg_multi_graph": (i32, [vp, P(vp)]),
         "sstg_multi_run": (i32, [vp, vp, u32, u32, ctypes.c_double, i32]),
         "sstg_multi_results": (i32, [vp, vp, vp, vp, vp, P(u32)]),
@@ -361,6 +363,69 @@ class MultiGraph:
         s, d = _u32(src), _u32(dst)
         _check(lib().sstg_multi_insert_batch(self._h, graph, _ptr(s), _ptr(d),
                                              s.size, 0), "multi insert")
+
+    def _batches(self, fn, batches, what):
+        g = np.array([b[0] for b in batches], dtype=np.uint32)
+        srcs = [_u32(b[1]) for b in batches]
+        dsts = [_u32(b[2]) for b in batches]
+        c = np.array([min(a.size, b.size) for a, b in zip(srcs, dsts)],
+                     dtype=np.uint64)
+        s = np.ascontiguousarray(np.concatenate(
+            [a[:k] for a, k in zip(srcs, c)] + [np.empty(0, np.uint32)]))
+        d = np.ascontiguousarray(np.concatenate(
+            [a[:k] for a, k in zip(dsts, c)] + [np.empty(0, np.uint32)]))
+        _check(fn(self._h, g.size, _ptr(g), _ptr(c), _ptr(s), _ptr(d), 0),
+               what)
+
+    def _batches_device(self, fn, graphs, counts, src, dst, what):
+        g = _u32(graphs)
+        c = np.ascontiguousarray(counts, dtype=np.uint64)
+        if int(c.sum()) > min(src.count, dst.count):
+            raise ValueError("counts exceed the device arrays")
+        _check(fn(self._h, g.size, _ptr(g), _ptr(c), src.ptr, dst.ptr,
+                  SSTG_DEVICE_PTRS), what)
+
+    def insert_batches(self, batches):
+        """insert_batch of several graphs in one update pass: batches is a
+        list of (graph, src, dst) with ids local to each graph."""
+        self._batches(lib().sstg_multi_insert_batches, batches,
+                      "multi insert batches")
+
+    def remove_batches(self, batches):
+        """remove_batch of several graphs in one update pass."""
+        self._batches(lib().sstg_multi_remove_batches, batches,
+                      "multi remove batches")
+
+    def insert_batches_device(self, graphs, counts, src, dst):
+        """insert_batches with the batches back to back in DeviceArrays."""
+        self._batches_device(lib().sstg_multi_insert_batches, graphs, counts,
+                             src, dst, "multi insert batches")
+
+    def remove_batches_device(self, graphs, counts, src, dst):
+        self._batches_device(lib().sstg_multi_remove_batches, graphs, counts,
+                             src, dst, "multi remove batches")
+
+    def edges(self):
+        """Directed edges stored over all graphs."""
+        h = ctypes.c_void_p()
+        _check(lib().sstg_multi_graph(self._h, ctypes.byref(h)), "multi graph")
+        out = ctypes.c_uint64(0)
+        _check(lib().sstg_num_edges(h, ctypes.byref(out)), "M")
+        return out.value
+
+    def export_csr(self, graph):
+        """Graph `graph`'s rows as (offsets, columns) with local ids."""
+        h = ctypes.c_void_p()
+        _check(lib().sstg_multi_graph(self._h, ctypes.byref(h)), "multi graph")
+        n = int(self.vbase[-1])
+        off = np.empty(n + 1, np.uint64)
+        _check(lib().sstg_export_csr(h, _ptr(off), None, 0), "export_csr")
+        cols = np.empty(max(int(off[-1]), 1), np.uint32)
+        _check(lib().sstg_export_csr(h, _ptr(off), _ptr(cols), 0), "export_csr")
+        lo, hi = int(self.vbase[graph]), int(self.vbase[graph + 1])
+        part = off[lo:hi + 1]
+        c = cols[int(part[0]):int(part[-1])] - np.uint32(lo)
+        return part - part[0], c
 
     def autotune(self, pr_iters=2):
         """Tunes the degree thresholds of the union store on PageRank

@@ -154,6 +154,8 @@ enum : int {
                       // 114 their bin offsets, 115 per-bin partial
                       // counts, 120 keep flags, 121
                       // per-block partial sums
+  kWsMultiOff = 116,  // multi store: batch offsets of a batched update
+  kWsMultiBase = 117, // multi store: vertex base and size of each batch
   kWsBfsBins = 122,   // degree bin lists of the BFS set, back to back
   kWsMarkPos = 123,   // update: row position of each sorted batch key
   kWsBPos = 124,      // update: the same for the kept keys

@@ -77,6 +77,32 @@ __global__ void k_localize(const uint32_t *src, const uint32_t *dst,
   gdst[i] = d < ng ? d + base : bad;
 }
 
+// Batches of several graphs, concatenated: element i of batch j (off[j] <= i
+// < off[j+1]) is shifted by its graph's vertex base bbase[j]; ids outside that
+// graph (>= bsize[j]) become `bad`, so that the update rejects the whole call.
+__global__ void k_localize_batches(const uint32_t *src, const uint32_t *dst,
+                                   uint64_t n, const uint64_t *off,
+                                   uint32_t nb, const uint32_t *bbase,
+                                   const uint32_t *bsize, uint32_t bad,
+                                   uint32_t *gsrc, uint32_t *gdst) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  // the batch holding i: largest j < nb with off[j] <= i (empty batches
+  // share their offset with the next one, which is larger)
+  uint32_t lo = 0, hi = nb;  // off[lo] <= i < off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const uint32_t base = bbase[lo], ng = bsize[lo];
+  const uint32_t s = src[i], d = dst[i];
+  gsrc[i] = s < ng ? s + base : bad;
+  gdst[i] = d < ng ? d + base : bad;
+}
+
 // ---- multi-source BFS, one kernel per level (top-down over all vertices)
 __global__ void k_mbfs_init(uint32_t n, const uint32_t *vbase,
                             const uint32_t *graph_of, const uint32_t *sources,
@@ -337,6 +363,59 @@ static int capture(sstg_multi *M, uint32_t max_levels, uint32_t iters,
   return SSTG_OK;
 }
 
+// One update pass over the union store for nb batches of uneven size: the
+// batches are localised into union ids by one kernel and merged by one
+// insert (or remove) of their concatenation.  Invalid graph ids are rejected
+// before anything runs; out-of-range vertex ids fail the whole call with
+// SSTG_E_RANGE and leave every graph unchanged.
+static int multi_update_batches(sstg_multi *M, uint32_t nb, const uint32_t *graph,
+                                const uint64_t *count, const uint32_t *src,
+                                const uint32_t *dst, uint32_t flags, bool insert) {
+  if (!M || (nb && (!graph || !count))) return SSTG_E_INVAL;
+  std::vector<uint64_t> off(nb + 1);
+  std::vector<uint32_t> bb(2 * (size_t)nb + 1);  // bases, then sizes
+  uint64_t n = 0;
+  for (uint32_t j = 0; j < nb; j++) {
+    if (graph[j] >= M->k) return SSTG_E_INVAL;
+    off[j] = n;
+    n += count[j];
+    bb[j] = M->vbase[graph[j]];
+    bb[nb + j] = M->vbase[graph[j] + 1] - M->vbase[graph[j]];
+  }
+  off[nb] = n;
+  if (n == 0) return SSTG_OK;
+  if (!src || !dst) return SSTG_E_INVAL;
+  sstg_graph *g = M->u;
+  hipSetDevice(g->device);
+  hipStream_t s = g->stream;
+  uint32_t *ds = nullptr, *dd = nullptr, *gs = nullptr, *gd = nullptr,
+           *d_bb = nullptr;
+  uint64_t *d_off = nullptr;
+  SSTG_TRY(g->ws.get(kWsMulti, n, &gs));
+  SSTG_TRY(g->ws.get(kWsMulti + 1, n, &gd));
+  SSTG_TRY(g->ws.get(kWsMultiOff, nb + 1, &d_off));
+  SSTG_TRY(g->ws.get(kWsMultiBase, 2 * (uint64_t)nb + 1, &d_bb));
+  SSTG_HIP(hipMemcpyAsync(d_off, off.data(), (nb + 1) * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, s));
+  SSTG_HIP(hipMemcpyAsync(d_bb, bb.data(), bb.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+  if (flags & SSTG_DEVICE_PTRS) {
+    ds = (uint32_t *)src;
+    dd = (uint32_t *)dst;
+  } else {
+    SSTG_TRY(g->ws.get(kWsMulti + 2, n, &ds));
+    SSTG_TRY(g->ws.get(kWsMulti + 3, n, &dd));
+    SSTG_HIP(hipMemcpyAsync(ds, src, n * 4, hipMemcpyHostToDevice, s));
+    SSTG_HIP(hipMemcpyAsync(dd, dst, n * 4, hipMemcpyHostToDevice, s));
+  }
+  SSTG_LAUNCH(k_localize_batches, blocks(n), kT, 0, s, (const uint32_t *)ds,
+              (const uint32_t *)dd, n, (const uint64_t *)d_off, nb,
+              (const uint32_t *)d_bb, (const uint32_t *)(d_bb + nb),
+              g->n_rows, gs, gd);
+  const bool skip_self = insert && !(flags & SSTG_KEEP_SELF_LOOPS);
+  return graph_update(g, gs, gd, n, insert, skip_self, true);
+}
+
 }  // namespace sstg
 
 using namespace sstg;
@@ -443,6 +522,18 @@ int sstg_multi_insert_batch(sstg_multi *M, uint32_t graph, const uint32_t *src,
               (const uint32_t *)dd, n, base, ng, g->n_rows, gs, gd);
   return graph_update(g, gs, gd, n, true, !(flags & SSTG_KEEP_SELF_LOOPS),
                       true);
+}
+
+int sstg_multi_insert_batches(sstg_multi *M, uint32_t nb, const uint32_t *graph,
+                              const uint64_t *count, const uint32_t *src,
+                              const uint32_t *dst, uint32_t flags) {
+  return multi_update_batches(M, nb, graph, count, src, dst, flags, true);
+}
+
+int sstg_multi_remove_batches(sstg_multi *M, uint32_t nb, const uint32_t *graph,
+                              const uint64_t *count, const uint32_t *src,
+                              const uint32_t *dst, uint32_t flags) {
+  return multi_update_batches(M, nb, graph, count, src, dst, flags, false);
 }
 
 int sstg_multi_run(sstg_multi *M, const uint32_t *sources, uint32_t max_levels,
