@@ -55,8 +55,8 @@ enum {
 // merged in place by one block through LDS; larger rows are staged.  A block
 // owns the rows whose inputs start in its kTile range, so its tile holds at
 // most kTile + kTileMax inputs.
-constexpr int kTile = 1024;
-constexpr int kTileMax = 2048;
+constexpr int kTile = 1536;
+constexpr int kTileMax = 1536;
 constexpr int kTileLds = kTile + kTileMax;
 
 __device__ __forceinline__ uint64_t grow_cap(uint64_t d, uint32_t n_cols) {
@@ -604,11 +604,13 @@ static_assert(kTileRows <= 256, "tile rows are numbered in bytes");
 // k_merge_tile runs 256 threads over one LDS tile: the 12 KB tile, a
 // 128-row table and the 3 KB row map take 18.5 KB, so eight tiles (the wave
 // limit) are in flight per CU; tiles with more rows read their row
-// descriptors from HBM and search them.  The
-// block shape was chosen by timing the 1e7-edge insert at scale 24 (merge
-// launch 1.13 ms; 1.29 ms with a 2048-input range and a 256-row table, 1.40
-// ms with 1024-thread blocks, 1.17 ms with 1024 for both tile bounds, whose
-// longer rows go to the slower big-row path).
+// descriptors from HBM and search them.  The block shape was chosen by
+// timing the 1e7-edge insert at scale 24 (input range / longest tile row:
+// 1536 / 1536 merge 1.046 ms, insert 2.50 ms; 1024 / 2048 1.09 / 2.54;
+// 2048 / 1024 1.015 / 2.66 and 1792 / 1280 1.01 / 2.60, whose longer rows
+// go to the slower big-row path; 2048 / 1536 1.09 / 2.54 and 1536 / 2048
+// 1.08 / 2.53 with a 14 KB tile, seven tiles per CU; 1.40 ms with
+// 1024-thread blocks).
 constexpr int kMergeThreads = 256;
 constexpr int kMergeItems = kTileLds / kMergeThreads;
 
