@@ -261,8 +261,8 @@ static void launch_level(sstg_multi *M, const DegreeBins &bins, uint32_t L,
                        dim3(kT), 0, s, v, (const uint32_t *)bins.list[1],
                        bins.count[1], L, M->level, M->counts);
   if (bins.count[2])
-    hipLaunchKernelGGL(k_mbfs_level<64>,
-                       dim3(ceil_div(bins.count[2] * 64, kT)), dim3(kT), 0, s,
+    hipLaunchKernelGGL(k_mbfs_level<16>,
+                       dim3(ceil_div(bins.count[2] * 16, kT)), dim3(kT), 0, s,
                        v, (const uint32_t *)bins.list[2], bins.count[2], L,
                        M->level, M->counts);
   if (bins.count[3])
