@@ -382,7 +382,10 @@ __device__ __forceinline__ uint64_t wave_lower_bound(const uint64_t *a,
                                                      uint64_t n, uint64_t x) {
   const uint32_t lane = lane_id();
   uint64_t lo = 0, hi = n;  // answer in [lo, hi]
-  while (hi - lo > (uint64_t)kWave) {
+  // the last round probes lo .. lo + 63 and must also see hi itself (as
+  // "past the end"), so it needs hi - lo < 64: with hi - lo == 64 and the
+  // answer at hi no lane would report it
+  while (hi - lo >= (uint64_t)kWave) {
     uint64_t step = (hi - lo + kWave - 1) / kWave;
     uint64_t p = lo + lane * step;
     bool ge = p >= hi || a[p] >= x;

@@ -73,6 +73,32 @@ def test_batch_sizes_around_the_sort_and_scan_paths():
         assert_same_csr(g, og, f"remove k={k}")
 
 
+@pytest.mark.parametrize("rows", [4095, 4096, 4159])
+def test_many_big_rows_bit_exact(rows):
+    """Batches whose rows are all merged by the big-row path (each row of
+    2100 ids gets one id below all of them, so the whole row moves): the
+    flat merge kernels find each block's rows with a 64-way wave search over
+    rows + 1 offsets, which must be exact at every size, including 4097 ..
+    4160 offsets, where the search's last round covers exactly 64 entries."""
+    n, deg = 1 << 16, 2100
+    r = np.repeat(np.arange(rows, dtype=np.int64), deg)
+    k = np.tile(np.arange(deg, dtype=np.int64), rows)
+    c = 1000 + (r * 7919 + k * 31) % 64536  # distinct per row, all >= 1000
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    g.insert_batch(r.astype(np.uint32), c.astype(np.uint32))
+    og.insert_batch(r.astype(np.uint32), c.astype(np.uint32))
+    bs = np.arange(rows, dtype=np.uint32)
+    bd = ((bs + 1) % 1000).astype(np.uint32)  # below every stored id, never bs
+    say("big rows", rows)
+    g.insert_batch(bs, bd)
+    og.insert_batch(bs, bd)
+    assert g.last_update_stats()["big_rows"] == rows
+    assert_same_csr(g, og, f"insert into {rows} big rows")
+    g.remove_batch(bs, bd)
+    og.remove_batch(bs, bd)
+    assert_same_csr(g, og, f"remove from {rows} big rows")
+
+
 def test_remove_batches_match_oracle():
     rng = np.random.default_rng(7)
     n = 5000
