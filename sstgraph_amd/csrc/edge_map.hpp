@@ -622,9 +622,9 @@ __global__ void __launch_bounds__(kT)
 }  // namespace em
 
 // Dense pull over the active vertices against ctx.bm_cur, split by degree
-// bin (1, 8, 16 lanes or a block per vertex; 16 lanes rather than a wave in
-// the third bin took BFS from the hub at rMat scale 24 from 2.71 to 2.49
-// ms); the output frontier is
+// bin (1, 4, 16 lanes or a block per vertex; 16 lanes rather than a wave in
+// the third bin took BFS from the hub at rMat scale 24 from 2.71 to 2.52
+// ms, and 4 rather than 8 in the second to 2.49 ms); the output frontier is
 // ctx.bm_next, after which the two bitmaps swap roles.  The bins' active
 // lists lie back to back, so one compaction of the keep flags carries the
 // survivors of every bin to the next step.
@@ -665,7 +665,7 @@ int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
     SSTG_LAUNCH((em::k_dense_pull<F, 1, 8>), ceil_div(cnt[0], em::kT), em::kT,
                 0, s, v, cur + off[0], cnt[0], f, bm, keep + off[0]);
   if (cnt[1])
-    SSTG_LAUNCH((em::k_dense_pull<F, 8, 8>), ceil_div(cnt[1] * 8, em::kT),
+    SSTG_LAUNCH((em::k_dense_pull<F, 4, 8>), ceil_div(cnt[1] * 4, em::kT),
                 em::kT, 0, s, v, cur + off[1], cnt[1], f, bm, keep + off[1]);
   if (cnt[2])
     SSTG_LAUNCH((em::k_dense_pull<F, 16, 8>), ceil_div(cnt[2] * 16, em::kT),
@@ -744,11 +744,12 @@ int vertex_pull_list(sstg_graph *g, const uint32_t *list, uint64_t count, F f) {
 
 // Vertex pull over the vertices of one BFS level (list: that level's
 // vertices, level[v] == L for each).  Large levels give their low-degree
-// vertices (up to the reductions' 8-lane bound) 8 lanes each and reach the
-// rest through the third (16 lanes) and block degree bins filtered by
+// vertices (up to the reductions' second-bin bound) 4 lanes each and reach
+// the rest through the third (16 lanes) and block degree bins filtered by
 // level, instead of a wave per vertex; small levels take a wave per vertex.
 // 16 lanes rather than a wave in the third bin took BC from the hub at rMat
-// scale 24 from 12.5 to 11.8 ms.
+// scale 24 from 12.5 to 11.8 ms, and 4 rather than 8 lanes for the low-degree
+// vertices to 11.6 ms.
 constexpr uint64_t kLevelSplitMin = 1u << 16;
 
 template <class F>
@@ -762,8 +763,8 @@ int vertex_pull_level(sstg_graph *g, const uint32_t *list, uint64_t count,
   const DevGraph v = g->view();
   const em::UpToDegree low{g->bin_hi[kBinsPull][1]};
   const em::OnLevel on{level, L};
-  SSTG_LAUNCH((em::k_vertex_pull_if<F, 8, em::UpToDegree>),
-              ceil_div(count * 8, em::kT), em::kT, 0, s, v, list, count, f, low);
+  SSTG_LAUNCH((em::k_vertex_pull_if<F, 4, em::UpToDegree>),
+              ceil_div(count * 4, em::kT), em::kT, 0, s, v, list, count, f, low);
   if (bins.count[2])
     SSTG_LAUNCH((em::k_vertex_pull_if<F, 16, em::OnLevel>),
                 ceil_div(bins.count[2] * 16, em::kT), em::kT, 0, s, v,

@@ -1,8 +1,8 @@
 // Autotuner for the per-graph launch settings.
 //
-// The vertex-pull kernels give each vertex 1, 8 or 16 lanes (a wave in TC's
-// orientation) or a whole 256-thread block by its degree, with one set of
-// degree boundaries for the
+// The vertex-pull kernels give each vertex 1 lane, 8 lanes (4 in the BFS
+// steps and BC's levels), 16 lanes (a wave in TC's orientation) or a whole
+// 256-thread block by its degree, with one set of degree boundaries for the
 // reductions (PageRank, BC, the TC orientation, the batched store's BFS
 // levels) and one for single-source BFS (the bottom-up steps), and BFS
 // switches between top-down and bottom-up steps by two ratios.  Where these
