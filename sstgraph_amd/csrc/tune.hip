@@ -1,7 +1,8 @@
 // Autotuner for the per-graph launch settings.
 //
 // The vertex-pull kernels give each vertex 1 lane, 8 lanes (4 in the BFS
-// steps and BC's levels), 16 lanes (a wave in TC's orientation) or a whole
+// steps, BC's levels and the batched store's BFS levels), 16 lanes (a wave
+// in TC's orientation) or a whole
 // 256-thread block by its degree, with one set of degree boundaries for the
 // reductions (PageRank, BC, the TC orientation, the batched store's BFS
 // levels) and one for single-source BFS (the bottom-up steps), and BFS
