@@ -34,24 +34,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+from sstgraph_amd.workloads import (c4_specs, serve_batches, serve_trace,  # noqa: E402
+                                    update_batch)
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def hash64_np(u):
-    """rmat_util.h:29-39 on uint64 numpy arrays (wrapping arithmetic)."""
-    u = u.astype(np.uint64)
-    with np.errstate(over="ignore"):
-        v = u * np.uint64(3935559000370003845) + np.uint64(2691343689449507681)
-        v ^= v >> np.uint64(21)
-        v ^= v << np.uint64(37)
-        v ^= v >> np.uint64(4)
-        v *= np.uint64(4768777513237032717)
-        v ^= v << np.uint64(20)
-        v ^= v >> np.uint64(41)
-        v ^= v << np.uint64(5)
-    return v
 
 
 RMAT_FLAGS = 0  # SSTG_RMAT_NORMALIZED with --rmat-normalized
@@ -75,14 +63,10 @@ def device_symmetric_rmat(S, n, seed, count):
 
 def make_batch(S, n, off, cols, size, seed):
     """80% fresh rMat(n, seed) edges + 20% edges present in the base graph
-    (edge index hash64(seed, j) mod m), as device arrays."""
-    fresh = int(size * 0.8)
-    fs, fd = S.rmat(n, seed, fresh, normalized=RMAT_FLAGS != 0)
-    j = np.arange(size - fresh, dtype=np.uint64) + np.uint64(seed) * np.uint64(1 << 32)
-    pick = (hash64_np(j) % np.uint64(len(cols))).astype(np.int64)
-    rows = (np.searchsorted(off, pick, side="right") - 1).astype(np.uint32)
-    bs = np.concatenate([fs, rows])
-    bd = np.concatenate([fd, cols[pick]])
+    (edge index hash64(seed, j) mod m), as device and host arrays."""
+    bs, bd = update_batch(
+        lambda nn, sd, c: S.rmat(nn, sd, c, normalized=RMAT_FLAGS != 0),
+        n, off, cols, size, seed)
     return S.DeviceArray.from_host(bs), S.DeviceArray.from_host(bd), bs, bd
 
 
@@ -153,22 +137,6 @@ def cpu_algos(off, cols, hub, iters=2):
                 pagerank_ms_per_iter=1e3 * t_pr / iters,
                 pagerank_threads=O.threads(),
                 pagerank_gteps=m * iters / t_pr / 1e9)
-
-
-def c4_specs(k=512, seed=3):
-    """Graph specs of the batched workload: scale = 8 + floor(11 U) in
-    [8, 18]; even graphs rMat with edge factor 8..32, odd graphs Erdos-Renyi
-    with average degree 4..64 (uniforms from hash64 of the seed)."""
-    specs = []
-    for g in range(k):
-        u = hash64_np(np.array([(seed << 32) | (3 * g + j) for j in range(3)],
-                               dtype=np.uint64)).astype(np.float64) / 2.0**64
-        scale = min(18, 8 + int(11 * u[0]))
-        if g % 2 == 0:
-            specs.append(("rmat", scale, 8 + int(25 * u[1]), 1000 + g))
-        else:
-            specs.append(("er", scale, 4 + int(61 * u[2]), 2000 + g))
-    return specs
 
 
 def c4_edges(S, spec):
@@ -363,33 +331,6 @@ def run_batched(args):
     print(json.dumps(out))
 
 
-def serve_trace(n, deg, count, rate, seed=4):
-    """Seeded request trace: Poisson arrivals at `rate` per second; 55% update
-    batches (70% inserts of fresh rMat edges, k log-uniform in [1e2, 1e6],
-    30% removes of an earlier inserted batch), 25% BFS from a random
-    non-isolated vertex, 12% PageRank (10 iterations), 8% TC."""
-    rng = np.random.default_rng(seed)
-    t = np.cumsum(rng.exponential(1.0 / rate, count))
-    nz = np.nonzero(deg)[0]
-    reqs, inserted = [], []
-    for i in range(count):
-        r = rng.random()
-        if r < 0.55:
-            if inserted and rng.random() < 0.3:
-                reqs.append(("remove", inserted[int(rng.integers(len(inserted)))]))
-            else:
-                size = int(10 ** rng.uniform(2, 6))
-                reqs.append(("insert", (size, 100 + i)))
-                inserted.append((size, 100 + i))
-        elif r < 0.80:
-            reqs.append(("bfs", int(nz[int(rng.integers(len(nz)))])))
-        elif r < 0.92:
-            reqs.append(("pr", 10))
-        else:
-            reqs.append(("tc", None))
-    return t, reqs
-
-
 def serve_once(S, g, t_arr, reqs, batches, slots, coalesce, snapshots,
                one_heavy=False):
     """Submits the trace (arrival offsets t_arr, or all at once when None)
@@ -476,11 +417,8 @@ def run_serve(args):
     _, reqs = serve_trace(n, deg, args.requests, 1.0)
     # the served graph stays undirected: an update batch of k rMat edges is
     # applied as its 2k directed pairs
-    batches = {}
-    for kind, arg in reqs:
-        if kind == "insert":
-            bs, bd = S.rmat(n, arg[1], arg[0], normalized=RMAT_FLAGS != 0)
-            batches[arg] = (np.concatenate([bs, bd]), np.concatenate([bd, bs]))
+    batches = serve_batches(
+        lambda nn, sd, c: S.rmat(nn, sd, c, normalized=RMAT_FLAGS != 0), n, reqs)
     log(f"serve: scale {scale}, m={m0}, {len(reqs)} requests")
     res, rate, t_arr = {}, None, None
     for name, mode in SERVE_MODES:

@@ -280,7 +280,8 @@ int sstg_server_destroy(sstg_server *s);
 int sstg_degree_order_map(sstg_graph *g, uint32_t *map, uint32_t flags);
 /* A new graph holding (map[i], map[j]) for every stored edge (i, j), self-
  * loops kept (SparseMatrix.hpp:751-766, rewrite_graph).  SSTG_E_RANGE if a
- * label is out of range.  Edge weights are not carried (the reference's
+ * label is out of range; SSTG_E_INVAL unless n_rows == n_cols (one map
+ * relabels both).  Edge weights are not carried (the reference's
  * rewrite re-inserts two-element tuples). */
 int sstg_rewrite_graph(sstg_graph *g, const uint32_t *map, uint32_t flags,
                        sstg_graph **out);

@@ -256,6 +256,15 @@ def _ptr(a):
 
 
 def _u32(a):
+    """Vertex ids as a contiguous uint32 array.  Ids of another integer type
+    must fit in [0, 2^32): a negative or wider id would otherwise wrap onto
+    a valid vertex."""
+    a = np.asarray(a)
+    if a.dtype != np.uint32 and a.size:
+        if not np.issubdtype(a.dtype, np.integer):
+            raise TypeError(f"vertex ids must be integers, not {a.dtype}")
+        if int(a.min()) < 0 or int(a.max()) >= 1 << 32:
+            raise IdOutOfRange(SSTG_E_RANGE, "vertex id outside [0, 2^32)")
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
@@ -638,7 +647,14 @@ class SparseMatrixV:
         if dst is None:
             e = np.asarray(src)
             src, dst, weights = e[:, 0], e[:, 1], e[:, 2]
-        s, d, w = _u32(src), _u32(dst), _u32(weights)
+        s, d = _u32(src), _u32(dst)
+        w = np.asarray(weights)
+        if w.dtype != np.uint32 and w.size and (
+                int(w.min()) < -(1 << 31) or int(w.max()) >= 1 << 32):
+            raise ValueError("weights must fit in 32 bits")
+        # negative weights keep their two's complement bits (read as int32)
+        w = np.ascontiguousarray(w.astype(np.int64) & 0xFFFFFFFF, dtype=np.uint32) \
+            if w.dtype != np.uint32 else np.ascontiguousarray(w)
         if s.shape != d.shape or s.shape != w.shape:
             raise ValueError("src, dst and weights differ in length")
         _check(lib().sstg_insert_batch_weighted(self._h, _ptr(s), _ptr(d), _ptr(w),

@@ -464,6 +464,8 @@ int sstg_rewrite_graph(sstg_graph *g, const uint32_t *map, uint32_t flags,
                        sstg_graph **out) {
   if (!g || !map || !out) return SSTG_E_INVAL;
   *out = nullptr;
+  // map relabels rows and columns alike, so it has n_rows == n_cols entries
+  if (g->n_rows != g->n_cols) return SSTG_E_INVAL;
   hipSetDevice(g->device);
   const uint32_t *dmap = map;
   if (!(flags & SSTG_DEVICE_PTRS)) {

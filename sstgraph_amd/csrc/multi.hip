@@ -248,8 +248,9 @@ int record_pagerank(sstg_multi *M, const DegreeBins &bins, uint32_t iters,
 }  // namespace
 
 // One BFS level over all graphs, by degree bin: 1, 4 and 16 lanes per vertex
-// and a block (4 and 16 rather than 8 and 64 took the batched pass from 66.7
-// to 59 ms).
+// and a block (16 rather than 64 lanes in the third bin took the batched
+// pass from 66.7 to 62.6 ms, 4 rather than 8 in the second from 62.2 to
+// 60.3 ms).
 static void launch_level(sstg_multi *M, const DegreeBins &bins, uint32_t L,
                          hipStream_t s) {
   const DevGraph v = M->u->view();

@@ -399,9 +399,12 @@ void slot_main(sstg_server *S, Slot *sl) {
       r = sl->req;
     }
     int rc = run_query(S, &sl->view, r);
-    if (rc == SSTG_OK) {
+    // drain the slot's stream whatever the query returned: a query that
+    // failed part-way may have kernels in flight on the snapshot and on
+    // r->result, which must not be recycled before they finish
+    {
       hipError_t e = hipStreamSynchronize(sl->view.stream);
-      if (e != hipSuccess) rc = SSTG_E_DEVICE;
+      if (e != hipSuccess && rc == SSTG_OK) rc = SSTG_E_DEVICE;
     }
     {
       std::lock_guard<std::mutex> lk(S->mu);

@@ -81,6 +81,18 @@ __global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n,
   keys[i] = ((uint64_t)s << 32) | d;
 }
 
+// ctr[kCtrErr] = 1 if any pair of the batch is out of range (the whole-batch
+// check of a batch applied in pieces)
+__global__ void k_check_range(const uint32_t *src, const uint32_t *dst,
+                              uint64_t n, uint32_t n_rows, uint32_t n_cols,
+                              uint64_t *ctr) {
+  const uint64_t stride = (uint64_t)gridDim.x * kT;
+  bool bad = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x; i < n; i += stride)
+    bad |= (src[i] >= n_rows) | (dst[i] >= n_cols);
+  if (bad) ctr[kCtrErr] = 1;
+}
+
 // pos[i] = the row position where key i sits or would go (its lower bound
 // in the old row); the segment step takes each row's first moved position
 // from the row's first kept key.
@@ -1194,8 +1206,27 @@ int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   g->last.batch = n;
   if (n == 0) return SSTG_OK;
   // batches above 2^31 pairs are applied in pieces; union and difference are
-  // unaffected by the split
+  // unaffected by the split.  Every id is checked before the first piece, so
+  // an out-of-range id leaves the graph unchanged however large the batch.
   const uint64_t kMaxChunk = 1ull << 31;
+  if (n > kMaxChunk) {
+    bool bad = false;
+    if (device_ptrs) {
+      SSTG_HIP(hipMemsetAsync(g->dev_ctr + kCtrErr, 0, sizeof(uint64_t),
+                              g->stream));
+      SSTG_LAUNCH(k_check_range, 4096, kT, 0, g->stream, src, dst, n,
+                  g->n_rows, g->n_cols, g->dev_ctr);
+      SSTG_HIP(hipMemcpyAsync(g->host_rb, g->dev_ctr + kCtrErr,
+                              sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              g->stream));
+      SSTG_HIP(hipStreamSynchronize(g->stream));
+      bad = g->host_rb[0] != 0;
+    } else {
+      for (uint64_t i = 0; i < n && !bad; i++)
+        bad = src[i] >= g->n_rows || dst[i] >= g->n_cols;
+    }
+    if (bad) return SSTG_E_RANGE;
+  }
   for (uint64_t base = 0; base < n; base += kMaxChunk) {
     uint64_t len = n - base < kMaxChunk ? n - base : kMaxChunk;
     const uint32_t *s = src + base, *d = dst + base;

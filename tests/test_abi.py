@@ -75,3 +75,18 @@ def test_cpp_facade_builds_and_skips_without_gpu():
     r = subprocess.run([out], capture_output=True, text=True, timeout=60)
     if S.device_count() == 0:
         assert r.returncode == 77 and "skipped" in r.stdout
+
+
+def test_ids_outside_u32_are_refused_before_any_call():
+    """Ids of a wider or signed integer type must fit in [0, 2^32): they are
+    refused rather than wrapped onto a valid vertex (no GPU needed)."""
+    import numpy as np
+    with pytest.raises(S.IdOutOfRange):
+        S._u32(np.array([1, -1], dtype=np.int64))
+    with pytest.raises(S.IdOutOfRange):
+        S._u32(np.array([1 << 32], dtype=np.int64))
+    with pytest.raises(TypeError):
+        S._u32(np.array([1.5]))
+    ok = S._u32(np.array([0, (1 << 32) - 1], dtype=np.int64))
+    assert ok.dtype == np.uint32 and ok.tolist() == [0, (1 << 32) - 1]
+    assert S._u32(np.zeros(0, dtype=np.int64)).size == 0

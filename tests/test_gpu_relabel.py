@@ -86,6 +86,16 @@ def test_rewrite_graph_refuses_out_of_range_labels():
         g.rewrite_graph(mp)
 
 
+def test_rewrite_graph_refuses_rectangular_graphs():
+    """One map relabels rows and columns, so a graph with more columns than
+    rows is refused instead of reading past the map."""
+    g = S.SparseMatrixV(4, 100)
+    g.insert_batch(np.array([1, 2], np.uint32), np.array([90, 3], np.uint32))
+    with pytest.raises(S.SSTGError) as e:
+        g.rewrite_graph(np.arange(4, dtype=np.uint32))
+    assert e.value.code == S.SSTG_E_INVAL
+
+
 def test_statistics_match_numpy():
     n = 3000
     off, cols = random_graph(n, 20000, 9)
