@@ -76,38 +76,43 @@ def timed(fn):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(off, cols, n, bs, bd, budget_s=30.0):
+def cpu_baseline(off, cols, n, batches, headline, budget_s=24.0):
     """The oracle's batch insert (per-row sorted merge, OpenMP) timed on
-    this host on the same base graph and the same batch."""
+    this host on the same base graph and the same batches, each batch
+    inserted and then removed again as in the reference's update loop
+    (test.hpp:668-677): on all host threads and on one, at every batch size
+    (median of up to 5 runs, within a time budget per size)."""
     from oracle import oracle as O  # CPU baseline leg only
     if not os.path.exists(O.LIB_PATH):
         O.build()
     threads = O.threads()
     og = O.Graph(n)
     og.load_csr(off, cols)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 5 and time.perf_counter() - t_start < budget_s:
-        ti = timed(lambda: og.insert_batch(bs, bd))
-        og.remove_batch(bs, bd)
-        times.append(ti)
-    med = float(np.median(times))
-    # the same on one core
-    O.set_threads(1)
-    t1 = []
-    t_start = time.perf_counter()
-    while len(t1) < 3 and time.perf_counter() - t_start < budget_s / 2:
-        t1.append(timed(lambda: og.insert_batch(bs, bd)))
-        og.remove_batch(bs, bd)
+    per = {}
+    for size, (bs, bd) in batches.items():
+        rec = {}
+        for t in (threads, 1):
+            O.set_threads(t)
+            times = []
+            t_start = time.perf_counter()
+            lim = budget_s / len(batches) / (2 if t == 1 else 1)
+            while len(times) < 5 and (len(times) < 1 or time.perf_counter() - t_start < lim):
+                times.append(timed(lambda: og.insert_batch(bs, bd)))
+                og.remove_batch(bs, bd)
+            med = float(np.median(times))
+            rec["all_cores" if t == threads else "one_core"] = dict(
+                inserts_per_s=size / med, insert_ms=1e3 * med, runs=len(times))
+        per[str(size)] = rec
     O.set_threads(threads)
-    med1 = float(np.median(t1))
-    return dict(value=len(bs) / med, unit="edges/s", cores=threads,
+    h = per[str(headline)]
+    return dict(value=h["all_cores"]["inserts_per_s"], unit="edges/s", cores=threads,
                 kind="port",
-                sample=f"oracle insert_batch of one {len(bs)}-edge batch into "
-                       f"the scale-24 base graph, median of {len(times)} runs",
-                single_core=dict(value=len(bs) / med1, unit="edges/s", cores=1,
-                                 runs=len(t1)),
-                host=host_cpu())
+                sample=f"oracle insert_batch of the {headline}-edge batch into the "
+                       f"scale-24 base graph, median of {h['all_cores']['runs']} runs; "
+                       f"every batch size and one core under per_batch",
+                single_core=dict(value=h["one_core"]["inserts_per_s"], unit="edges/s",
+                                 cores=1, runs=h["one_core"]["runs"]),
+                per_batch=per, host=host_cpu())
 
 
 def host_cpu():
@@ -125,18 +130,37 @@ def host_cpu():
 
 
 def cpu_algos(off, cols, hub, iters=2):
-    """The oracle's BFS (one thread, queue order) from the hub and FP64
-    PageRank iterations (OpenMP) on the same base graph, for the BFS/PR
-    figures of the line."""
+    """The oracle's BFS and BC from the hub and FP64 PageRank iterations on
+    the same base graph, on all host threads (OpenMP, level-synchronous BFS
+    and Brandes) and on one thread."""
     from oracle import oracle as O  # CPU baseline leg only
-    t_bfs = timed(lambda: O.bfs(off, cols, hub))
-    t_pr = timed(lambda: O.pagerank(off, cols, iters))
+    threads = O.threads()
     m = int(off[-1])
-    return dict(bfs_hub_ms=1e3 * t_bfs, bfs_threads=1,
-                bfs_gteps=m / t_bfs / 1e9,
-                pagerank_ms_per_iter=1e3 * t_pr / iters,
-                pagerank_threads=O.threads(),
-                pagerank_gteps=m * iters / t_pr / 1e9)
+    out = {"threads": threads}
+    for t in (threads, 1):
+        O.set_threads(t)
+        key = "all_cores" if t == threads else "one_core"
+        t_bfs = timed(lambda: O.bfs_par(off, cols, hub))
+        t_bc = timed(lambda: O.bc_par(off, cols, hub))
+        t_pr = timed(lambda: O.pagerank(off, cols, iters))
+        out[key] = dict(cores=t, bfs_hub_ms=1e3 * t_bfs, bfs_gteps=m / t_bfs / 1e9,
+                        bc_hub_ms=1e3 * t_bc, pagerank_ms_per_iter=1e3 * t_pr / iters,
+                        pagerank_gteps=m * iters / t_pr / 1e9)
+    O.set_threads(threads)
+    return out
+
+
+def op_model_bytes(n, deg, hs, hd, skip_self=True):
+    """Row-merge model of one insert batch: 8 B per batch pair + 4 B per old
+    and per new element of every touched row + 16 B per touched row."""
+    keys = np.unique(hs.astype(np.int64) * n + hd.astype(np.int64))
+    ks = keys // n
+    if skip_self:
+        ks = ks[ks != keys % n]
+    trow, tcount = np.unique(ks, return_counts=True)
+    d_old = int(deg[trow].sum())
+    # new rows hold at most the old ones plus the batch's distinct ids
+    return 8 * len(hs) + 4 * (2 * d_old + int(tcount.sum())) + 16 * len(trow)
 
 
 def c4_edges(S, spec):
@@ -332,11 +356,11 @@ def run_batched(args):
 
 
 def serve_once(S, g, t_arr, reqs, batches, slots, coalesce, snapshots,
-               one_heavy=False):
+               one_heavy=False, maintain=False):
     """Submits the trace (arrival offsets t_arr, or all at once when None)
     and returns latency and throughput figures."""
     srv = S.Server(g, slots=slots, coalesce=coalesce, snapshots=snapshots,
-                   one_heavy=one_heavy)
+                   one_heavy=one_heavy, maintain=maintain)
     tickets = []
     t0 = time.perf_counter()
     for k, (kind, arg) in enumerate(reqs):
@@ -384,8 +408,57 @@ SERVE_MODES = (
     ("serialised", dict(slots=1, coalesce=False, snapshots=False)),
     ("concurrent", dict(slots=4, coalesce=True, snapshots=False)),
     ("snapshots", dict(slots=4, coalesce=True, snapshots=True)),
-    ("served", dict(slots=4, coalesce=True, snapshots=True, one_heavy=True)),
+    ("served", dict(slots=4, coalesce=True, snapshots=True, one_heavy=True,
+                    maintain=True)),
 )
+
+
+def serve_compare(S, scale, edge_factor, requests, load, modes):
+    """The served-stream comparison on rMat scale `scale` (seed 4,
+    symmetrised): each mode first takes the whole seeded trace at once
+    (capacity, requests/s), then the trace again with Poisson arrivals at
+    `load` times the first mode's capacity (latency from arrival to result,
+    p50/p99).  Each mode starts from a freshly built graph.  Returns
+    (results per mode, Poisson rate, directed edges, requests)."""
+    n = 1 << scale
+    count = edge_factor * n
+
+    def fresh_graph():
+        src, dst = device_symmetric_rmat(S, n, 4, count)
+        g = S.SparseMatrixV(n)
+        g.insert_batch(src, dst)
+        src.free()
+        dst.free()
+        return g
+
+    g = fresh_graph()
+    m0 = g.M()
+    deg = g.degrees()
+    _, reqs = serve_trace(n, deg, requests, 1.0)
+    # the served graph stays undirected: an update batch of k rMat edges is
+    # applied as its 2k directed pairs
+    batches = serve_batches(
+        lambda nn, sd, c: S.rmat(nn, sd, c, normalized=RMAT_FLAGS != 0), n, reqs)
+    log(f"serve: scale {scale}, m={m0}, {len(reqs)} requests")
+    res, rate, t_arr = {}, None, None
+    for name, mode in modes:
+        if g is None:
+            g = fresh_graph()
+        burst = serve_once(S, g, None, reqs, batches, **mode)
+        if rate is None:
+            rate = load * burst["throughput_rps"]
+            t_arr, _ = serve_trace(n, deg, requests, rate)
+        g.close()
+        g = fresh_graph()
+        paced = serve_once(S, g, t_arr, reqs, batches, **mode)
+        res[name] = dict(burst=burst, poisson=paced, **mode)
+        log(f"{name}: capacity {burst['throughput_rps']:.1f} req/s; at {rate:.1f} req/s "
+            f"p50 {paced['p50_ms']:.1f} ms p99 {paced['p99_ms']:.1f} ms; "
+            f"{paced['device_update_batches']} update batches, "
+            f"{paced['store_snapshots']} snapshots")
+        g.close()
+        g = None
+    return res, rate, m0, reqs
 
 
 def run_serve(args):
@@ -400,42 +473,8 @@ def run_serve(args):
     capacity (latency from arrival to result, p50/p99)."""
     import sstgraph_amd as S
     scale = args.serve_scale
-    n = 1 << scale
-    count = args.edge_factor * n
-
-    def fresh_graph():
-        src, dst = device_symmetric_rmat(S, n, 4, count)
-        g = S.SparseMatrixV(n)
-        g.insert_batch(src, dst)
-        src.free()
-        dst.free()
-        return g
-
-    g = fresh_graph()
-    m0 = g.M()
-    deg = g.degrees()
-    _, reqs = serve_trace(n, deg, args.requests, 1.0)
-    # the served graph stays undirected: an update batch of k rMat edges is
-    # applied as its 2k directed pairs
-    batches = serve_batches(
-        lambda nn, sd, c: S.rmat(nn, sd, c, normalized=RMAT_FLAGS != 0), n, reqs)
-    log(f"serve: scale {scale}, m={m0}, {len(reqs)} requests")
-    res, rate, t_arr = {}, None, None
-    for name, mode in SERVE_MODES:
-        if g is None:
-            g = fresh_graph()
-        burst = serve_once(S, g, None, reqs, batches, **mode)
-        if rate is None:
-            rate = args.load * burst["throughput_rps"]
-            t_arr, _ = serve_trace(n, deg, args.requests, rate)
-        paced = serve_once(S, g, t_arr, reqs, batches, **mode)
-        res[name] = dict(burst=burst, poisson=paced, **mode)
-        log(f"{name}: capacity {burst['throughput_rps']:.1f} req/s; at {rate:.1f} req/s "
-            f"p50 {paced['p50_ms']:.1f} ms p99 {paced['p99_ms']:.1f} ms; "
-            f"{paced['device_update_batches']} update batches, "
-            f"{paced['store_snapshots']} snapshots")
-        g.close()
-        g = None
+    res, rate, m0, reqs = serve_compare(S, scale, args.edge_factor, args.requests,
+                                        args.load, SERVE_MODES)
     sv, se = res["served"], res["serialised"]
     out = {
         "metric": "served stream p99 latency (updates + BFS + PR + TC)",
@@ -472,11 +511,18 @@ def run_algo(args):
     dst.free()
     m = g.M()
     deg = g.degrees().astype(np.int64)
+    extra = {}
     if args.workload == "pagerank":
         out = S.DeviceArray(n, np.float64)
+        if args.pr_segments is not None:
+            g.set_pagerank_segments(args.pr_segments)
         step = lambda: EMV.PR_S(g, args.pr_iters, out=out)
         work = m * args.pr_iters
         name = f"PageRank FP64 x{args.pr_iters}"
+        # the first call on a graph version builds the column-segment layout
+        S.synchronize()
+        extra["first_call_ms"] = 1e3 * timed(lambda: (step(), S.synchronize()))
+        extra["segment_bits"] = g.pagerank_segments()
     else:
         hub = int(np.argmax(deg))
         d_par = S.DeviceArray(n, np.int32)
@@ -503,7 +549,8 @@ def run_algo(args):
         "data": "synthetic (rMat generator of the reference, seeded; " + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
         "config": {"workload": f"{name} on rMat scale {args.scale} (seed 2, edge "
                                f"factor {args.edge_factor}, symmetrised)",
-                   "directed_edges": int(m)}}))
+                   "directed_edges": int(m)},
+        "extra": extra}))
 
 
 def run_c2(args):
@@ -614,6 +661,12 @@ def main():
     ap.add_argument("--batch", type=float, default=1e7)
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served-stream comparison of the default run")
+    ap.add_argument("--grow-batches", type=int, default=60,
+                    help="fresh 1e6-edge inserts of the growing sequence")
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="requests of the default run's served-stream comparison")
     ap.add_argument("--no-algos", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the degree-threshold autotuner")
@@ -623,6 +676,9 @@ def main():
     ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs",
                                            "algos"],
                     default="updates")
+    ap.add_argument("--pr-segments", type=int, default=None,
+                    help="PageRank column-segment width in log2 vertices "
+                         "(0: vertex pull; default: the library's)")
     ap.add_argument("--graphs", type=int, default=512)
     ap.add_argument("--serve-scale", type=int, default=25)
     ap.add_argument("--requests", type=int, default=200)
@@ -735,9 +791,11 @@ def main():
     # ------------------------------------------------------- update batches
     sizes = [1000, 10000, 100000, 1000000, B]
     per_size = {}
+    host_batches = {}
     headline = None
     for k, size in enumerate(sizes):
         ds, dd, hs, hd = make_batch(S, n, off, cols, size, 100 + k)
+        host_batches[size] = (hs, hd)
         steps = args.steps if size == B else max(3, min(args.steps, 20))
         for _ in range(args.warmup):
             g.insert_batch(ds, dd)
@@ -752,24 +810,48 @@ def main():
         # that were present before (test.hpp:668-677), so after the first
         # step the edge count is stable at m minus those edges
         m_after = g.M()
+        # the row-merge model of this insert on the graph as each timed step
+        # found it: 8 B per batch pair, 4 B per old and new element of every
+        # touched row, 16 B per touched row
+        uk = np.unique(hs.astype(np.int64) * n + hd.astype(np.int64))
+        ur, uc = (uk // n), (uk % n)
+        keep = ur != uc
+        ur, uc = ur[keep], uc[keep]
+        present = g.has_batch(ur.astype(np.uint32), uc.astype(np.uint32)).astype(bool)
+        deg_now = g.degrees().astype(np.int64)
+        trow, tcount = np.unique(ur, return_counts=True)
+        _, tnew = np.unique(ur[~present], return_counts=True) if (~present).any() \
+            else (None, np.zeros(0, np.int64))
+        d_old = int(deg_now[trow].sum())
+        d_new = d_old + int(tnew.sum())
+        op_bytes = 8 * size + 4 * (d_old + d_new) + 16 * len(trow)
         g.insert_batch(ds, dd)
         new_edges = g.last_update_stats()["effective"]
         g.remove_batch(ds, dd)
+        assert new_edges == d_new - d_old, "model and store disagree on new edges"
         assert g.M() == m_after, "update steps must leave the graph stable"
-        rec = dict(steps=steps, insert_ms=1e3 * float(np.mean(ti)),
+        t_ins = float(np.mean(ti))
+        rec = dict(steps=steps, insert_ms=1e3 * t_ins,
                    remove_ms=1e3 * float(np.mean(tr)),
-                   inserts_per_s=size / float(np.mean(ti)),
+                   insert_p99_ms=1e3 * float(np.percentile(ti, 99)),
+                   insert_max_ms=1e3 * float(np.max(ti)),
+                   inserts_per_s=size / t_ins,
                    removes_per_s=size / float(np.mean(tr)),
                    # edges each timed insert actually added (distinct, not
                    # self-loops, not already present)
                    unique_new_edges=new_edges,
-                   unique_new_per_s=new_edges / float(np.mean(ti)))
+                   unique_new_per_s=new_edges / t_ins,
+                   touched_rows=int(len(trow)),
+                   model_bytes=op_bytes,
+                   model_gbs=op_bytes / t_ins / 1e9,
+                   model_frac=op_bytes / t_ins / 1e9 / HBM_PEAK_GBS)
         rec["m_after"] = m_after
         per_size[str(size)] = rec
         log(f"batch {size}: insert {rec['insert_ms']:.3f} ms "
-            f"({rec['inserts_per_s']:.3e}/s), remove {rec['remove_ms']:.3f} ms")
+            f"({rec['inserts_per_s']:.3e}/s, {rec['model_gbs']:.0f} GB/s model), "
+            f"remove {rec['remove_ms']:.3f} ms")
         if size == B:
-            headline = (ds, dd, hs, hd, ti, tr)
+            headline = (ds, dd, hs, hd, ti, tr, op_bytes)
         else:
             ds.free()
             dd.free()
@@ -778,7 +860,7 @@ def main():
     # ------------------------------- kernel timings of the headline batch
     # HIP events around each kernel launch on the graph's stream; the
     # insert launches are profiled on their own.
-    ds, dd, hs, hd, ti, tr = headline
+    ds, dd, hs, hd, ti, tr, op_bytes = headline
     S.Profile.reset()
     prof_steps = max(3, min(args.steps, 5))
     stats = None
@@ -795,47 +877,99 @@ def main():
     extra["insert_stats"] = stats
     dom_name, dom_n, dom_ms = kernels[0]
     ms_per = dom_ms / max(dom_n, 1)
-    # algorithmic bytes of the dominant kernel per launch
-    if "k_merge_tile" in dom_name:
-        # read each tile row's moved suffix (or whole old row) and its batch
-        # run, write the merged row: 4 B in + 4 B out per merge input
-        alg_bytes = 8 * stats["tile_inputs"]
-    elif "k_merge_insert" in dom_name:
-        alg_bytes = 8 * stats["merged"] + 4 * stats["effective"]
-    else:
-        alg_bytes = 16 * B  # radix pass: read + write 8-byte keys
-    achieved = alg_bytes / (ms_per * 1e-3) / 1e9 if ms_per > 0 else 0.0
-    traffic = None
+    # algorithmic bytes: the row-merge model of the whole insert of the batch
+    # (8B + 4 sum(d_old + d_new) + 16|T|), attributed to its dominant
+    # kernel, which carries the merge of nearly every touched row; the
+    # tile-input count (4 B in + 4 B out per element the LDS tiles merge)
+    # beside it
+    achieved = op_bytes / (ms_per * 1e-3) / 1e9 if ms_per > 0 else 0.0
+    tile_bytes = 8 * stats["tile_inputs"]
+    traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "profiles", "k_merge_tile_scale24_batch1e7.traffic.json")
     if os.path.exists(tfile) and args.scale == 24 and B == 10_000_000:
         traffic = json.load(open(tfile)).get("bytes_per_launch")
+        traffic_src = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command, "
+                       "profiles/k_merge_tile_scale24_batch1e7.traffic.json")
     roofline = dict(kernel=dom_name, bound="hbm", achieved=achieved,
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=traffic, algorithmic_bytes=alg_bytes,
-                    avg_launch_ms=ms_per)
-    # whole-operation row-merge model (8 B per batch pair + 4 B per old and
-    # new element of each touched row + 16 B per touched row)
-    deg = np.diff(off).astype(np.int64)
-    keys = np.unique(hs.astype(np.int64) * n + hd.astype(np.int64))
-    ks = keys // n
-    keep = ks != keys % n
-    trow, tcount = np.unique(ks[keep], return_counts=True)
-    d_old = int(deg[trow].sum())
-    op_bytes = 8 * B + 4 * (2 * d_old + int(tcount.sum())) + 16 * len(trow)
+                    traffic=traffic, traffic_source=traffic_src,
+                    algorithmic_bytes=op_bytes,
+                    algorithmic_model="8B + 4 sum_T(d_old + d_new) + 16|T| of the "
+                                      "whole insert, attributed to its dominant kernel",
+                    avg_launch_ms=ms_per,
+                    tile_input_bytes=tile_bytes,
+                    tile_input_gbs=tile_bytes / (ms_per * 1e-3) / 1e9 if ms_per > 0 else 0.0,
+                    whole_insert_ms=1e3 * float(np.mean(ti)),
+                    whole_insert_frac=op_bytes / float(np.mean(ti)) / 1e9 / HBM_PEAK_GBS)
     extra["insert_op_model_bytes"] = op_bytes
     extra["insert_op_model_gbs"] = op_bytes / float(np.mean(ti)) / 1e9
+
+    # ------------------------- a growing sequence that crosses rebuilds
+    # fresh 1e6-edge batches inserted one after another (no removes): rows
+    # outgrow their slots and move to the pool tail, and the pool is
+    # compacted or grown inside whichever insert crosses the threshold; the
+    # per-batch latency distribution shows what those inserts cost
+    ds.free()
+    dd.free()
+    seq_ms, seq_rebuild = [], []
+    r0 = g.last_update_stats()["rebuilds"]
+    for i in range(args.grow_batches):
+        gs, gd = S.rmat(n, 5000 + i, 1_000_000, device=True,
+                        normalized=RMAT_FLAGS != 0)
+        S.synchronize()
+        seq_ms.append(1e3 * timed(lambda: g.insert_batch(gs, gd)))
+        seq_rebuild.append(g.last_update_stats()["rebuilds"])
+        gs.free()
+        gd.free()
+    if seq_ms:
+        sq = np.array(seq_ms)
+        crossed = [i for i in range(len(seq_rebuild))
+                   if seq_rebuild[i] != (seq_rebuild[i - 1] if i else r0)]
+        extra["growing_sequence"] = dict(
+            batches=len(sq), batch_edges=1_000_000,
+            p50_ms=float(np.percentile(sq, 50)), p99_ms=float(np.percentile(sq, 99)),
+            max_ms=float(sq.max()), rebuilds=int(seq_rebuild[-1] - r0),
+            rebuild_batches_ms=[float(sq[i]) for i in crossed],
+            m_end=g.M())
+        log(f"growing sequence: {len(sq)} x 1e6 inserts, p50 {np.percentile(sq, 50):.3f} ms "
+            f"p99 {np.percentile(sq, 99):.3f} ms max {sq.max():.3f} ms, "
+            f"{seq_rebuild[-1] - r0} rebuilds")
+
+    # ------------------------------------------------ served stream (C5)
+    if not args.no_serve:
+        g.close()
+        res, rate, m5, reqs5 = serve_compare(
+            S, args.serve_scale, args.edge_factor, args.serve_requests, args.load,
+            [m for m in SERVE_MODES if m[0] in ("serialised", "served")])
+        sv, se = res["served"], res["serialised"]
+        extra["served"] = dict(
+            workload=f"rMat scale {args.serve_scale} (seed 4, symmetrised, "
+                     f"{m5} directed edges), {len(reqs5)} seeded requests "
+                     f"(55% updates of 1e2..1e6 edges, 25% BFS, 12% PR x10, 8% TC), "
+                     f"Poisson at {rate:.1f}/s ({args.load:.0%} of the serialised capacity)",
+            served_p50_ms=sv["poisson"]["p50_ms"], served_p99_ms=sv["poisson"]["p99_ms"],
+            serialised_p50_ms=se["poisson"]["p50_ms"],
+            serialised_p99_ms=se["poisson"]["p99_ms"],
+            served_capacity_rps=sv["burst"]["throughput_rps"],
+            serialised_capacity_rps=se["burst"]["throughput_rps"],
+            capacity_ratio=sv["burst"]["throughput_rps"] / se["burst"]["throughput_rps"],
+            per_kind_served=sv["poisson"]["per_kind"],
+            per_kind_serialised=se["poisson"]["per_kind"])
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(off, cols, n, hs, hd)
+        cpu = cpu_baseline(off, cols, n, host_batches, B)
         log(f"cpu baseline {cpu['value']:.3e} edges/s on {cpu['cores']} threads, "
             f"{cpu['single_core']['value']:.3e} on one")
         if not args.no_algos:
             hub = int(np.argmax(np.diff(off)))
             extra["cpu_algos"] = cpu_algos(off, cols, hub)
-            log(f"cpu BFS from hub {extra['cpu_algos']['bfs_hub_ms']:.0f} ms, PR "
-                f"{extra['cpu_algos']['pagerank_ms_per_iter']:.0f} ms/iter")
+            ca = extra["cpu_algos"]
+            log(f"cpu BFS from hub {ca['all_cores']['bfs_hub_ms']:.0f} ms "
+                f"({ca['one_core']['bfs_hub_ms']:.0f} on one), BC "
+                f"{ca['all_cores']['bc_hub_ms']:.0f} ms, PR "
+                f"{ca['all_cores']['pagerank_ms_per_iter']:.0f} ms/iter")
 
     ins = float(np.mean(ti))
     out = {

@@ -143,7 +143,11 @@ int sstg_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
 int sstg_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level,
              uint32_t flags);
 /* PR_S<double>(g, iters), TST:527: x0 = 1/n, x' = damping * sum over
- * neighbours u of x(u)/deg(u) + (1 - damping)/n, FP64. */
+ * neighbours u of x(u)/deg(u) + (1 - damping)/n, FP64.  The neighbours of
+ * v are its row N+(v) (out-neighbours) and deg is the out-degree: on a
+ * directed graph this is the transpose of Ligra's PageRank, which sums over
+ * in-neighbours; on a graph whose edges all have their reverses the two
+ * agree.  For the in-neighbour form pass the transposed graph. */
 int sstg_pagerank_f64(sstg_graph *g, uint32_t iters, double damping,
                       double *out, uint32_t flags);
 /* The same PageRank with FP32 ranks and Kahan-compensated neighbour sums. */
@@ -234,6 +238,12 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
 #define SSTG_SERVE_SNAPSHOTS 2u
 /* at most one PageRank or TC at a time; lighter queries pass a waiting one */
 #define SSTG_SERVE_ONE_HEAVY 4u
+/* the triangle count is kept through updates while every edge has its
+ * reverse: the first TC query counts in full, each later update run moves
+ * the count by the triangles its changed edges close (insert) or break
+ * (remove), and a TC query on a version whose count is known is answered
+ * from it without a device pass.  Results are those of a full count. */
+#define SSTG_SERVE_MAINTAIN 8u
 #define SSTG_REQ_INSERT 1u /* insert_batch (SM:556-643) */
 #define SSTG_REQ_REMOVE 2u /* remove_batch (SM:645-718) */
 #define SSTG_REQ_BFS 3u    /* BFS levels from `source` (TST:486) */
@@ -322,6 +332,22 @@ int sstg_get_bfs_degree_thresholds(sstg_graph *g, uint32_t t[3]);
  * than n / beta vertices (defaults 15 and 18; both > 0).  Per graph. */
 int sstg_set_bfs_switch(sstg_graph *g, double alpha, double beta);
 int sstg_get_bfs_switch(sstg_graph *g, double *alpha, double *beta);
+/* Column segments of FP64 PageRank (no reference counterpart; the pull
+ * itself is PR_S<double>, TST:527).  With bits in [8, 30] (default 0: off),
+ * sstg_pagerank_f64 on a graph of more than 2^bits vertices cuts the
+ * column ids into segments of 2^bits vertices and each row into one run per
+ * segment it touches, sweeps the runs segment by segment (so the gathered
+ * contributions of a sweep fit one XCD's L2) and then sums each vertex's
+ * runs in segment order.  The layout is built on the first call after a
+ * change of the graph and kept until the next change (about 4 bytes per
+ * edge plus 16 bytes per run of device memory).  0 turns segments off (the
+ * degree-binned vertex pull).  Results agree with the vertex pull within
+ * FP64 rounding of the different summation order.  On rMat graphs the rows
+ * split into about six runs each at 2^18-vertex segments and the per-run
+ * traffic outweighs the cached gathers, so the vertex pull stays the
+ * default.  Mutating calls. */
+int sstg_set_pagerank_segments(sstg_graph *g, uint32_t bits);
+int sstg_get_pagerank_segments(sstg_graph *g, uint32_t *bits);
 #define SSTG_TUNE_PR 1u  /* objective: pr_iters FP64 PageRank iterations */
 #define SSTG_TUNE_BFS 2u /* objective: one BFS from src */
 typedef struct sstg_tune_report {

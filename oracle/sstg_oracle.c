@@ -596,6 +596,182 @@ void orc_bc(uint32_t n, const uint64_t *off, const uint32_t *cols,
   free(sigma);
 }
 
+/* ----- all-core CPU baselines of BFS and BC (timed by bench.py) -------
+ * The serial functions above are the parity checker; these compute the same
+ * objects with OpenMP so the baseline uses every host core, as the
+ * reference's fork-join build would (its BFS and BC come from Ligra-style
+ * EdgeMapVertexMap, TST:486 and TST:547).  Level-synchronous; a level is a
+ * top-down step over the frontier (claims by compare-and-swap) or, on
+ * symmetric input with a frontier whose edges exceed the unexplored edges
+ * / 15, a bottom-up step in which every unvisited vertex looks for a
+ * neighbour in the frontier.  Levels are unique, so they equal orc_bfs's. */
+static uint32_t bfs_levels_par(uint32_t n, const uint64_t *off,
+                               const uint32_t *cols, uint32_t src, int sym,
+                               uint32_t *lv, uint32_t *order) {
+  for (int64_t i = 0; i < (int64_t)n; i++) lv[i] = UINT32_MAX;
+  if (src >= n) return 0;
+  lv[src] = 0;
+  order[0] = src;
+  uint64_t head = 0, tail = 1, m = off[n], explored = off[src + 1] - off[src];
+  uint32_t L = 0;
+  int T = 1;
+#ifdef _OPENMP
+  T = omp_get_max_threads();
+#endif
+  uint64_t *cnt = (uint64_t *)calloc((size_t)T + 1, sizeof(uint64_t));
+  while (tail > head) {
+    uint64_t fe = 0;
+#pragma omp parallel for reduction(+ : fe) schedule(static)
+    for (int64_t i = (int64_t)head; i < (int64_t)tail; i++)
+      fe += off[order[i] + 1] - off[order[i]];
+    const uint64_t unexplored = m > explored ? m - explored : 0;
+    const int bottom_up = sym && (double)fe * 15.0 > (double)unexplored;
+    if (bottom_up) {
+#pragma omp parallel for schedule(dynamic, 1024)
+      for (int64_t v = 0; v < (int64_t)n; v++) {
+        if (lv[v] != UINT32_MAX) continue;
+        for (uint64_t e = off[v]; e < off[v + 1]; e++)
+          if (lv[cols[e]] == L) {
+            lv[v] = L + 1; /* read by no one at level L + 1 in this step */
+            break;
+          }
+      }
+      /* the new frontier in ascending id order, per-thread counts first */
+#pragma omp parallel num_threads(T)
+      {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        uint64_t lo = (uint64_t)n * t / T, hi = (uint64_t)n * (t + 1) / T, c = 0;
+        for (uint64_t v = lo; v < hi; v++) c += lv[v] == L + 1;
+        cnt[t + 1] = c;
+#pragma omp barrier
+#pragma omp single
+        for (int k = 0; k < T; k++) cnt[k + 1] += cnt[k];
+        uint64_t w = tail + cnt[t];
+        for (uint64_t v = lo; v < hi; v++)
+          if (lv[v] == L + 1) order[w++] = (uint32_t)v;
+      }
+      head = tail;
+      tail += cnt[T];
+    } else {
+      uint64_t next = tail;
+#pragma omp parallel
+      {
+        uint32_t buf[256];
+        uint32_t nb = 0;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = (int64_t)head; i < (int64_t)tail; i++) {
+          const uint32_t u = order[i];
+          for (uint64_t e = off[u]; e < off[u + 1]; e++) {
+            const uint32_t w = cols[e];
+            uint32_t want = UINT32_MAX;
+            if (lv[w] == UINT32_MAX &&
+                __atomic_compare_exchange_n(&lv[w], &want, L + 1, 0,
+                                            __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+              buf[nb++] = w;
+              if (nb == 256) {
+                const uint64_t at = __atomic_fetch_add(&next, nb, __ATOMIC_RELAXED);
+                memcpy(order + at, buf, nb * sizeof(uint32_t));
+                nb = 0;
+              }
+            }
+          }
+        }
+        if (nb) {
+          const uint64_t at = __atomic_fetch_add(&next, nb, __ATOMIC_RELAXED);
+          memcpy(order + at, buf, nb * sizeof(uint32_t));
+        }
+      }
+      head = tail;
+      tail = next;
+    }
+    uint64_t ne = 0;
+#pragma omp parallel for reduction(+ : ne) schedule(static)
+    for (int64_t i = (int64_t)head; i < (int64_t)tail; i++)
+      ne += off[order[i] + 1] - off[order[i]];
+    explored += ne;
+    L++;
+  }
+  free(cnt);
+  return (uint32_t)tail;
+}
+
+static int csr_symmetric(uint32_t n, const uint64_t *off, const uint32_t *cols) {
+  int ok = 1;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(& : ok)
+  for (int64_t v = 0; v < (int64_t)n; v++)
+    for (uint64_t e = off[v]; e < off[v + 1] && ok; e++) {
+      const uint32_t w = cols[e];
+      const uint32_t k = lower_bound_u32(cols + off[w], (uint32_t)(off[w + 1] - off[w]),
+                                         (uint32_t)v);
+      if (off[w] + k >= off[w + 1] || cols[off[w] + k] != (uint32_t)v) ok = 0;
+    }
+  return ok;
+}
+
+void orc_bfs_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
+                 uint32_t src, uint32_t *level) {
+  uint32_t *order = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+  bfs_levels_par(n, off, cols, src, csr_symmetric(n, off, cols), level, order);
+  free(order);
+}
+
+/* Brandes from one source with the same conventions as orc_bc, for input
+ * where every edge has its reverse: sigma and delta are pulled level by
+ * level (a vertex's parents and children are its neighbours one level up
+ * and down), in parallel within a level.  Directed input falls back to
+ * orc_bc. */
+void orc_bc_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
+                uint32_t src, double *out) {
+  if (!csr_symmetric(n, off, cols)) {
+    orc_bc(n, off, cols, src, out);
+    return;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) out[i] = 0.0;
+  if (src >= n) return;
+  uint32_t *lv = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+  uint32_t *order = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+  double *sigma = (double *)malloc((size_t)n * sizeof(double));
+  const uint32_t reached = bfs_levels_par(n, off, cols, src, 1, lv, order);
+  /* level boundaries in order[] (a BFS order lists levels contiguously) */
+  uint32_t levels = 0;
+  for (uint32_t i = 0; i < reached; i++)
+    if (lv[order[i]] + 1 > levels) levels = lv[order[i]] + 1;
+  uint64_t *start = (uint64_t *)calloc((size_t)levels + 1, sizeof(uint64_t));
+  for (uint32_t i = 0; i < reached; i++) start[lv[order[i]] + 1]++;
+  for (uint32_t k = 0; k < levels; k++) start[k + 1] += start[k];
+  sigma[src] = 1.0;
+  for (uint32_t L = 1; L < levels; L++) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = (int64_t)start[L]; i < (int64_t)start[L + 1]; i++) {
+      const uint32_t w = order[i];
+      double sg = 0.0;
+      for (uint64_t e = off[w]; e < off[w + 1]; e++)
+        if (lv[cols[e]] == L - 1) sg += sigma[cols[e]];
+      sigma[w] = sg;
+    }
+  }
+  for (int64_t L = (int64_t)levels - 1; L >= 0; L--) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = (int64_t)start[L]; i < (int64_t)start[L + 1]; i++) {
+      const uint32_t v = order[i];
+      double d = 0.0;
+      for (uint64_t e = off[v]; e < off[v + 1]; e++) {
+        const uint32_t w = cols[e];
+        if (lv[w] == (uint32_t)L + 1) d += sigma[v] / sigma[w] * (1.0 + out[w]);
+      }
+      out[v] = d;
+    }
+  }
+  free(start);
+  free(lv);
+  free(order);
+  free(sigma);
+}
+
 static uint32_t uf_find(uint32_t *p, uint32_t x) {
   while (p[x] != x) {
     p[x] = p[p[x]];

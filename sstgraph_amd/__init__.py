@@ -38,6 +38,7 @@ SSTG_DEVICE_PTRS = 0x1
 SSTG_SERVE_COALESCE = 0x1
 SSTG_SERVE_SNAPSHOTS = 0x2
 SSTG_SERVE_ONE_HEAVY = 0x4
+SSTG_SERVE_MAINTAIN = 0x8
 SSTG_SEMI_SORTED = 0x2
 SSTG_KEEP_SELF_LOOPS = 0x4
 SSTG_LOAD_WEIGHTS = 0x8
@@ -198,6 +199,8 @@ def lib():
         "sstg_get_degree_thresholds": (i32, [vp, vp]),
         "sstg_autotune": (i32, [vp, u32, u32, u32, P(TuneReport)]),
         "sstg_set_bfs_switch": (i32, [vp, ctypes.c_double, ctypes.c_double]),
+        "sstg_set_pagerank_segments": (i32, [vp, u32]),
+        "sstg_get_pagerank_segments": (i32, [vp, P(u32)]),
         "sstg_set_bfs_degree_thresholds": (i32, [vp, vp]),
         "sstg_get_bfs_degree_thresholds": (i32, [vp, vp]),
         "sstg_get_bfs_switch": (i32, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
@@ -563,6 +566,17 @@ class SparseMatrixV:
     def set_bfs_switch(self, alpha, beta):
         _check(lib().sstg_set_bfs_switch(self._h, alpha, beta), "bfs_switch")
 
+    def pagerank_segments(self):
+        """Column-segment width (log2 vertices) of FP64 PageRank; 0 = off."""
+        b = ctypes.c_uint32(0)
+        _check(lib().sstg_get_pagerank_segments(self._h, ctypes.byref(b)),
+               "pagerank_segments")
+        return b.value
+
+    def set_pagerank_segments(self, bits):
+        _check(lib().sstg_set_pagerank_segments(self._h, int(bits)),
+               "pagerank_segments")
+
     def autotune(self, pr=True, bfs_source=None, pr_iters=2):
         """Picks the degree thresholds by timing PageRank iterations and/or a
         BFS from bfs_source on this graph (sstg_autotune); returns the
@@ -872,12 +886,13 @@ class Server:
     store while earlier queries finish on the old one."""
 
     def __init__(self, g, slots=4, hbm_budget=0, coalesce=True, snapshots=False,
-                 one_heavy=False):
+                 one_heavy=False, maintain=False):
         self.g = g
         h = ctypes.c_void_p()
         flags = (SSTG_SERVE_COALESCE if coalesce else 0) | \
             (SSTG_SERVE_SNAPSHOTS if snapshots else 0) | \
-            (SSTG_SERVE_ONE_HEAVY if one_heavy else 0)
+            (SSTG_SERVE_ONE_HEAVY if one_heavy else 0) | \
+            (SSTG_SERVE_MAINTAIN if maintain else 0)
         _check(lib().sstg_server_create(g.handle, slots, hbm_budget, flags,
                                         ctypes.byref(h)),
                "sstg_server_create")

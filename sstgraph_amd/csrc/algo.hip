@@ -625,8 +625,10 @@ static int pagerank_impl(sstg_graph *g, uint32_t iters, double damping,
   return SSTG_OK;
 }
 
-int algo_pagerank(sstg_graph *g, uint32_t iters, double damping,
-                  double *out) {
+int algo_pagerank(sstg_graph *g, uint32_t iters, double damping, double *out,
+                  bool allow_segments) {
+  if (allow_segments && pagerank_segmented_applies(g))
+    return pagerank_segmented(g, iters, damping, out);
   return pagerank_impl<double, false>(g, iters, damping, out);
 }
 

@@ -7,7 +7,13 @@
 namespace sstg {
 
 int algo_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level);
-int algo_pagerank(sstg_graph *g, uint32_t iters, double damping, double *out);
+// FP64 PageRank; column segments (pagerank.hip) when they apply and
+// allow_segments is set, the degree-binned vertex pull otherwise
+int algo_pagerank(sstg_graph *g, uint32_t iters, double damping, double *out,
+                  bool allow_segments = true);
+bool pagerank_segmented_applies(const sstg_graph *g);
+int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,
+                       double *out);
 int algo_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,
                        float *out);
 int algo_bc(sstg_graph *g, uint32_t src, double *out);
@@ -20,6 +26,11 @@ int tc_oriented(sstg_graph *g, uint64_t *out);
 int count_list_triangles(sstg_graph *g, const uint64_t *base,
                          const uint32_t *lists, const uint32_t *len,
                          uint64_t *out);
+// triangles containing at least one of the changed undirected edges `keys`
+// (sorted row << 32 | col pairs, both directions of each edge), each once;
+// valid when every edge of g has its reverse (tc.hip)
+int tc_delta(sstg_graph *g, const uint64_t *keys, uint64_t count,
+             uint64_t *out);
 int algo_bf(sstg_graph *g, uint32_t src, int32_t *out);
 int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
                           int early_end, uint64_t *out);

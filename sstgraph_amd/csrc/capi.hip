@@ -372,6 +372,19 @@ int sstg_get_bfs_switch(sstg_graph *g, double *alpha, double *beta) {
   return SSTG_OK;
 }
 
+int sstg_set_pagerank_segments(sstg_graph *g, uint32_t bits) {
+  if (!g) return SSTG_E_INVAL;
+  if (bits != 0 && (bits < 8 || bits > 30)) return SSTG_E_INVAL;
+  g->pr_seg_bits = bits;
+  return SSTG_OK;
+}
+
+int sstg_get_pagerank_segments(sstg_graph *g, uint32_t *bits) {
+  if (!g || !bits) return SSTG_E_INVAL;
+  *bits = g->pr_seg_bits;
+  return SSTG_OK;
+}
+
 int sstg_autotune(sstg_graph *g, uint32_t what, uint32_t src,
                   uint32_t pr_iters, sstg_tune_report *out) {
   if (!g) return SSTG_E_INVAL;

@@ -41,6 +41,17 @@ struct sstg_graph {
   // grows and its edges exceed unexplored / alpha; back when it holds fewer
   // than n / beta vertices
   double bfs_alpha = 15.0, bfs_beta = 18.0;
+  // FP64 PageRank over column segments of 2^pr_seg_bits vertices (0, the
+  // default: the degree-binned vertex pull); the layout is cached for graph
+  // version prseg_version
+  uint32_t pr_seg_bits = 0;
+  uint64_t prseg_version = ~0ull;
+  uint32_t prseg_bits = 0;
+  uint64_t prseg_runs = 0;
+  uint64_t prseg_cls[4] = {};
+  // a recorded pair of PageRank iterations (hipGraph_t / hipGraphExec_t)
+  // and what it was recorded for
+  void *pr_graph = nullptr, *pr_exec = nullptr, *pr_key = nullptr;
   // cached: does every stored edge have its reverse?
   uint64_t sym_version = ~0ull;
   bool symmetric = false;
@@ -65,6 +76,10 @@ void graph_free(sstg_graph *g);
 int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                  uint64_t n, bool insert, bool skip_self, bool device_ptrs);
 int graph_rebuild(sstg_graph *g, uint64_t extra_elems);
+int graph_effective_keys(sstg_graph *g, const uint32_t *src,
+                         const uint32_t *dst, uint64_t n, bool insert,
+                         bool skip_self, const uint64_t **out_keys,
+                         uint64_t *out_count);
 int graph_degree_prefix(sstg_graph *g, const uint64_t **out);
 int graph_export(sstg_graph *g, uint64_t *offsets, uint32_t *cols,
                  bool device_ptrs);
@@ -73,6 +88,8 @@ int graph_has_batch(sstg_graph *g, const uint32_t *rows, const uint32_t *cols,
 int graph_touch_sum(sstg_graph *g, uint64_t *out);
 // algo.hip
 int graph_is_symmetric(sstg_graph *g, bool *out);
+// pagerank.hip: drops the graph's recorded PageRank iterations
+void pagerank_release(sstg_graph *g);
 // tune.hip
 int graph_set_thresholds(sstg_graph *g, int set, const uint32_t t[3]);
 int graph_set_bfs_switch(sstg_graph *g, double alpha, double beta);
@@ -160,6 +177,9 @@ enum : int {
   kWsMarkPos = 123,   // update: row position of each sorted batch key
   kWsBPos = 124,      // update: the same for the kept keys
   kWsSlots = 126,
+  kWsPrSeg = 130,     // 130..142 FP64 PageRank column-segment layout
+  kWsEffKeys = 143,   // the pairs an update changes (graph_effective_keys)
+  kWsTcDelta = 144,   // per-block partial counts of tc_delta
 };
 
 // degree threshold sets of a graph (sstg_graph::bin_hi): the reductions

@@ -133,6 +133,14 @@ struct sstg_server {
   uint64_t next_ticket = 1;
   uint64_t running = 0;  // queries on slots
   uint64_t device_batches = 0;
+  // SSTG_SERVE_MAINTAIN: the triangle count of store version tc_version,
+  // carried through updates by the triangles their changed edges close or
+  // break
+  int maintain = 0;
+  bool tc_known = false;
+  uint64_t tc_count = 0, tc_version = 0;
+  uint64_t tc_from_count = 0;   // TC queries answered from the count
+  uint64_t tc_carried = 0;      // update runs folded into the count
 };
 
 namespace sstg {
@@ -145,7 +153,10 @@ uint64_t estimate(const sstg_graph *g, const sstg_request &r) {
     case SSTG_REQ_BFS:
       return n * 4 + n * 24;            // levels + queue, flags, bitmaps
     case SSTG_REQ_PR:
-      return n * 8 + n * 16 + n * 20;   // ranks + contribs + bins
+      // ranks, contributions, run ranges; the column-segment layout and its
+      // build (4 B per edge + at most one run per edge at 40 B, 1 B per
+      // edge of counts) when it applies, else the degree bins
+      return n * 32 + (pagerank_segmented_applies(g) ? g->m * 45 : n * 20);
     case SSTG_REQ_CC:
       return n * 8;
     case SSTG_REQ_TC:
@@ -381,8 +392,20 @@ int run_query(sstg_server *S, sstg_graph *v, Request *r) {
       r->result_bytes = n * 4;
       return algo_cc(v, lab);
     }
-    case SSTG_REQ_TC:
-      return algo_tc(v, &r->scalar);
+    case SSTG_REQ_TC: {
+      SSTG_TRY(algo_tc(v, &r->scalar));
+      if (S->maintain) {
+        // the count starts from a full recount of the live version
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->writing && v->version == S->g->version &&
+            S->g->sym_version == S->g->version && S->g->symmetric) {
+          S->tc_known = true;
+          S->tc_count = r->scalar;
+          S->tc_version = v->version;
+        }
+      }
+      return SSTG_OK;
+    }
     default:
       return SSTG_E_INVAL;
   }
@@ -479,6 +502,40 @@ void start_updates(sstg_server *S, size_t at) {
   S->cv.notify_all();
 }
 
+// Applies an update run to the served graph and carries its triangle count
+// (every edge with its reverse before and after): the pairs the run changes
+// are found first; a removal's broken triangles are counted on the graph
+// before it, an insertion's closed ones on the graph after it.  *carried is
+// false when the count could not be carried (then the run was applied all
+// the same, or not at all if an id was out of range).
+int update_carrying_tc(sstg_graph *g, const std::vector<uint32_t> &src,
+                       const std::vector<uint32_t> &dst, bool ins,
+                       uint64_t count0, uint64_t *count1, bool *carried) {
+  *carried = false;
+  const uint64_t n = src.size();
+  if (n >= (1ull << 31))
+    return graph_update(g, src.data(), dst.data(), n, ins, ins, false);
+  uint32_t *ds = nullptr, *dd = nullptr;
+  SSTG_TRY(g->ws.get(kWsInSrc, n + 1, &ds));
+  SSTG_TRY(g->ws.get(kWsInDst, n + 1, &dd));
+  SSTG_HIP(hipMemcpyAsync(ds, src.data(), n * 4, hipMemcpyHostToDevice,
+                          g->stream));
+  SSTG_HIP(hipMemcpyAsync(dd, dst.data(), n * 4, hipMemcpyHostToDevice,
+                          g->stream));
+  const uint64_t *keys = nullptr;
+  uint64_t count = 0;
+  int rc = graph_effective_keys(g, ds, dd, n, ins, ins, &keys, &count);
+  if (rc != SSTG_OK)  // e.g. an id out of range: the caller sorts it out
+    return graph_update(g, src.data(), dst.data(), n, ins, ins, false);
+  uint64_t delta = 0;
+  if (!ins) SSTG_TRY(tc_delta(g, keys, count, &delta));
+  SSTG_TRY(graph_update(g, ds, dd, n, ins, ins, true));
+  if (ins) SSTG_TRY(tc_delta(g, keys, count, &delta));
+  *count1 = ins ? count0 + delta : count0 - delta;
+  *carried = true;
+  return SSTG_OK;
+}
+
 void writer_main(sstg_server *S) {
   hipSetDevice(S->g->device);
   std::unique_lock<std::mutex> lk(S->mu);
@@ -488,9 +545,22 @@ void writer_main(sstg_server *S) {
     std::vector<Request *> run = S->write_run;
     const uint32_t kind = run.front()->spec.kind;
     const bool ins = kind == SSTG_REQ_INSERT;
+    sstg_graph *g = S->g;
+    const bool carry = S->maintain && S->tc_known &&
+                       S->tc_version == g->version &&
+                       g->sym_version == g->version && g->symmetric;
+    const uint64_t count0 = S->tc_count;
     lk.unlock();
-    int rc = graph_update(S->g, S->write_src.data(), S->write_dst.data(),
-                          S->write_src.size(), ins, ins, false);
+    int rc = SSTG_OK;
+    bool carried = false;
+    uint64_t count1 = 0;
+    if (carry) {
+      rc = update_carrying_tc(g, S->write_src, S->write_dst, ins, count0,
+                              &count1, &carried);
+    } else {
+      rc = graph_update(g, S->write_src.data(), S->write_dst.data(),
+                        S->write_src.size(), ins, ins, false);
+    }
     std::vector<int> rcs(run.size(), rc);
     if (rc == SSTG_E_RANGE && run.size() > 1) {
       // one request holds a bad id: apply them one by one so only it fails
@@ -499,6 +569,14 @@ void writer_main(sstg_server *S) {
                               run[i]->src.size(), ins, ins, false);
     }
     lk.lock();
+    if (carried && rc == SSTG_OK && g->sym_version == g->version &&
+        g->symmetric) {
+      S->tc_count = count1;
+      S->tc_version = g->version;
+      S->tc_carried++;
+    } else {
+      S->tc_known = false;
+    }
     const double t1 = now_s();
     for (size_t i = 0; i < run.size(); i++) {
       Request *r = run[i];
@@ -567,6 +645,25 @@ bool sched_step(sstg_server *S) {
       }
       return changed;  // later queries must see this update
     }
+    if (r->spec.kind == SSTG_REQ_TC && S->tc_known &&
+        (r->set ? r->set->version == S->tc_version
+                : !S->writing && S->g->version == S->tc_version)) {
+      // the maintained count of the version this query reads
+      S->pending.erase(S->pending.begin() + i);
+      i--;
+      r->t_start = r->t_done = now_s();
+      r->scalar = S->tc_count;
+      r->status = SSTG_OK;
+      r->done = true;
+      if (r->set) {
+        drop_reader(S, r->set);
+        r->set = nullptr;
+      }
+      S->tc_from_count++;
+      S->done_cv.notify_all();
+      changed = true;
+      continue;
+    }
     if (admissible(S, r)) {
       S->pending.erase(S->pending.begin() + i);
       admit(S, r);
@@ -607,6 +704,7 @@ int view_init(sstg_graph *v, const sstg_graph *g) {
 
 void view_free(sstg_graph *v) {
   if (v->stream) hipStreamSynchronize(v->stream);
+  pagerank_release(v);
   v->ws.release_all();
   hipFree(v->dev_ctr);
   hipFree(v->deg_prefix);
@@ -628,7 +726,7 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
                        uint32_t flags, sstg_server **out) {
   if (!g || !out || slots == 0 || slots > 64) return SSTG_E_INVAL;
   if (flags & ~(SSTG_SERVE_COALESCE | SSTG_SERVE_SNAPSHOTS |
-                SSTG_SERVE_ONE_HEAVY))
+                SSTG_SERVE_ONE_HEAVY | SSTG_SERVE_MAINTAIN))
     return SSTG_E_INVAL;
   *out = nullptr;
   // store copies and update batches carry no weight table
@@ -642,6 +740,7 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
   S->coalesce = (flags & SSTG_SERVE_COALESCE) ? 1 : 0;
   S->snapshots = (flags & SSTG_SERVE_SNAPSHOTS) ? 1 : 0;
   S->one_heavy = (flags & SSTG_SERVE_ONE_HEAVY) ? 1 : 0;
+  S->maintain = (flags & SSTG_SERVE_MAINTAIN) ? 1 : 0;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
     delete S;

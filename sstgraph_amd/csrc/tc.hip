@@ -321,6 +321,75 @@ __global__ void k_tc_acc_sum(unsigned long long *acc) {
   if (threadIdx.x == 0) acc[0] = v;
 }
 
+// Triangles of a graph whose every edge has its reverse that contain at
+// least one changed undirected edge, each counted once, by its smallest
+// changed edge in (min << 32 | max) order.  keys: the changed pairs, sorted,
+// with both directions of every edge.  One wave per changed edge {u, v},
+// u < v: lanes take the ids of the shorter row and look each up in the
+// longer one; a common neighbour w counts unless {u, w} or {v, w} is also
+// changed with a smaller key.  Per-block partial sums, no atomics.
+__device__ __forceinline__ bool sorted_has(const uint64_t *a, uint64_t n,
+                                           uint64_t x) {
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && a[lo] == x;
+}
+
+__global__ void __launch_bounds__(256)
+    k_tc_delta(DevGraph g, const uint64_t *keys, uint64_t count,
+               unsigned long long *part) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t waves = (uint64_t)gridDim.x * 4;
+  uint64_t found = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64; i < count;
+       i += waves) {
+    const uint64_t kuv = keys[i];
+    const uint32_t u = (uint32_t)(kuv >> 32), v = (uint32_t)kuv;
+    if (u >= v) continue;  // the wave's edge: each undirected edge once
+    const uint32_t du = g.deg[u], dv = g.deg[v];
+    const uint32_t a = du <= dv ? u : v, b = du <= dv ? v : u;
+    const uint32_t da = du <= dv ? du : dv, db = du <= dv ? dv : du;
+    const uint32_t *ra = g.pool + g.off[a], *rb = g.pool + g.off[b];
+    for (uint32_t j = lane; j < da; j += 64) {
+      const uint32_t w = ra[j];
+      if (w == u || w == v) continue;
+      const uint64_t p = lower_bound_dev(rb, (uint64_t)db, w);
+      if (p >= db || rb[p] != w) continue;
+      const uint64_t kuw = u < w ? ((uint64_t)u << 32) | w
+                                 : ((uint64_t)w << 32) | u;
+      const uint64_t kvw = v < w ? ((uint64_t)v << 32) | w
+                                 : ((uint64_t)w << 32) | v;
+      if (kuw < kuv && sorted_has(keys, count, kuw)) continue;
+      if (kvw < kuv && sorted_has(keys, count, kvw)) continue;
+      found++;
+    }
+  }
+  __shared__ unsigned long long wsum[4];
+  found = wave_sum(found);
+  if (lane == 0) wsum[threadIdx.x / 64] = found;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ void __launch_bounds__(256)
+    k_tc_delta_sum(const unsigned long long *part, uint32_t nb,
+                   unsigned long long *out) {
+  unsigned long long t = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += 256) t += part[i];
+  __shared__ unsigned long long wsum[4];
+  t = wave_sum(t);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x / 64] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
 uint32_t grid_for(uint64_t count) {
   return (uint32_t)(count < (1u << 24) ? count : (1u << 24));
 }
@@ -424,6 +493,24 @@ int tc_oriented(sstg_graph *g, uint64_t *out) {
                   s, v, (const uint32_t *)bins.list[b], bins.count[b], prefix,
                   dag, dplus);
   return count_list_triangles(g, prefix, dag, dplus, out);
+}
+
+int tc_delta(sstg_graph *g, const uint64_t *keys, uint64_t count,
+             uint64_t *out) {
+  *out = 0;
+  if (count == 0) return SSTG_OK;
+  hipStream_t s = g->stream;
+  const uint32_t nb = (uint32_t)std::min<uint64_t>(4096, (count + 7) / 8);
+  unsigned long long *part = nullptr;
+  SSTG_TRY(g->ws.get(kWsTcDelta, nb + 1, &part));
+  SSTG_LAUNCH(k_tc_delta, nb, 256, 0, s, g->view(), keys, count, part);
+  SSTG_LAUNCH(k_tc_delta_sum, 1, 256, 0, s, (const unsigned long long *)part,
+              nb, part + nb);
+  SSTG_HIP(hipMemcpyAsync(g->host_rb + 40, part + nb, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  *out = g->host_rb[40];
+  return SSTG_OK;
 }
 
 }  // namespace sstg

@@ -80,8 +80,8 @@ struct Objective {
   }
 
   int once() {
-    if (set == kBinsPull)
-      SSTG_TRY(algo_pagerank(g, pr_iters, 0.85, rank));
+    if (set == kBinsPull)  // the degree bins drive the vertex pull
+      SSTG_TRY(algo_pagerank(g, pr_iters, 0.85, rank, false));
     else
       SSTG_TRY(algo_bfs(g, src, nullptr, level));
     SSTG_HIP(hipStreamSynchronize(g->stream));

@@ -930,6 +930,7 @@ int graph_alloc(sstg_graph *g) {
 
 void graph_free(sstg_graph *g) {
   if (g->stream) hipStreamSynchronize(g->stream);
+  pagerank_release(g);
   weights_free(g);
   hipFree(g->off);
   hipFree(g->deg);
@@ -1197,6 +1198,53 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   g->garbage += garbage_add;
   g->m = insert ? g->m + b : g->m - b;
   g->version++;
+  return SSTG_OK;
+}
+
+// The pairs an update would change, without applying it: the batch packed,
+// sorted and deduplicated, self-loops dropped for an insert that skips
+// them, and then kept if absent (insert) or present (remove).  The sorted
+// keys (row << 32 | col) land in workspace slot kWsEffKeys, which updates
+// do not use, so they outlive a following graph_update.  Device arrays,
+// n < 2^31.
+int graph_effective_keys(sstg_graph *g, const uint32_t *src,
+                         const uint32_t *dst, uint64_t n, bool insert,
+                         bool skip_self, const uint64_t **out_keys,
+                         uint64_t *out_count) {
+  hipStream_t s = g->stream;
+  Workspace &ws = g->ws;
+  uint64_t *ctr = g->dev_ctr;
+  *out_count = 0;
+  uint64_t *out = nullptr;
+  SSTG_TRY(ws.get(kWsEffKeys, n + 1, &out));
+  *out_keys = out;
+  if (n == 0) return SSTG_OK;
+  uint64_t *keys = nullptr, *tmp = nullptr;
+  uint8_t *flags = nullptr;
+  uint32_t *pos = nullptr;
+  SSTG_TRY(ws.get(kWsKeysA, n, &keys));
+  SSTG_TRY(ws.get(kWsKeysB, n, &tmp));
+  SSTG_TRY(ws.get(kWsFlags, n + 1, &flags));
+  SSTG_TRY(ws.get(kWsMarkPos, n, &pos));
+  SSTG_HIP(hipMemsetAsync(ctr, 0, kCtrCount * sizeof(uint64_t), s));
+  SSTG_LAUNCH(k_pack, blocks(n), kT, 0, s, src, dst, n, g->n_rows, g->n_cols,
+              keys, ctr);
+  int shifts[16], np = 0;
+  uint32_t cb = bit_width_u32(g->n_cols ? g->n_cols - 1 : 0);
+  uint32_t rb = bit_width_u32(g->n_rows ? g->n_rows - 1 : 0);
+  for (uint32_t b = 0; b < cb; b += 8) shifts[np++] = (int)b;
+  for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
+  uint64_t *sorted = nullptr;
+  SSTG_TRY(radix_sort_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  SSTG_LAUNCH(k_mark, blocks(n), kT, 0, s, (const uint64_t *)sorted, n,
+              g->view(), insert ? 1 : 0, skip_self ? 1 : 0, flags, pos);
+  SSTG_TRY(select_flagged_u64(flags, sorted, n, out, ctr + kCtrB, ws, kWsPrim,
+                              s));
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, 2 * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  if (g->host_rb[kCtrErr]) return SSTG_E_RANGE;
+  *out_count = g->host_rb[kCtrB];
   return SSTG_OK;
 }
 

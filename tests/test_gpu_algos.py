@@ -229,3 +229,63 @@ def test_cc_many_components():
     o_off, o_cols = og.csr()
     assert np.array_equal(EMV.CC(g), O.cc(o_off, o_cols))
 
+
+
+@pytest.mark.parametrize("bits", [8, 10, 12, 14])
+def test_pagerank_column_segments_match_oracle(c1, bits):
+    """FP64 PageRank over column segments of 2^bits vertices equals the
+    oracle within 1e-9 and the vertex pull within FP64 reordering, for 1, 10
+    and 20 iterations; the layout cached for one graph version is rebuilt
+    after an update."""
+    n, g0, off, cols = c1
+    s = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
+    g = S.SparseMatrixV(n)
+    g.insert_batch(s, cols)
+    g.set_pagerank_segments(0)
+    plain = {it: EMV.PR_S(g, it) for it in (1, 10, 20)}
+    g.set_pagerank_segments(bits)
+    assert g.pagerank_segments() == bits
+    for it in (1, 10, 20, 10):  # the repeat reuses the cached layout
+        say("segments", bits, "iters", it)
+        got = EMV.PR_S(g, it)
+        ok, w = rel_close(got, O.pagerank(off, cols, it, 0.85), RTOL)
+        assert ok, (bits, it, w)
+        ok, w = rel_close(got, plain[it], 1e-12)
+        assert ok, (bits, it, w)
+    # an update changes the graph version: the next call rebuilds
+    rng = np.random.default_rng(bits)
+    bs = rng.integers(0, n, 5000).astype(np.uint32)
+    bd = rng.integers(0, n, 5000).astype(np.uint32)
+    g.insert_batch(bs, bd)
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+    og.insert_batch(bs, bd)
+    o_off, o_cols = og.csr()
+    ok, w = rel_close(EMV.PR_S(g, 10), O.pagerank(o_off, o_cols, 10, 0.85), RTOL)
+    assert ok, w
+
+
+def test_pagerank_segments_long_runs_and_directed_rows():
+    """Runs of every length class (a hub whose ids fill one segment, mid
+    rows, single ids), empty rows, rows without their reverses and a
+    self-loop, on segments of 2^8 vertices, against the oracle."""
+    n = 1 << 12
+    rng = np.random.default_rng(23)
+    s = [np.full(3000, 7, np.int64), np.full(200, 9, np.int64),
+         rng.integers(0, n, 20000)]
+    d = [np.arange(3000) % 256 + 512, rng.integers(1024, 1300, 200),
+         rng.integers(0, n, 20000)]
+    S_ = np.concatenate(s).astype(np.uint32)
+    D_ = np.concatenate(d).astype(np.uint32)
+    g = S.SparseMatrixV(n)
+    g.insert_batch(S_, D_)           # directed: no reverses added
+    g.insert(11, 11)                  # a self-loop (single insert keeps it)
+    og = O.Graph(n)
+    og.insert_batch(S_, D_)
+    og.insert(11, 11)
+    off, cols = og.csr()
+    assert (np.diff(off) == 0).any()
+    g.set_pagerank_segments(8)
+    for it in (1, 5, 20):
+        ok, w = rel_close(EMV.PR_S(g, it), O.pagerank(off, cols, it, 0.85), RTOL)
+        assert ok, (it, w)

@@ -62,11 +62,11 @@ def _expected(n, base_s, base_d, reqs):
 
 
 def _serve(n, base_s, base_d, reqs, slots, coalesce, snapshots=False,
-           one_heavy=False):
+           one_heavy=False, maintain=False):
     g = S.SparseMatrixV(n)
     g.insert_batch(base_s, base_d)
     srv = S.Server(g, slots=slots, coalesce=coalesce, snapshots=snapshots,
-                   one_heavy=one_heavy)
+                   one_heavy=one_heavy, maintain=maintain)
     tickets = []
     for kind, a, b in reqs:  # submitted back to back: they queue and overlap
         if kind == "insert":
@@ -138,6 +138,10 @@ def test_served_results_equal_alone_results():
                              snapshots=True, one_heavy=True)
     _compare(reqs, got3, exp)
     say("store snapshots", stats3["snapshots"])
+    say("serve with the triangle count carried through updates")
+    got4, _, _ = _serve(n, s, d, reqs, slots=4, coalesce=True, snapshots=True,
+                        one_heavy=True, maintain=True)
+    _compare(reqs, got4, exp)
     say("serve serialised")
     got1, coalesced1, stats1 = _serve(n, s, d, reqs, slots=1, coalesce=False)
     _compare(reqs, got1, exp)
@@ -165,3 +169,39 @@ def test_admission_by_hbm_budget():
         assert np.array_equal(srv.fetch(t, np.uint32), O.bfs(o_off, o_cols, v)[0])
         srv.release(t)
     srv.close()
+
+
+def test_triangle_count_carried_through_updates():
+    """With the count maintained, TC queries between symmetric inserts and
+    removes (some of them closing or breaking many triangles at once: a
+    clique's edges inserted and removed in one batch) equal the oracle's
+    count at their position; one-directional updates stop the carrying and a
+    later TC counts in full again."""
+    need_gpu()
+    rng = np.random.default_rng(8)
+    n, s, d = O.rmat_graph_edges(12, 8, 9)
+    clique = np.arange(100, 140, dtype=np.uint32)
+    a, b = np.triu_indices(len(clique), 1)
+    cs, cd = clique[a], clique[b]
+    reqs = [("tc", None, None)]
+    for i in range(24):
+        if i == 5:
+            reqs.append(("insert", np.concatenate([cs, cd]), np.concatenate([cd, cs])))
+        elif i == 11:
+            reqs.append(("remove", np.concatenate([cs[::2], cd[::2]]),
+                         np.concatenate([cd[::2], cs[::2]])))
+        elif i == 17:  # one direction only: the graph turns directed
+            reqs.append(("insert", rng.integers(0, n, 50).astype(np.uint32),
+                         rng.integers(0, n, 50).astype(np.uint32)))
+        else:
+            k = int(rng.integers(10, 3000))
+            es, ed = O.rmat(n, 900 + i, k)
+            kind = "insert" if i % 3 else "remove"
+            reqs.append((kind, np.concatenate([es, ed]), np.concatenate([ed, es])))
+        reqs.append(("tc", None, None))
+    exp = _expected(n, s, d, reqs)
+    for one_heavy in (True, False):
+        say("carried triangle count, one heavy" if one_heavy else "carried, concurrent")
+        got, _, _ = _serve(n, s, d, reqs, slots=4, coalesce=True, snapshots=True,
+                           one_heavy=one_heavy, maintain=True)
+        _compare(reqs, got, exp)

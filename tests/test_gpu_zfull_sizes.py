@@ -167,9 +167,10 @@ def _fingerprint(a):
 @pytest.mark.timeout(1200)
 def test_c5_scale25_served_equals_alone():
     """A served stream (4 slots, coalesced updates, store snapshots, one
-    heavy query at a time) on rMat scale 25: each query's result equals the
-    same query run alone, in arrival order, on a second copy of the graph
-    that receives the same updates."""
+    heavy query at a time, the triangle count carried through updates) on
+    rMat scale 25: each query's result equals the same query run alone, in
+    arrival order, on a second copy of the graph that receives the same
+    updates."""
     need_gpu()
     n = 1 << 25
     src, dst = S.rmat(n, 4, 16 * n, device=True)
@@ -196,7 +197,8 @@ def test_c5_scale25_served_equals_alone():
     _, reqs = W.serve_trace(n, served_g.degrees(), 40, 1.0)
     reqs.append(("cc", None))
     batches = W.serve_batches(lambda nn, sd, c: S.rmat(nn, sd, c), n, reqs)
-    srv = S.Server(served_g, slots=4, coalesce=True, snapshots=True, one_heavy=True)
+    srv = S.Server(served_g, slots=4, coalesce=True, snapshots=True, one_heavy=True,
+                   maintain=True)
     tickets = []
     for kind, arg in reqs:
         if kind in ("insert", "remove"):

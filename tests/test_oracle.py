@@ -414,3 +414,23 @@ def test_er_generator_is_uniform_and_seeded():
     assert np.array_equal(s2, s[5:15])
     s3, _ = O.er(n, 12, 10)
     assert not np.array_equal(s3, s[:10])
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_all_core_bfs_and_bc_equal_the_serial_oracle(symmetric):
+    """The all-core baselines compute the same levels (bit-exact, both step
+    directions taken on the symmetric rMat graph) and the same dependencies
+    (within FP64 reordering) as the serial checker."""
+    n, s, d = O.rmat_graph_edges(14, 16, 5)
+    if not symmetric:
+        s, d = s[: len(s) // 2], d[: len(d) // 2]
+    off, cols = csr_from_edges(n, s, d)
+    deg = np.diff(off)
+    for src in (0, int(np.argmax(deg)), int(np.nonzero(deg)[0][-1])):
+        lev, _ = O.bfs(off, cols, src)
+        assert np.array_equal(O.bfs_par(off, cols, src), lev)
+        want = O.bc(off, cols, src)
+        got = O.bc_par(off, cols, src)
+        tol = 1e-9 * np.abs(want) + 1e-12
+        assert (np.abs(got - want) <= tol).all()
+    assert O.bfs_par(off, cols, n - 1).dtype == np.uint32
