@@ -38,6 +38,7 @@ extern "C" {
 #define SSTG_E_NODEVICE -5 /* no usable gfx950 device */
 #define SSTG_E_UNSUPPORTED -6
 #define SSTG_E_IO -7       /* a file could not be opened, read or written */
+#define SSTG_E_TOO_LARGE -8 /* beyond an allocation-free call's fixed scratch */
 
 /* flags */
 #define SSTG_DEVICE_PTRS 0x1u  /* array arguments are device pointers */
@@ -72,6 +73,24 @@ int sstg_insert_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
  * self-loops are not skipped. */
 int sstg_remove_batch(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                       uint64_t n, uint32_t flags);
+/* Allocation-free insert_batch.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during the call (code
+ * under HIP graph capture, tight serving loops, hosts that forbid it).
+ * Same result as sstg_insert_batch, but it calls no hipMalloc/hipFree: its
+ * kernels work in one fixed device scratch array per GPU (64 MiB, held by
+ * the library for the life of the process), so
+ *  - src/dst must be device arrays (flags must include SSTG_DEVICE_PTRS);
+ *  - n above SSTG_NOALLOC_MAX_PAIRS is SSTG_E_TOO_LARGE, graph unchanged;
+ *  - a batch whose rows need more pool room than the pool has left, or
+ *    whose long rows need more staging than the scratch has left, returns
+ *    SSTG_E_NOMEM with the graph unchanged (sstg_insert_batch grows the
+ *    pool);
+ *  - the optional compaction of abandoned slots is skipped.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on
+ * one GPU (several host threads, or graphs with different streams) race. */
+#define SSTG_NOALLOC_MAX_PAIRS (1ull << 18)
+int sstg_insert_batch_noalloc(sstg_graph *g, const uint32_t *src,
+                              const uint32_t *dst, uint64_t n, uint32_t flags);
 /* SparseMatrixV::insert(row, col), SM:536-544 (no self-loop skip). */
 /* Weighted batch insert (SparseMatrixV<true, uint32_t>::insert_batch with
  * (row, col, value) tuples, SparseMatrix.hpp:556-643 and the valued PMA

@@ -100,6 +100,19 @@ void Profiler::reset() {
 // ----------------------------------------------------------- workspace
 int Workspace::get(int slot, size_t bytes, void **out) {
   if (slot < 0) return SSTG_E_INVAL;
+  if (arena) {
+    if ((size_t)slot >= arena_slots.size()) arena_slots.resize(slot + 1);
+    Slot &a = arena_slots[slot];
+    if (a.bytes < bytes) {
+      const size_t want = (bytes + 255) & ~(size_t)255;
+      if (want > arena_bytes - arena_used) return SSTG_E_NOMEM;
+      a.p = arena + arena_used;
+      a.bytes = want;
+      arena_used += want;
+    }
+    *out = a.p;
+    return SSTG_OK;
+  }
   if ((size_t)slot >= slots.size()) slots.resize(slot + 1);
   Slot &s = slots[slot];
   if (s.bytes < bytes) {
@@ -191,6 +204,8 @@ const char *sstg_strerror(int status) {
       return "file could not be opened, read or written";
     case SSTG_E_UNSUPPORTED:
       return "unsupported";
+    case SSTG_E_TOO_LARGE:
+      return "input larger than the fixed scratch of an allocation-free call";
     default:
       return "unknown status";
   }

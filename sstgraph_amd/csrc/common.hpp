@@ -86,6 +86,25 @@ struct Workspace {
   std::vector<Slot> slots;
   hipStream_t stream = nullptr;
   size_t total = 0;
+  // Arena mode (begin_arena .. end_arena): get() carves each slot from one
+  // fixed device range instead of allocating, and fails with SSTG_E_NOMEM
+  // when the range is spent.  A slot asked for again with no more bytes
+  // keeps its memory; a larger request takes a fresh piece (its contents
+  // are lost, as when an allocated slot grows).
+  uint8_t *arena = nullptr;
+  size_t arena_bytes = 0, arena_used = 0;
+  std::vector<Slot> arena_slots;
+  void begin_arena(void *base, size_t bytes) {
+    arena = static_cast<uint8_t *>(base);
+    arena_bytes = bytes;
+    arena_used = 0;
+    arena_slots.clear();
+  }
+  void end_arena() {
+    arena = nullptr;
+    arena_bytes = arena_used = 0;
+    arena_slots.clear();
+  }
   int get(int slot, size_t bytes, void **out);
   template <class T> int get(int slot, size_t count, T **out) {
     void *p = nullptr;

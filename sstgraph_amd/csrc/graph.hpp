@@ -21,6 +21,10 @@ struct sstg_graph {
   sstg_update_stats last{};
   // per-call scratch
   sstg::Workspace ws;
+  // set for the duration of an allocation-free call: updates then skip the
+  // optional pool compaction and fail (graph unchanged) when the pool tail
+  // has no room, instead of allocating
+  bool noalloc = false;
   // small pinned host block for read-backs
   uint64_t *host_rb = nullptr;
   // device-side counters block (see update.hip)

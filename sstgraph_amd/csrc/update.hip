@@ -1013,7 +1013,7 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   uint64_t *ctr = g->dev_ctr;
   // density rule at the root: once abandoned slots exceed half of the used
   // pool, compact before taking more space
-  if (insert && g->garbage > 0 && g->garbage * 2 > g->pool_tail)
+  if (insert && !g->noalloc && g->garbage > 0 && g->garbage * 2 > g->pool_tail)
     SSTG_TRY(rebuild_impl(g, 0, true));
 
   uint64_t *keys = nullptr, *tmp = nullptr, *B = nullptr, *seg_idx = nullptr;
@@ -1121,6 +1121,9 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   g->last.allocated += alloc_tot;
   if (b == 0) return SSTG_OK;
   if (g->pool_tail + alloc_tot > g->pool_cap) {
+    // nothing has been written to the store yet: an allocation-free call
+    // stops here with the graph unchanged
+    if (g->noalloc) return SSTG_E_NOMEM;
     // slot sizes are kept, so the decisions made above stay valid
     SSTG_TRY(rebuild_impl(g, alloc_tot, false));
   }
