@@ -614,7 +614,28 @@ static int pagerank_impl(sstg_graph *g, uint32_t iters, double damping,
               x0, out, contrib);
   const Real base = (Real)((1.0 - damping) / (double)n);
   const Real damp = (Real)damping;
-  for (uint32_t it = 0; it < iters; it++) {
+  // pairs of iterations replayed from one recording (contributions back in
+  // their first buffer after two); an odd last iteration launched plainly
+  uint32_t plain = iters;
+  if (iters >= 2 && iteration_capture_enabled()) {
+    IterPairKey key{};
+    key.path = kKahan ? 2 : 1;
+    key.version = g->version;
+    key.setting = g->tune_epoch;
+    key.n = n;
+    const void *ptrs[10] = {contrib, contrib2, out, g->deg, g->off, g->pool,
+                            bins.list[0], bins.list[3], nullptr, nullptr};
+    for (int i = 0; i < 10; i++) key.ptr[i] = ptrs[i];
+    key.damping = damping;
+    SSTG_TRY(replay_iteration_pairs(g, key, iters / 2, [&]() {
+      SSTG_TRY(vertex_pull_all(
+          g, bins, PrPull<Real, kKahan>{contrib, contrib2, out, g->deg, damp, base}));
+      return vertex_pull_all(
+          g, bins, PrPull<Real, kKahan>{contrib2, contrib, out, g->deg, damp, base});
+    }));
+    plain = iters & 1u;
+  }
+  for (uint32_t it = 0; it < plain; it++) {
     PrPull<Real, kKahan> f{contrib, contrib2, out, g->deg, damp, base};
     SSTG_TRY(vertex_pull_all(g, bins, f));
     Real *t = contrib;

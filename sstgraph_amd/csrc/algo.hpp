@@ -2,6 +2,8 @@
 // All outputs are device arrays of n_rows entries.
 #pragma once
 
+#include <functional>
+
 #include "graph.hpp"
 
 namespace sstg {
@@ -12,6 +14,22 @@ int algo_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level);
 int algo_pagerank(sstg_graph *g, uint32_t iters, double damping, double *out,
                   bool allow_segments = true);
 bool pagerank_segmented_applies(const sstg_graph *g);
+// A pair of PageRank iterations recorded into a hipGraph and replayed
+// (pagerank.hip): the graph keeps one recording and its key, and records
+// again when the key changes.  The environment variable
+// SSTG_NO_GRAPH_CAPTURE makes every iteration launch its kernels plainly.
+struct IterPairKey {
+  uint32_t path;     // which PageRank: 1 vertex pull FP64, 2 FP32 + Kahan,
+                     // 3 column segments
+  uint32_t n;
+  uint64_t version;  // graph version
+  uint64_t setting;  // degree-threshold epoch or segment width
+  const void *ptr[10];
+  double damping;
+};
+bool iteration_capture_enabled();
+int replay_iteration_pairs(sstg_graph *g, const IterPairKey &key,
+                           uint32_t pairs, const std::function<int()> &record);
 int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,
                        double *out);
 int algo_pagerank_f32k(sstg_graph *g, uint32_t iters, double damping,

@@ -19,6 +19,7 @@
 // the graph's workspace.
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 
 #include "algo.hpp"
 #include "edge_map.hpp"
@@ -355,22 +356,61 @@ int pr_iteration(sstg_graph *g, const SegLayout &L, const double *contrib,
   return SSTG_OK;
 }
 
-// Everything a captured pair of iterations depends on.
-struct PrCaptureKey {
-  uint64_t version;
-  uint32_t bits, n;
-  const void *ptr[10];
-  double damping;
-};
-
 }  // namespace
 
 void pagerank_release(sstg_graph *g) {
   if (g->pr_exec) hipGraphExecDestroy((hipGraphExec_t)g->pr_exec);
   if (g->pr_graph) hipGraphDestroy((hipGraph_t)g->pr_graph);
   g->pr_exec = g->pr_graph = nullptr;
-  delete static_cast<PrCaptureKey *>(g->pr_key);
+  delete static_cast<IterPairKey *>(g->pr_key);
   g->pr_key = nullptr;
+}
+
+bool iteration_capture_enabled() {
+  static const bool off = std::getenv("SSTG_NO_GRAPH_CAPTURE") != nullptr;
+  return !off;
+}
+
+// The graph keeps one recorded pair of iterations: it is recorded again
+// whenever the key (path, graph version, settings, buffers, damping)
+// differs from the one it was recorded for.
+int replay_iteration_pairs(sstg_graph *g, const IterPairKey &key,
+                           uint32_t pairs, const std::function<int()> &record) {
+  hipStream_t s = g->stream;
+  const IterPairKey *have = static_cast<const IterPairKey *>(g->pr_key);
+  if (!g->pr_exec || !have || std::memcmp(have, &key, sizeof(key)) != 0) {
+    pagerank_release(g);
+    Profiler &prof = profiler();
+    const bool prof_on = prof.on;
+    prof.on = false;  // no event records inside the capture
+    SSTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = record();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    prof.on = prof_on;
+    if (rc != SSTG_OK) {
+      if (graph) hipGraphDestroy(graph);
+      return rc;
+    }
+    SSTG_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      hipGraphDestroy(graph);
+      SSTG_HIP(ei);
+    }
+    g->pr_graph = graph;
+    g->pr_exec = exec;
+    g->pr_key = new IterPairKey(key);
+  }
+  Profiler &prof = profiler();
+  for (uint32_t k = 0; k < pairs; k++) {
+    hipEvent_t ev = nullptr;
+    if (prof.on) prof.begin("pagerank_pair_replay", s, &ev);
+    SSTG_HIP(hipGraphLaunch((hipGraphExec_t)g->pr_exec, s));
+    if (prof.on) prof.end("pagerank_pair_replay", s, ev);
+  }
+  return SSTG_OK;
 }
 
 bool pagerank_segmented_applies(const sstg_graph *g) {
@@ -380,10 +420,7 @@ bool pagerank_segmented_applies(const sstg_graph *g) {
 
 // The iterations run in pairs (contributions back in their first buffer
 // after two), and a pair is recorded once into a hipGraph and replayed: one
-// launch per pair instead of eight.  The recording is kept until the graph,
-// the layout's buffers, the output or the damping change.  With the
-// environment variable SSTG_NO_GRAPH_CAPTURE set, every kernel is launched
-// plainly.
+// launch per pair instead of eight (replay_iteration_pairs).
 int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,
                        double *out) {
   hipStream_t s = g->stream;
@@ -398,51 +435,20 @@ int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,
   SSTG_LAUNCH(k_pr_seg_init, blocks_for(n), kT, 0, s, n,
               (const uint32_t *)g->deg, x0, out, contrib);
   uint32_t plain = iters;
-  static const bool no_capture = std::getenv("SSTG_NO_GRAPH_CAPTURE") != nullptr;
-  if (iters >= 2 && !no_capture) {
-    PrCaptureKey key{};
+  if (iters >= 2 && iteration_capture_enabled()) {
+    IterPairKey key{};
+    key.path = 3;  // column segments, FP64
     key.version = g->version;
-    key.bits = g->pr_seg_bits;
+    key.setting = g->pr_seg_bits;
     key.n = n;
     const void *ptrs[10] = {L.cols, L.pstart, L.pos, L.first, L.endr,
                             contrib, contrib2, partial, out, g->deg};
     for (int i = 0; i < 10; i++) key.ptr[i] = ptrs[i];
     key.damping = damping;
-    const PrCaptureKey *have = static_cast<const PrCaptureKey *>(g->pr_key);
-    if (!g->pr_exec || !have || std::memcmp(have, &key, sizeof(key)) != 0) {
-      pagerank_release(g);
-      Profiler &prof = profiler();
-      const bool prof_on = prof.on;
-      prof.on = false;  // no event records inside the capture
-      SSTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = pr_iteration(g, L, contrib, contrib2, partial, damping, out);
-      if (rc == SSTG_OK)
-        rc = pr_iteration(g, L, contrib2, contrib, partial, damping, out);
-      hipGraph_t graph = nullptr;
-      const hipError_t e = hipStreamEndCapture(s, &graph);
-      prof.on = prof_on;
-      if (rc != SSTG_OK) {
-        if (graph) hipGraphDestroy(graph);
-        return rc;
-      }
-      SSTG_HIP(e);
-      hipGraphExec_t exec = nullptr;
-      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      if (ei != hipSuccess) {
-        hipGraphDestroy(graph);
-        SSTG_HIP(ei);
-      }
-      g->pr_graph = graph;
-      g->pr_exec = exec;
-      g->pr_key = new PrCaptureKey(key);
-    }
-    for (uint32_t k = 0; k + 2 <= iters; k += 2) {
-      Profiler &prof = profiler();
-      hipEvent_t ev = nullptr;
-      if (prof.on) prof.begin("pr_pair_replay", s, &ev);
-      SSTG_HIP(hipGraphLaunch((hipGraphExec_t)g->pr_exec, s));
-      if (prof.on) prof.end("pr_pair_replay", s, ev);
-    }
+    SSTG_TRY(replay_iteration_pairs(g, key, iters / 2, [&]() {
+      SSTG_TRY(pr_iteration(g, L, contrib, contrib2, partial, damping, out));
+      return pr_iteration(g, L, contrib2, contrib, partial, damping, out);
+    }));
     plain = iters & 1u;
   }
   for (uint32_t it = 0; it < plain; it++) {

@@ -289,3 +289,43 @@ def test_pagerank_segments_long_runs_and_directed_rows():
     for it in (1, 5, 20):
         ok, w = rel_close(EMV.PR_S(g, it), O.pagerank(off, cols, it, 0.85), RTOL)
         assert ok, (it, w)
+
+
+def test_pagerank_replayed_pairs_stay_exact(c1):
+    """PageRank iterations run as replays of one recorded pair: calls on an
+    unchanged graph repeat bit for bit, with other device work (BFS, CC)
+    and synchronisations between them, and a call after an update or a
+    threshold change (which records again) still matches the oracle; odd
+    iteration counts finish with one plain iteration."""
+    n, g0, off, cols = c1
+    s = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
+    g = S.SparseMatrixV(n)
+    g.insert_batch(s, cols)
+    first = EMV.PR_S(g, 10)
+    ok, w = rel_close(first, O.pagerank(off, cols, 10, 0.85), RTOL)
+    assert ok, w
+    for k in range(4):
+        EMV.BFS(g, k, levels=True)
+        EMV.CC(g)
+        S.synchronize()
+        assert np.array_equal(EMV.PR_S(g, 10), first), k
+    for it in (2, 3, 9):
+        ok, w = rel_close(EMV.PR_S(g, it), O.pagerank(off, cols, it, 0.85), RTOL)
+        assert ok, (it, w)
+    g.set_degree_thresholds([4, 32, 1024])
+    ok, w = rel_close(EMV.PR_S(g, 10), O.pagerank(off, cols, 10, 0.85), RTOL)
+    assert ok, w
+    rng = np.random.default_rng(5)
+    bs = rng.integers(0, n, 3000).astype(np.uint32)
+    bd = rng.integers(0, n, 3000).astype(np.uint32)
+    g.insert_batch(bs, bd)
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+    og.insert_batch(bs, bd)
+    o_off, o_cols = og.csr()
+    for it in (10, 11):
+        ok, w = rel_close(EMV.PR_S(g, it), O.pagerank(o_off, o_cols, it, 0.85), RTOL)
+        assert ok, (it, w)
+    ok, w = rel_close(EMV.PR_F32K(g, 10).astype(np.float64),
+                      O.pagerank(o_off, o_cols, 10, 0.85), 1e-5)
+    assert ok, w
