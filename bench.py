@@ -140,8 +140,9 @@ def cpu_algos(off, cols, hub, iters=2):
     for t in (threads, 1):
         O.set_threads(t)
         key = "all_cores" if t == threads else "one_core"
-        t_bfs = timed(lambda: O.bfs_par(off, cols, hub))
-        t_bc = timed(lambda: O.bc_par(off, cols, hub))
+        # the base graph is symmetrised by construction
+        t_bfs = timed(lambda: O.bfs_par(off, cols, hub, symmetric=True))
+        t_bc = timed(lambda: O.bc_par(off, cols, hub, symmetric=True))
         t_pr = timed(lambda: O.pagerank(off, cols, iters))
         out[key] = dict(cores=t, bfs_hub_ms=1e3 * t_bfs, bfs_gteps=m / t_bfs / 1e9,
                         bc_hub_ms=1e3 * t_bc, pagerank_ms_per_iter=1e3 * t_pr / iters,
@@ -663,8 +664,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served-stream comparison of the default run")
-    ap.add_argument("--grow-batches", type=int, default=60,
-                    help="fresh 1e6-edge inserts of the growing sequence")
+    ap.add_argument("--grow-batches", type=int, default=24,
+                    help="fresh headline-size inserts of the growing sequence")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the default run's served-stream comparison")
     ap.add_argument("--no-algos", action="store_true")
@@ -905,17 +906,17 @@ def main():
     extra["insert_op_model_gbs"] = op_bytes / float(np.mean(ti)) / 1e9
 
     # ------------------------- a growing sequence that crosses rebuilds
-    # fresh 1e6-edge batches inserted one after another (no removes): rows
-    # outgrow their slots and move to the pool tail, and the pool is
-    # compacted or grown inside whichever insert crosses the threshold; the
-    # per-batch latency distribution shows what those inserts cost
+    # fresh batches of the headline size inserted one after another (no
+    # removes): rows outgrow their slots and move to the pool tail, and the
+    # pool is compacted or grown inside whichever insert crosses the
+    # threshold; the per-batch latency distribution shows what those
+    # inserts cost
     ds.free()
     dd.free()
     seq_ms, seq_rebuild = [], []
     r0 = g.last_update_stats()["rebuilds"]
     for i in range(args.grow_batches):
-        gs, gd = S.rmat(n, 5000 + i, 1_000_000, device=True,
-                        normalized=RMAT_FLAGS != 0)
+        gs, gd = S.rmat(n, 5000 + i, B, device=True, normalized=RMAT_FLAGS != 0)
         S.synchronize()
         seq_ms.append(1e3 * timed(lambda: g.insert_batch(gs, gd)))
         seq_rebuild.append(g.last_update_stats()["rebuilds"])
@@ -926,12 +927,12 @@ def main():
         crossed = [i for i in range(len(seq_rebuild))
                    if seq_rebuild[i] != (seq_rebuild[i - 1] if i else r0)]
         extra["growing_sequence"] = dict(
-            batches=len(sq), batch_edges=1_000_000,
+            batches=len(sq), batch_edges=B,
             p50_ms=float(np.percentile(sq, 50)), p99_ms=float(np.percentile(sq, 99)),
             max_ms=float(sq.max()), rebuilds=int(seq_rebuild[-1] - r0),
             rebuild_batches_ms=[float(sq[i]) for i in crossed],
             m_end=g.M())
-        log(f"growing sequence: {len(sq)} x 1e6 inserts, p50 {np.percentile(sq, 50):.3f} ms "
+        log(f"growing sequence: {len(sq)} x {B} inserts, p50 {np.percentile(sq, 50):.3f} ms "
             f"p99 {np.percentile(sq, 99):.3f} ms max {sq.max():.3f} ms, "
             f"{seq_rebuild[-1] - r0} rebuilds")
 

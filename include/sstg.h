@@ -258,10 +258,10 @@ int sstg_rmat(uint32_t n, uint32_t seed, double a, double b, double c,
 /* at most one PageRank or TC at a time; lighter queries pass a waiting one */
 #define SSTG_SERVE_ONE_HEAVY 4u
 /* the triangle count is kept through updates while every edge has its
- * reverse: the first TC query counts in full, each later update run moves
- * the count by the triangles its changed edges close (insert) or break
- * (remove), and a TC query on a version whose count is known is answered
- * from it without a device pass.  Results are those of a full count. */
+ * reverse: each update run is logged with the triangles its changed edges
+ * close (insert) or break (remove); once one TC query has counted a logged
+ * version in full, a TC query on any logged version is answered from the
+ * log without a device pass.  Results are those of a full count. */
 #define SSTG_SERVE_MAINTAIN 8u
 #define SSTG_REQ_INSERT 1u /* insert_batch (SM:556-643) */
 #define SSTG_REQ_REMOVE 2u /* remove_batch (SM:645-718) */

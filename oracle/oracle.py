@@ -62,8 +62,8 @@ def lib():
         "orc_bfs": (None, [u32, _u64p, _u32p, u32, _u32p, _i32p]),
         "orc_pagerank": (None, [u32, _u64p, _u32p, u32, f64, _f64p]),
         "orc_bc": (None, [u32, _u64p, _u32p, u32, _f64p]),
-        "orc_bfs_par": (None, [u32, _u64p, _u32p, u32, _u32p]),
-        "orc_bc_par": (None, [u32, _u64p, _u32p, u32, _f64p]),
+        "orc_bfs_par": (None, [u32, _u64p, _u32p, u32, ctypes.c_int, _u32p]),
+        "orc_bc_par": (None, [u32, _u64p, _u32p, u32, ctypes.c_int, _f64p]),
         "orc_cc": (None, [u32, _u64p, _u32p, _u32p]),
         "orc_common_neighbors": (u64, [_u64p, _u32p, u32, u32, ctypes.c_int]),
         "orc_tc": (u64, [u32, _u64p, _u32p]),
@@ -198,21 +198,24 @@ def bfs(off, cols, src):
     return level, parent
 
 
-def bfs_par(off, cols, src):
-    """BFS levels on every host core (the timed baseline; equals bfs)."""
+def bfs_par(off, cols, src, symmetric=None):
+    """BFS levels on every host core (the timed baseline; equals bfs).
+    symmetric: whether every edge has its reverse (None: checked first)."""
     off, cols = _csr_args(off, cols)
     n = len(off) - 1
     level = np.empty(n, dtype=np.uint32)
-    lib().orc_bfs_par(n, off, cols, src, level)
+    lib().orc_bfs_par(n, off, cols, src, -1 if symmetric is None else int(symmetric),
+                      level)
     return level
 
 
-def bc_par(off, cols, src):
+def bc_par(off, cols, src, symmetric=None):
     """Brandes dependencies on every host core (the timed baseline)."""
     off, cols = _csr_args(off, cols)
     n = len(off) - 1
     out = np.empty(n, dtype=np.float64)
-    lib().orc_bc_par(n, off, cols, src, out)
+    lib().orc_bc_par(n, off, cols, src, -1 if symmetric is None else int(symmetric),
+                     out)
     return out
 
 

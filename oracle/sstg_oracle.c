@@ -602,17 +602,23 @@ void orc_bc(uint32_t n, const uint64_t *off, const uint32_t *cols,
  * reference's fork-join build would (its BFS and BC come from Ligra-style
  * EdgeMapVertexMap, TST:486 and TST:547).  Level-synchronous; a level is a
  * top-down step over the frontier (claims by compare-and-swap) or, on
- * symmetric input with a frontier whose edges exceed the unexplored edges
- * / 15, a bottom-up step in which every unvisited vertex looks for a
- * neighbour in the frontier.  Levels are unique, so they equal orc_bfs's. */
+ * symmetric input, a bottom-up step in which every unvisited vertex looks
+ * for a neighbour in the frontier, taken with Beamer's switch (a growing
+ * frontier whose edges exceed the unexplored edges / 15; back to top-down
+ * below n / 18 frontier vertices).  Levels are unique, so they equal
+ * orc_bfs's. */
 static uint32_t bfs_levels_par(uint32_t n, const uint64_t *off,
                                const uint32_t *cols, uint32_t src, int sym,
                                uint32_t *lv, uint32_t *order) {
+  /* a top-down step after a bottom-up one reads the frontier from order[]
+   * (the bottom-up step lists it there in ascending id order) */
   for (int64_t i = 0; i < (int64_t)n; i++) lv[i] = UINT32_MAX;
   if (src >= n) return 0;
   lv[src] = 0;
   order[0] = src;
   uint64_t head = 0, tail = 1, m = off[n], explored = off[src + 1] - off[src];
+  uint64_t prev_nf = 0;
+  int in_bu = 0;
   uint32_t L = 0;
   int T = 1;
 #ifdef _OPENMP
@@ -625,7 +631,14 @@ static uint32_t bfs_levels_par(uint32_t n, const uint64_t *off,
     for (int64_t i = (int64_t)head; i < (int64_t)tail; i++)
       fe += off[order[i] + 1] - off[order[i]];
     const uint64_t unexplored = m > explored ? m - explored : 0;
-    const int bottom_up = sym && (double)fe * 15.0 > (double)unexplored;
+    const uint64_t nf = tail - head;
+    /* Beamer's switch: bottom-up once a growing frontier's edges exceed the
+     * unexplored edges / 15, back to top-down below n / 18 vertices */
+    const int bottom_up =
+        sym && (in_bu ? (double)nf * 18.0 >= (double)n
+                      : nf > prev_nf && (double)fe * 15.0 > (double)unexplored);
+    in_bu = bottom_up;
+    prev_nf = nf;
     if (bottom_up) {
 #pragma omp parallel for schedule(dynamic, 1024)
       for (int64_t v = 0; v < (int64_t)n; v++) {
@@ -711,10 +724,13 @@ static int csr_symmetric(uint32_t n, const uint64_t *off, const uint32_t *cols) 
   return ok;
 }
 
+/* symmetric: every edge has its reverse (< 0: check it here, outside the
+ * traversal proper) */
 void orc_bfs_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
-                 uint32_t src, uint32_t *level) {
+                 uint32_t src, int symmetric, uint32_t *level) {
+  if (symmetric < 0) symmetric = csr_symmetric(n, off, cols);
   uint32_t *order = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
-  bfs_levels_par(n, off, cols, src, csr_symmetric(n, off, cols), level, order);
+  bfs_levels_par(n, off, cols, src, symmetric, level, order);
   free(order);
 }
 
@@ -724,8 +740,9 @@ void orc_bfs_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
  * and down), in parallel within a level.  Directed input falls back to
  * orc_bc. */
 void orc_bc_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
-                uint32_t src, double *out) {
-  if (!csr_symmetric(n, off, cols)) {
+                uint32_t src, int symmetric, double *out) {
+  if (symmetric < 0) symmetric = csr_symmetric(n, off, cols);
+  if (!symmetric) {
     orc_bc(n, off, cols, src, out);
     return;
   }

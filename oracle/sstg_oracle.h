@@ -76,11 +76,12 @@ void orc_pagerank(uint32_t n, const uint64_t *off, const uint32_t *cols,
 /* Single-source Brandes dependency delta_s(v), FP64 (test.hpp:547). */
 void orc_bc(uint32_t n, const uint64_t *off, const uint32_t *cols,
             uint32_t src, double *out);
-/* All-core CPU baselines: the same levels / dependencies with OpenMP. */
+/* All-core CPU baselines: the same levels / dependencies with OpenMP.
+ * symmetric: every edge has its reverse (< 0: checked first). */
 void orc_bfs_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
-                 uint32_t src, uint32_t *level);
+                 uint32_t src, int symmetric, uint32_t *level);
 void orc_bc_par(uint32_t n, const uint64_t *off, const uint32_t *cols,
-                uint32_t src, double *out);
+                uint32_t src, int symmetric, double *out);
 /* Connected components, label = minimum vertex id of the component
  * (test.hpp:571-601 only uses the partition). */
 void orc_cc(uint32_t n, const uint64_t *off, const uint32_t *cols,

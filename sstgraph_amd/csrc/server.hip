@@ -133,14 +133,17 @@ struct sstg_server {
   uint64_t next_ticket = 1;
   uint64_t running = 0;  // queries on slots
   uint64_t device_batches = 0;
-  // SSTG_SERVE_MAINTAIN: the triangle count of store version tc_version,
-  // carried through updates by the triangles their changed edges close or
-  // break
+  // SSTG_SERVE_MAINTAIN: while every edge has its reverse, the writer logs
+  // for each store version the triangles closed minus broken since the
+  // chain began (tc_chain: version -> cumulative delta); one full count of
+  // any version in the chain (the anchor) then gives the count of every
+  // version in it
   int maintain = 0;
-  bool tc_known = false;
-  uint64_t tc_count = 0, tc_version = 0;
-  uint64_t tc_from_count = 0;   // TC queries answered from the count
-  uint64_t tc_carried = 0;      // update runs folded into the count
+  std::map<uint64_t, int64_t> tc_chain;
+  bool tc_anchor = false;
+  uint64_t tc_anchor_count = 0;
+  int64_t tc_anchor_delta = 0;
+  uint64_t tc_from_count = 0;   // TC queries answered from the chain
 };
 
 namespace sstg {
@@ -395,13 +398,13 @@ int run_query(sstg_server *S, sstg_graph *v, Request *r) {
     case SSTG_REQ_TC: {
       SSTG_TRY(algo_tc(v, &r->scalar));
       if (S->maintain) {
-        // the count starts from a full recount of the live version
+        // a full count of a version in the chain anchors it
         std::lock_guard<std::mutex> lk(S->mu);
-        if (!S->writing && v->version == S->g->version &&
-            S->g->sym_version == S->g->version && S->g->symmetric) {
-          S->tc_known = true;
-          S->tc_count = r->scalar;
-          S->tc_version = v->version;
+        auto it = S->tc_chain.find(v->version);
+        if (it != S->tc_chain.end() && !S->tc_anchor) {
+          S->tc_anchor = true;
+          S->tc_anchor_count = r->scalar;
+          S->tc_anchor_delta = it->second;
         }
       }
       return SSTG_OK;
@@ -502,15 +505,15 @@ void start_updates(sstg_server *S, size_t at) {
   S->cv.notify_all();
 }
 
-// Applies an update run to the served graph and carries its triangle count
-// (every edge with its reverse before and after): the pairs the run changes
-// are found first; a removal's broken triangles are counted on the graph
-// before it, an insertion's closed ones on the graph after it.  *carried is
-// false when the count could not be carried (then the run was applied all
-// the same, or not at all if an id was out of range).
+// Applies an update run to the served graph and carries the cumulative
+// triangle delta count0 (every edge with its reverse before the run): the
+// pairs the run changes are found first; a removal's broken triangles are
+// counted on the graph before it, an insertion's closed ones on the graph
+// after it.  *carried is false when the delta could not be carried (then the
+// run was applied all the same, or not at all if an id was out of range).
 int update_carrying_tc(sstg_graph *g, const std::vector<uint32_t> &src,
                        const std::vector<uint32_t> &dst, bool ins,
-                       uint64_t count0, uint64_t *count1, bool *carried) {
+                       int64_t count0, int64_t *count1, bool *carried) {
   *carried = false;
   const uint64_t n = src.size();
   if (n >= (1ull << 31))
@@ -531,7 +534,7 @@ int update_carrying_tc(sstg_graph *g, const std::vector<uint32_t> &src,
   if (!ins) SSTG_TRY(tc_delta(g, keys, count, &delta));
   SSTG_TRY(graph_update(g, ds, dd, n, ins, ins, true));
   if (ins) SSTG_TRY(tc_delta(g, keys, count, &delta));
-  *count1 = ins ? count0 + delta : count0 - delta;
+  *count1 = ins ? count0 + (int64_t)delta : count0 - (int64_t)delta;
   *carried = true;
   return SSTG_OK;
 }
@@ -546,17 +549,17 @@ void writer_main(sstg_server *S) {
     const uint32_t kind = run.front()->spec.kind;
     const bool ins = kind == SSTG_REQ_INSERT;
     sstg_graph *g = S->g;
-    const bool carry = S->maintain && S->tc_known &&
-                       S->tc_version == g->version &&
+    auto link = S->tc_chain.find(g->version);
+    const bool carry = S->maintain && link != S->tc_chain.end() &&
                        g->sym_version == g->version && g->symmetric;
-    const uint64_t count0 = S->tc_count;
+    const int64_t delta0 = carry ? link->second : 0;
     lk.unlock();
     int rc = SSTG_OK;
     bool carried = false;
-    uint64_t count1 = 0;
+    int64_t delta1 = 0;
     if (carry) {
-      rc = update_carrying_tc(g, S->write_src, S->write_dst, ins, count0,
-                              &count1, &carried);
+      rc = update_carrying_tc(g, S->write_src, S->write_dst, ins, delta0,
+                              &delta1, &carried);
     } else {
       rc = graph_update(g, S->write_src.data(), S->write_dst.data(),
                         S->write_src.size(), ins, ins, false);
@@ -569,13 +572,18 @@ void writer_main(sstg_server *S) {
                               run[i]->src.size(), ins, ins, false);
     }
     lk.lock();
-    if (carried && rc == SSTG_OK && g->sym_version == g->version &&
-        g->symmetric) {
-      S->tc_count = count1;
-      S->tc_version = g->version;
-      S->tc_carried++;
-    } else {
-      S->tc_known = false;
+    if (S->maintain) {
+      const bool sym = g->sym_version == g->version && g->symmetric;
+      if (!(carried && rc == SSTG_OK && sym)) {
+        // the chain breaks; a symmetric graph starts a new one, whose
+        // versions a later full count anchors again
+        S->tc_chain.clear();
+        S->tc_anchor = false;
+        if (sym) S->tc_chain[g->version] = 0;
+      } else {
+        S->tc_chain[g->version] = delta1;
+        while (S->tc_chain.size() > 4096) S->tc_chain.erase(S->tc_chain.begin());
+      }
     }
     const double t1 = now_s();
     for (size_t i = 0; i < run.size(); i++) {
@@ -645,14 +653,17 @@ bool sched_step(sstg_server *S) {
       }
       return changed;  // later queries must see this update
     }
-    if (r->spec.kind == SSTG_REQ_TC && S->tc_known &&
-        (r->set ? r->set->version == S->tc_version
-                : !S->writing && S->g->version == S->tc_version)) {
-      // the maintained count of the version this query reads
+    std::map<uint64_t, int64_t>::const_iterator link;
+    if (r->spec.kind == SSTG_REQ_TC && S->tc_anchor &&
+        (r->set || !S->writing) &&
+        (link = S->tc_chain.find(r->set ? r->set->version : S->g->version)) !=
+            S->tc_chain.end()) {
+      // the count of the version this query reads, from the chain
       S->pending.erase(S->pending.begin() + i);
       i--;
       r->t_start = r->t_done = now_s();
-      r->scalar = S->tc_count;
+      r->scalar = (uint64_t)((int64_t)S->tc_anchor_count + link->second -
+                             S->tc_anchor_delta);
       r->status = SSTG_OK;
       r->done = true;
       if (r->set) {
@@ -741,6 +752,7 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
   S->snapshots = (flags & SSTG_SERVE_SNAPSHOTS) ? 1 : 0;
   S->one_heavy = (flags & SSTG_SERVE_ONE_HEAVY) ? 1 : 0;
   S->maintain = (flags & SSTG_SERVE_MAINTAIN) ? 1 : 0;
+  if (S->maintain && sym) S->tc_chain[g->version] = 0;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
     delete S;
