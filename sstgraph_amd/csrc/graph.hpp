@@ -74,6 +74,24 @@ struct sstg_graph {
 
 namespace sstg {
 
+// Slot capacity of a row of d ids: the next power of two of at least 4, but
+// never more than the row could ever hold (n_cols) when d fits in that.
+__host__ __device__ inline uint64_t slot_capacity(uint64_t d, uint32_t n_cols) {
+  uint64_t c = 4;
+  while (c < d) c <<= 1;
+  const uint64_t lim = n_cols < 4 ? 4 : n_cols;
+  if (c > lim && d <= lim) c = lim;
+  return c;
+}
+
+// update_small.hip: a whole update of at most kSmallBatch pairs (device
+// arrays) in one workgroup.  *fallback is set, with the graph unchanged,
+// when the batch needs more pool room than is left or moves a very long
+// row; the caller then runs the general pipeline.
+constexpr uint32_t kSmallBatch = 4096;
+int update_small(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
+                 uint32_t n, bool insert, bool skip_self, bool *fallback);
+
 // update.hip
 int graph_alloc(sstg_graph *g);
 void graph_free(sstg_graph *g);
@@ -184,6 +202,10 @@ enum : int {
   kWsPrSeg = 130,     // 130..142 FP64 PageRank column-segment layout
   kWsEffKeys = 143,   // the pairs an update changes (graph_effective_keys)
   kWsTcDelta = 144,   // per-block partial counts of tc_delta
+  kWsSmallKeys = 145, // small updates: the plan (update_small.hip)
+  kWsSmallPos = 146,
+  kWsSmallRows = 147,
+  kWsSmallAlloc = 148,
 };
 
 // degree threshold sets of a graph (sstg_graph::bin_hi): the reductions

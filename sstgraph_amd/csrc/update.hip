@@ -55,16 +55,14 @@ enum {
 // merged in place by one block through LDS; larger rows are staged.  A block
 // owns the rows whose inputs start in its kTile range, so its tile holds at
 // most kTile + kTileMax inputs.
+// slots (live elements of capacity) below which removal never compacts
+constexpr uint64_t kShrinkMinElems = 1ull << 20;
 constexpr int kTile = 1536;
 constexpr int kTileMax = 1536;
 constexpr int kTileLds = kTile + kTileMax;
 
 __device__ __forceinline__ uint64_t grow_cap(uint64_t d, uint32_t n_cols) {
-  uint64_t c = 4;
-  while (c < d) c <<= 1;
-  uint64_t lim = n_cols < 4 ? 4 : n_cols;
-  if (c > lim && d <= lim) c = lim;
-  return c;
+  return slot_capacity(d, n_cols);
 }
 
 __global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n,
@@ -1015,6 +1013,15 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   // pool, compact before taking more space
   if (insert && !g->noalloc && g->garbage > 0 && g->garbage * 2 > g->pool_tail)
     SSTG_TRY(rebuild_impl(g, 0, true));
+  // a batch of a few thousand pairs in two launches and one host wait
+  // (update_small.hip), unless it needs the pool to grow or moves a very
+  // long row
+  if (n <= kSmallBatch) {
+    bool fallback = false;
+    SSTG_TRY(update_small(g, src, dst, (uint32_t)n, insert, skip_self,
+                          &fallback));
+    if (!fallback) return SSTG_OK;
+  }
 
   uint64_t *keys = nullptr, *tmp = nullptr, *B = nullptr, *seg_idx = nullptr;
   uint8_t *flags = nullptr;
@@ -1294,6 +1301,13 @@ int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
     }
     SSTG_TRY(update_chunk(g, s, d, len, insert, skip_self));
   }
+  // density rule on removal (the packed-memory array's lower bound, as
+  // PMA.hpp:686-702 halves a list): once the live ids fill less than a
+  // quarter of the live slots, compact the pool with every slot reset to the
+  // capacity of its row, which gives the memory back
+  if (!insert && !g->noalloc && g->pool_tail - g->garbage > kShrinkMinElems &&
+      g->m * 4 < g->pool_tail - g->garbage)
+    SSTG_TRY(rebuild_impl(g, 0, true));
   g->last.rebuilds = g->rebuilds;
   return SSTG_OK;
 }

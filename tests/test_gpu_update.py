@@ -89,6 +89,9 @@ def test_many_big_rows_bit_exact(rows):
     og.insert_batch(r.astype(np.uint32), c.astype(np.uint32))
     bs = np.arange(rows, dtype=np.uint32)
     bd = ((bs + 1) % 1000).astype(np.uint32)  # below every stored id, never bs
+    # each pair twice: the batch exceeds the small-batch path's 4096 pairs,
+    # so the general pipeline's big-row merge runs for every row count
+    bs, bd = np.concatenate([bs, bs]), np.concatenate([bd, bd])
     say("big rows", rows)
     g.insert_batch(bs, bd)
     og.insert_batch(bs, bd)
@@ -248,3 +251,31 @@ def test_scale22_update_batches_bit_exact():
         assert_same_csr(g, og, f"scale 22 remove {bsz}")
     say("paths taken", seen)
     assert seen["big_rows"] > 0 and seen["allocated"] > 0 and seen["tile_rows"] > 0
+
+
+def test_heavy_removal_compacts_the_pool():
+    """Removing most of a graph's edges (live ids below a quarter of the
+    live slots) compacts the pool and shrinks every slot to its row, as the
+    packed-memory array's lower density bound does; contents stay exact and
+    later inserts grow rows again."""
+    n = 1 << 16
+    rng = np.random.default_rng(21)
+    s = rng.integers(0, n, 3_000_000).astype(np.uint32)
+    d = rng.integers(0, n, 3_000_000).astype(np.uint32)
+    g, og = S.SparseMatrixV(n), O.Graph(n)
+    g.insert_batch(s, d)
+    og.insert_batch(s, d)
+    before = g.statistics()
+    keep = rng.random(len(s)) < 0.1
+    g.remove_batch(s[~keep], d[~keep])
+    og.remove_batch(s[~keep], d[~keep])
+    after = g.statistics()
+    say("pool capacity", before["pool_capacity"], "->", after["pool_capacity"],
+        "rebuilds", before["rebuilds"], "->", after["rebuilds"])
+    assert after["rebuilds"] > before["rebuilds"]
+    assert after["pool_capacity"] < before["pool_capacity"] // 2
+    assert after["slot_capacity"] <= 4 * max(after["edges"], 1) + 4 * n
+    assert_same_csr(g, og, "after compaction")
+    g.insert_batch(s[:200000], d[:200000])
+    og.insert_batch(s[:200000], d[:200000])
+    assert_same_csr(g, og, "regrown")
