@@ -18,7 +18,7 @@ reported as GTEPS under "extra".
     python bench.py --workload batched   # 512 graphs, one hipGraph replay
     python bench.py --workload serve     # served request stream, p50/p99
     python bench.py --workload algos     # BFS, PR, CC, BC, TC at scale 22
-    python bench.py --workload pagerank  # (or bfs) one algorithm, for profiles
+    python bench.py --workload pagerank  # (or bfs, bc) one algorithm, for profiles
 """
 import argparse
 import ctypes
@@ -524,6 +524,16 @@ def run_algo(args):
         S.synchronize()
         extra["first_call_ms"] = 1e3 * timed(lambda: (step(), S.synchronize()))
         extra["segment_bits"] = g.pagerank_segments()
+    elif args.workload == "bc":
+        hub = int(np.argmax(deg))
+        d_bc = S.DeviceArray(n, np.float64)
+        L = S.lib()
+        step = lambda: S._check(L.sstg_bc(g.handle, hub, d_bc.ptr, S.SSTG_DEVICE_PTRS), "bc")
+        d_lev = S.DeviceArray(n, np.uint32)
+        EMV.BFS(g, hub, out=(None, d_lev))
+        lev = d_lev.to_host()
+        work = int(deg[lev != S.UNREACHED].sum())
+        name = f"BC from vertex {hub}"
     else:
         hub = int(np.argmax(deg))
         d_par = S.DeviceArray(n, np.int32)
@@ -546,7 +556,7 @@ def run_algo(args):
         "metric": f"{name} traversed edges/s", "value": work / t, "unit": "edges/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * t, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if args.workload == "pagerank" else "u32",
+        "vs_baseline": None, "dtype": "f64" if args.workload in ("pagerank", "bc") else "u32",
         "data": "synthetic (rMat generator of the reference, seeded; " + RMAT_NOTE[RMAT_FLAGS != 0] + ")",
         "config": {"workload": f"{name} on rMat scale {args.scale} (seed 2, edge "
                                f"factor {args.edge_factor}, symmetrised)",
@@ -675,7 +685,7 @@ def main():
                     help="rMat draws in [0,1) (probabilities 0.5/0.1/0.1/0.3) "
                          "instead of the reference's [0,2)")
     ap.add_argument("--workload", choices=["updates", "batched", "serve", "pagerank", "bfs",
-                                           "algos"],
+                                           "bc", "algos"],
                     default="updates")
     ap.add_argument("--pr-segments", type=int, default=None,
                     help="PageRank column-segment width in log2 vertices "
@@ -700,7 +710,7 @@ def main():
         return run_batched(args)
     if args.workload == "serve":
         return run_serve(args)
-    if args.workload in ("pagerank", "bfs"):
+    if args.workload in ("pagerank", "bfs", "bc"):
         return run_algo(args)
     if args.workload == "algos":
         return run_c2(args)

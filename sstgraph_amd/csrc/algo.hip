@@ -237,6 +237,84 @@ struct DeltaPull {
   }
 };
 
+// Brandes pulls over one BFS level's vertices with the levels as bytes (a
+// graph of fewer than 64 levels): a 16 MB level array at rMat scale 24
+// instead of 64 MB, so the per-edge level gather mostly hits the cache;
+// sigma and delta are gathered only for the parents (forward) and children
+// (backward) it selects.
+struct SigmaPull8 {
+  const uint8_t *lev;
+  double *sigma;
+  uint32_t L;  // level of the vertices being computed
+  using Acc = double;
+  __device__ __forceinline__ Acc zero() const { return 0.0; }
+  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t u) const {
+    if ((uint32_t)lev[u] + 1 == L) a += sigma[u];
+  }
+  __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
+    return __shfl_xor(a, o, kWave);
+  }
+  __device__ __forceinline__ void store(uint32_t v, Acc s) const {
+    sigma[v] = s;
+  }
+};
+
+// delta(v) = sigma(v) * sum over children w of (1 + delta(w)) / sigma(w)
+struct DeltaPull8 {
+  const uint8_t *lev;
+  const double *sigma;
+  double *delta;
+  uint32_t L;
+  using Acc = double;
+  __device__ __forceinline__ Acc zero() const { return 0.0; }
+  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t w) const {
+    if ((uint32_t)lev[w] == L + 1) a += (1.0 + delta[w]) / sigma[w];
+  }
+  __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
+  __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
+    return __shfl_xor(a, o, kWave);
+  }
+  __device__ __forceinline__ void store(uint32_t v, Acc s) const {
+    delta[v] = sigma[v] * s;
+  }
+};
+
+__global__ void k_level_bytes(const uint32_t *level, uint32_t n, uint8_t *lev) {
+  const uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) lev[v] = level[v] < 255u ? (uint8_t)level[v] : (uint8_t)255;
+}
+
+// Sort key of the i-th reached vertex: (level * 4 + lane class) << 32 | v,
+// class 0: degree <= t1 (4 lanes), 1: <= t2 (16 lanes), 2: larger (a block)
+__global__ void k_bc_keys(const uint32_t *order, uint64_t count,
+                          const uint32_t *level, const uint32_t *deg,
+                          uint32_t t1, uint32_t t2, uint64_t *key) {
+  const uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t v = order[i], d = deg[v];
+  const uint32_t c = d <= t1 ? 0u : d <= t2 ? 1u : 2u;
+  key[i] = ((uint64_t)(level[v] * 4u + c) << 32) | v;
+}
+
+// first index of each (level, class) bucket in the sorted keys (lower bound
+// of bucket << 32), buckets 0 .. nb
+__global__ void k_bc_bucket_starts(const uint64_t *sorted, uint64_t count,
+                                   uint32_t nb, uint64_t *start) {
+  const uint32_t b = blockIdx.x * kT + threadIdx.x;
+  if (b > nb) return;
+  const uint64_t want = (uint64_t)b << 32;
+  uint64_t lo = 0, hi = count;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (sorted[mid] < want)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  start[b] = lo;
+}
+
 // sigma push for directed graphs: every vertex v of level L-1 adds sigma(v)
 // to its out-neighbours on level L (integer-valued sums, exact below 2^53).
 __global__ void k_sigma_push(DevGraph g, const uint32_t *list, uint64_t count,
@@ -617,7 +695,7 @@ static int pagerank_impl(sstg_graph *g, uint32_t iters, double damping,
   // pairs of iterations replayed from one recording (contributions back in
   // their first buffer after two); an odd last iteration launched plainly
   uint32_t plain = iters;
-  if (iters >= 2 && iteration_capture_enabled()) {
+  if (iters >= 2 && iteration_capture_enabled(g)) {
     IterPairKey key{};
     key.path = kKahan ? 2 : 1;
     key.version = g->version;
@@ -737,6 +815,68 @@ int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
 }
 
 // ====================================================================== BC
+// Brandes from src on a graph of fewer than 64 BFS levels: the reached
+// vertices grouped by (level, lane class) with one radix pass, so each
+// level's pulls run over exactly its vertices (4, 16 lanes or a block by
+// degree) instead of whole degree bins filtered by level.
+static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
+                      const uint32_t *order, const BfsRun &run, double *sigma,
+                      double *out) {
+  hipStream_t s = g->stream;
+  const uint32_t n = g->n_rows;
+  const uint64_t reached = run.reached;
+  const uint32_t nb = run.levels * 4;
+  uint64_t *key = nullptr, *tmp = nullptr, *start = nullptr;
+  uint8_t *lev = nullptr;
+  SSTG_TRY(g->ws.get(kWsAlgo + 16, reached + 1, &key));
+  SSTG_TRY(g->ws.get(kWsAlgo + 17, reached + 1, &tmp));
+  SSTG_TRY(g->ws.get(kWsAlgo + 18, (uint64_t)nb + 1, &start));
+  SSTG_TRY(g->ws.get(kWsAlgo + 19, n, &lev));
+  const uint32_t t1 = g->bin_hi[kBinsPull][1], t2 = g->bin_hi[kBinsPull][2];
+  SSTG_LAUNCH(k_level_bytes, blocks(n), kT, 0, s, level, n, lev);
+  SSTG_LAUNCH(k_bc_keys, blocks(reached), kT, 0, s, order, reached, level,
+              (const uint32_t *)g->deg, t1, t2, key);
+  uint64_t *sorted = key;
+  const int shifts[1] = {32};
+  SSTG_TRY(radix_sort_u64(key, tmp, reached, shifts, 1, g->ws, kWsPrim, s,
+                          &sorted));
+  SSTG_LAUNCH(k_bc_bucket_starts, ceil_div(nb + 1, kT), kT, 0, s,
+              (const uint64_t *)sorted, reached, nb, start);
+  // the vertex ids of the sorted keys, as u32 over the other buffer
+  uint32_t *list = reinterpret_cast<uint32_t *>(sorted == key ? tmp : key);
+  SSTG_TRY(narrow_u64_to_u32(sorted, reached, list, s));
+  std::vector<uint64_t> st(nb + 1);
+  SSTG_HIP(hipMemcpyAsync(st.data(), start, (nb + 1) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipStreamSynchronize(s));
+  const DevGraph v = g->view();
+  auto pull = [&](uint32_t L, auto f) -> int {
+    const uint64_t b0 = st[L * 4], b1 = st[L * 4 + 1], b2 = st[L * 4 + 2],
+                   b3 = st[L * 4 + 3];
+    if (b1 > b0)
+      SSTG_LAUNCH((em::k_vertex_pull<decltype(f), 4>), ceil_div((b1 - b0) * 4, kT),
+                  kT, 0, s, v, (const uint32_t *)(list + b0), b1 - b0, f);
+    if (b2 > b1)
+      SSTG_LAUNCH((em::k_vertex_pull<decltype(f), 16>),
+                  ceil_div((b2 - b1) * 16, kT), kT, 0, s, v,
+                  (const uint32_t *)(list + b1), b2 - b1, f);
+    if (b3 > b2)
+      SSTG_LAUNCH((em::k_vertex_pull<decltype(f), 256>), b3 - b2, kT, 0, s, v,
+                  (const uint32_t *)(list + b2), b3 - b2, f);
+    return SSTG_OK;
+  };
+  SSTG_LAUNCH(k_bc_init, blocks(n), kT, 0, s, n, sigma, out, src);
+  // forward: sigma from the parents, level by level
+  for (uint32_t L = 1; L < run.levels; L++)
+    SSTG_TRY(pull(L, SigmaPull8{lev, sigma, L}));
+  // backward: dependencies from the children, the deepest level (no
+  // children, delta 0) excluded
+  for (int64_t L = (int64_t)run.levels - 2; L >= 0; L--)
+    SSTG_TRY(pull((uint32_t)L, DeltaPull8{lev, sigma, out, (uint32_t)L}));
+  SSTG_HIP(hipStreamSynchronize(s));
+  return SSTG_OK;
+}
+
 int algo_bc(sstg_graph *g, uint32_t src, double *out) {
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
@@ -747,11 +887,13 @@ int algo_bc(sstg_graph *g, uint32_t src, double *out) {
   SSTG_TRY(g->ws.get(kWsAlgo + 15, n, &sigma));
   BfsRun run;
   SSTG_TRY(bfs_core(g, src, level, nullptr, order, false, &run));
+  bool symmetric = false;
+  SSTG_TRY(graph_is_symmetric(g, &symmetric));
+  if (symmetric && run.levels < 64)
+    return bc_leveled(g, src, level, order, run, sigma, out);
   SSTG_LAUNCH(k_bc_init, blocks(n), kT, 0, s, n, sigma, out, src);
   // forward: sigma level by level (level 0 is the source, sigma = 1); pulled
   // from the parents on symmetric graphs, pushed along out-edges otherwise
-  bool symmetric = false;
-  SSTG_TRY(graph_is_symmetric(g, &symmetric));
   for (uint32_t L = 1; L < run.levels; L++) {
     if (symmetric) {
       uint64_t b = run.level_start[L], e = run.level_start[L + 1];

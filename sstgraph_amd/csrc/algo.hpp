@@ -16,8 +16,11 @@ int algo_pagerank(sstg_graph *g, uint32_t iters, double damping, double *out,
 bool pagerank_segmented_applies(const sstg_graph *g);
 // A pair of PageRank iterations recorded into a hipGraph and replayed
 // (pagerank.hip): the graph keeps one recording and its key, and records
-// again when the key changes.  The environment variable
-// SSTG_NO_GRAPH_CAPTURE makes every iteration launch its kernels plainly.
+// again when the key changes.  Opt-in, with the environment variable
+// SSTG_GRAPH_CAPTURE=1 (read at each call): while a stream captures, HIP
+// refuses synchronous copies on the legacy stream from any other thread,
+// and the replay saves 0.2% of a scale-24 PageRank.  Never on a server
+// slot's view (g->no_capture).
 struct IterPairKey {
   uint32_t path;     // which PageRank: 1 vertex pull FP64, 2 FP32 + Kahan,
                      // 3 column segments
@@ -27,7 +30,7 @@ struct IterPairKey {
   const void *ptr[10];
   double damping;
 };
-bool iteration_capture_enabled();
+bool iteration_capture_enabled(const sstg_graph *g);
 int replay_iteration_pairs(sstg_graph *g, const IterPairKey &key,
                            uint32_t pairs, const std::function<int()> &record);
 int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,

@@ -54,8 +54,10 @@ struct sstg_graph {
   uint64_t prseg_runs = 0;
   uint64_t prseg_cls[4] = {};
   // a recorded pair of PageRank iterations (hipGraph_t / hipGraphExec_t)
-  // and what it was recorded for
+  // and what it was recorded for; no_capture: never record (server views,
+  // whose threads share the device with synchronous copies)
   void *pr_graph = nullptr, *pr_exec = nullptr, *pr_key = nullptr;
+  bool no_capture = false;
   // cached: does every stored edge have its reverse?
   uint64_t sym_version = ~0ull;
   bool symmetric = false;

@@ -366,9 +366,10 @@ void pagerank_release(sstg_graph *g) {
   g->pr_key = nullptr;
 }
 
-bool iteration_capture_enabled() {
-  static const bool off = std::getenv("SSTG_NO_GRAPH_CAPTURE") != nullptr;
-  return !off;
+bool iteration_capture_enabled(const sstg_graph *g) {
+  // read at every call, so a caller can switch it around one call
+  const char *on = std::getenv("SSTG_GRAPH_CAPTURE");
+  return !g->no_capture && on && on[0] == '1';
 }
 
 // The graph keeps one recorded pair of iterations: it is recorded again
@@ -435,7 +436,7 @@ int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,
   SSTG_LAUNCH(k_pr_seg_init, blocks_for(n), kT, 0, s, n,
               (const uint32_t *)g->deg, x0, out, contrib);
   uint32_t plain = iters;
-  if (iters >= 2 && iteration_capture_enabled()) {
+  if (iters >= 2 && iteration_capture_enabled(g)) {
     IterPairKey key{};
     key.path = 3;  // column segments, FP64
     key.version = g->version;

@@ -138,6 +138,8 @@ struct sstg_server {
   // chain began (tc_chain: version -> cumulative delta); one full count of
   // any version in the chain (the anchor) then gives the count of every
   // version in it
+  hipStream_t fetch_stream = nullptr;  // result copies to the host
+  std::mutex fetch_mu;
   int maintain = 0;
   std::map<uint64_t, int64_t> tc_chain;
   bool tc_anchor = false;
@@ -709,6 +711,7 @@ int view_init(sstg_graph *v, const sstg_graph *g) {
   SSTG_HIP(hipHostMalloc(&v->host_rb, 64 * sizeof(uint64_t),
                          hipHostMallocDefault));
   v->version = ~0ull;
+  v->no_capture = true;
   refresh_view(v, g);
   return SSTG_OK;
 }
@@ -752,9 +755,15 @@ int sstg_server_create(sstg_graph *g, uint32_t slots, uint64_t hbm_budget,
   S->snapshots = (flags & SSTG_SERVE_SNAPSHOTS) ? 1 : 0;
   S->one_heavy = (flags & SSTG_SERVE_ONE_HEAVY) ? 1 : 0;
   S->maintain = (flags & SSTG_SERVE_MAINTAIN) ? 1 : 0;
+  if (hipStreamCreateWithFlags(&S->fetch_stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    delete S;
+    return SSTG_E_DEVICE;
+  }
   if (S->maintain && sym) S->tc_chain[g->version] = 0;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    hipStreamDestroy(S->fetch_stream);
     delete S;
     return SSTG_E_DEVICE;
   }
@@ -844,8 +853,12 @@ int sstg_server_fetch(sstg_server *S, uint64_t ticket, void *host_out,
   if (!r->result || !host_out) return SSTG_E_INVAL;
   if (bytes < r->result_bytes) return SSTG_E_INVAL;
   hipSetDevice(S->g->device);
-  SSTG_HIP(hipMemcpy(host_out, r->result, r->result_bytes,
-                     hipMemcpyDeviceToHost));
+  // on the server's own non-blocking stream: a synchronous legacy-stream
+  // copy would wait on (or be refused beside) the slots' streams
+  std::lock_guard<std::mutex> lk(S->fetch_mu);
+  SSTG_HIP(hipMemcpyAsync(host_out, r->result, r->result_bytes,
+                          hipMemcpyDeviceToHost, S->fetch_stream));
+  SSTG_HIP(hipStreamSynchronize(S->fetch_stream));
   return SSTG_OK;
 }
 
@@ -912,6 +925,7 @@ int sstg_server_destroy(sstg_server *S) {
   for (auto &kv : S->spare_results) hipFree(kv.second);
   for (StoreSet &t : S->spare) free_set_arrays(t);
   for (StoreSet &t : S->to_free) free_set_arrays(t);
+  if (S->fetch_stream) hipStreamDestroy(S->fetch_stream);
   delete S->live;  // after the drain no query reads it; never owned
   delete S;
   return SSTG_OK;

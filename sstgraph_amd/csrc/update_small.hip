@@ -42,7 +42,7 @@ constexpr uint32_t kSmMax = kSmallBatch;
 constexpr int kSmItems = kSmMax / kSmT;  // keys per thread
 constexpr int kSmBins = 256;
 constexpr uint64_t kSmMaxMove = 1u << 15;  // longer moves: general pipeline
-constexpr int kSmMergeBlocks = 256;       // workgroups of the merge launch
+constexpr int kSmMergeBlocks = 1024;      // workgroups of the merge launch
 
 // the counter block written back (graph.dev_ctr)
 enum {
@@ -147,8 +147,16 @@ __device__ __forceinline__ uint32_t lower_bound_4(const uint32_t *a,
       hi = p1;
     }
   }
-  while (lo < hi && a[lo] < x) lo++;
-  return lo;
+  // at most four candidates left: one more round of loads together
+  const uint32_t w0 = lo < hi ? a[lo] : 0u, w1 = lo + 1 < hi ? a[lo + 1] : 0u,
+                 w2 = lo + 2 < hi ? a[lo + 2] : 0u,
+                 w3 = lo + 3 < hi ? a[lo + 3] : 0u;
+  uint32_t below = 0;
+  below += lo < hi && w0 < x;
+  below += lo + 1 < hi && w1 < x;
+  below += lo + 2 < hi && w2 < x;
+  below += lo + 3 < hi && w3 < x;
+  return lo + below;
 }
 
 // One row's merge by the whole workgroup (thread me of width): the batch
@@ -219,6 +227,9 @@ __global__ void __launch_bounds__(kSmT)
   __shared__ uint64_t key[kSmMax];
   __shared__ uint32_t pos[kSmMax];
   __shared__ uint32_t rstart[kSmMax + 1];
+  __shared__ uint32_t rdeg[kSmMax];  // touched rows' degree and capacity
+  __shared__ uint32_t rcap[kSmMax];
+  __shared__ uint8_t kflag[kSmMax];  // the keep decision of each sorted key
   __shared__ uint32_t wcnt[kSmWaves][kSmBins];
   __shared__ uint32_t tstart[kSmBins];
   __shared__ uint32_t wsum32[kSmWaves];
@@ -306,7 +317,31 @@ __global__ void __launch_bounds__(kSmT)
   }
 
   // 3. keep: first copy, no self-loop on insert, absent (insert) or
-  // present (remove); each thread owns keys [kSmItems * tid, +kSmItems)
+  // present (remove).  The row searches are latency chains, so key i is
+  // searched by thread i mod kSmT (one chain per thread for up to kSmT
+  // keys); the decisions go through LDS to the compaction, whose threads
+  // own contiguous keys [kSmItems * tid, +kSmItems)
+  for (uint32_t i = tid; i < n; i += kSmT) {
+    const uint64_t kk = key[i];
+    bool k = i == 0 || key[i - 1] != kk;
+    const uint32_t r = (uint32_t)(kk >> 32), c = (uint32_t)kk;
+    if (insert && skip_self && r == c) k = false;
+    uint32_t p = 0;
+    if (k) {
+      const uint32_t d = g.deg[r];
+      const uint64_t o = g.off[r];
+      bool exists = false;
+      if (d) {
+        const uint32_t *row = g.pool + o;
+        p = lower_bound_4(row, d, c);
+        exists = p < d && row[p] == c;
+      }
+      k = insert ? !exists : exists;
+    }
+    kflag[i] = k;
+    pos[i] = p;
+  }
+  __syncthreads();
   uint64_t kv[kSmItems];
   uint32_t kp[kSmItems];
   bool keep[kSmItems];
@@ -314,26 +349,10 @@ __global__ void __launch_bounds__(kSmT)
 #pragma unroll
   for (int j = 0; j < kSmItems; j++) {
     const uint32_t i = tid * kSmItems + j;
-    keep[j] = false;
-    kp[j] = 0;
-    kv[j] = 0;
-    if (i >= n) continue;
-    kv[j] = key[i];
-    bool k = i == 0 || key[i - 1] != kv[j];
-    const uint32_t r = (uint32_t)(kv[j] >> 32), c = (uint32_t)kv[j];
-    if (insert && skip_self && r == c) k = false;
-    if (k) {
-      const uint32_t d = g.deg[r];
-      bool exists = false;
-      if (d) {
-        const uint32_t *row = g.pool + g.off[r];
-        kp[j] = lower_bound_4(row, d, c);
-        exists = kp[j] < d && row[kp[j]] == c;
-      }
-      k = insert ? !exists : exists;
-    }
-    keep[j] = k;
-    cnt += k;
+    keep[j] = i < n && kflag[i];
+    kv[j] = i < n ? key[i] : 0;
+    kp[j] = i < n ? pos[i] : 0;
+    cnt += keep[j];
   }
   // 4. compact the kept keys in place (every thread holds its keys)
   uint32_t b = 0;
@@ -375,6 +394,13 @@ __global__ void __launch_bounds__(kSmT)
   }
   if (tid == 0) rstart[t] = b;
   __syncthreads();
+  // each touched row's degree and capacity, loaded row r by thread r mod kSmT
+  for (uint32_t r = tid; r < t; r += kSmT) {
+    const uint32_t v = (uint32_t)(key[rstart[r]] >> 32);
+    rdeg[r] = g.deg[v];
+    rcap[r] = g.cap[v];
+  }
+  __syncthreads();
 
   // 5. per row: slot, pool offset, garbage, length of its move
   uint64_t alloc[kSmItems];
@@ -385,8 +411,7 @@ __global__ void __launch_bounds__(kSmT)
     alloc[j] = 0;
     if (r >= t) continue;
     const uint32_t bs = rstart[r], blen = rstart[r + 1] - bs;
-    const uint32_t v = (uint32_t)(key[bs] >> 32);
-    const uint32_t od = g.deg[v], cp = g.cap[v];
+    const uint32_t od = rdeg[r], cp = rcap[r];
     uint64_t move;
     if (insert && (uint64_t)od + blen > cp) {
       alloc[j] = slot_capacity((uint64_t)od + blen, n_cols);

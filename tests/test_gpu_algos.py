@@ -291,7 +291,20 @@ def test_pagerank_segments_long_runs_and_directed_rows():
         assert ok, (it, w)
 
 
-def test_pagerank_replayed_pairs_stay_exact(c1):
+@pytest.fixture
+def graph_capture():
+    """Recorded PageRank iteration pairs are opt-in (SSTG_GRAPH_CAPTURE=1,
+    read by the library at each call)."""
+    old = os.environ.get("SSTG_GRAPH_CAPTURE")
+    os.environ["SSTG_GRAPH_CAPTURE"] = "1"
+    yield
+    if old is None:
+        del os.environ["SSTG_GRAPH_CAPTURE"]
+    else:
+        os.environ["SSTG_GRAPH_CAPTURE"] = old
+
+
+def test_pagerank_replayed_pairs_stay_exact(c1, graph_capture):
     """PageRank iterations run as replays of one recorded pair: calls on an
     unchanged graph repeat bit for bit, with other device work (BFS, CC)
     and synchronisations between them, and a call after an update or a
