@@ -23,29 +23,21 @@ constexpr uint32_t kInf = 0xFFFFFFFFu;
 inline uint64_t blocks(uint64_t n) { return ceil_div(n ? n : 1, kT); }
 
 // ----------------------------------------------------------------- BFS
-__global__ void k_bfs_init(uint32_t n, uint32_t *level, int32_t *parent,
-                           uint32_t src) {
-  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (v >= n) return;
-  level[v] = (v == src) ? 0u : kInf;
-  if (parent) parent[v] = (v == src) ? (int32_t)src : -1;
-}
-
 // Top-down (sparse push) step of a BFS level as an EdgeMap functor: the
 // first writer of an unvisited neighbour claims it.
 struct BfsPush {
   uint32_t *level;
   int32_t *parent;
   uint32_t next_level;
-  __device__ __forceinline__ bool cond(uint32_t d) const {
-    return level[d] == kInf;
+  __device__ __forceinline__ uint32_t peek(uint32_t d) const {
+    return level[d];
   }
-  __device__ __forceinline__ bool update_atomic(uint32_t s, uint32_t d) const {
-    if (atomicCAS(&level[d], kInf, next_level) == kInf) {
-      if (parent) parent[d] = (int32_t)s;
-      return true;
-    }
-    return false;
+  __device__ static __forceinline__ bool open(uint32_t x) { return x == kInf; }
+  __device__ __forceinline__ uint32_t try_claim(uint32_t d) const {
+    return atomicCAS(&level[d], kInf, next_level);
+  }
+  __device__ __forceinline__ void claimed(uint32_t s, uint32_t d) const {
+    if (parent) parent[d] = (int32_t)s;
   }
 };
 
@@ -67,10 +59,33 @@ struct BfsPull {
   }
 };
 
-__global__ void k_level_flags(const uint32_t *level, uint32_t n, uint32_t L,
-                              uint8_t *flags) {
-  uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (v < n) flags[v] = level[v] == L;
+// frontier and visited predicates of the BFS bitmaps
+struct LevelIs {
+  const uint32_t *level;
+  uint32_t L;
+  __device__ __forceinline__ bool operator()(uint32_t v) const {
+    return level[v] == L;
+  }
+};
+struct Reached {
+  const uint32_t *level;
+  __device__ __forceinline__ bool operator()(uint32_t v) const {
+    return level[v] != kInf;
+  }
+};
+
+// Start of a BFS from src: levels and parents reset, the queue holding src,
+// the visited filter holding only src (threads cover max(n, words)).
+__global__ void k_bfs_start(uint32_t n, uint32_t *level, int32_t *parent,
+                            uint32_t src, uint32_t *queue, uint32_t *vis,
+                            uint64_t words) {
+  const uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (v < n) {
+    level[v] = (v == src) ? 0u : kInf;
+    if (parent) parent[v] = (v == src) ? (int32_t)src : -1;
+  }
+  if (v < words) vis[v] = (v == (src >> 5)) ? 1u << (src & 31) : 0u;
+  if (v == 0) queue[0] = src;
 }
 
 __global__ void k_level_dist(const uint32_t *level, uint32_t n, int32_t *out) {
@@ -260,25 +275,38 @@ struct SigmaPull8 {
   }
 };
 
-// delta(v) = sigma(v) * sum over children w of (1 + delta(w)) / sigma(w)
+// delta(v) = sigma(v) * sum over children w of (1 + delta(w)) / sigma(w).
+// Once a vertex's delta is final its sigma slot takes the quotient
+// (1 + delta) / sigma, which its parents' pull (one level up, the next
+// launch) gathers: one 8-byte value and no divide per child.  Within a
+// launch only level-L slots are written and only level-(L+1) slots read.
 struct DeltaPull8 {
   const uint8_t *lev;
-  const double *sigma;
+  double *q;  // sigma below level L+1, (1 + delta) / sigma from L+1 down
   double *delta;
   uint32_t L;
   using Acc = double;
   __device__ __forceinline__ Acc zero() const { return 0.0; }
   __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t w) const {
-    if ((uint32_t)lev[w] == L + 1) a += (1.0 + delta[w]) / sigma[w];
+    if ((uint32_t)lev[w] == L + 1) a += q[w];
   }
   __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
   __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
     return __shfl_xor(a, o, kWave);
   }
   __device__ __forceinline__ void store(uint32_t v, Acc s) const {
-    delta[v] = sigma[v] * s;
+    const double sv = q[v], d = sv * s;
+    delta[v] = d;
+    q[v] = (1.0 + d) / sv;
   }
 };
+
+// the deepest level's quotients (1 + 0) / sigma
+__global__ void k_bc_last_quotients(const uint32_t *list, uint64_t count,
+                                    double *q) {
+  const uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < count) q[list[i]] = 1.0 / q[list[i]];
+}
 
 __global__ void k_level_bytes(const uint32_t *level, uint32_t n, uint8_t *lev) {
   const uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
@@ -419,6 +447,9 @@ constexpr uint32_t kTailFrontier = 16384;
 constexpr uint32_t kTailEdges = 1u << 17;
 constexpr uint32_t kTailLevels = 1024;
 constexpr uint32_t kTailPer = kTailFrontier / kTailThreads;
+constexpr uint32_t kTailBatch = 4;
+// a source of larger degree starts with a grid-wide push instead
+constexpr uint32_t kTailStartEdges = 2048;
 
 __global__ void __launch_bounds__(kTailThreads)
     k_bfs_tail(DevGraph g, uint32_t *level, int32_t *parent, uint32_t *queue,
@@ -464,24 +495,48 @@ __global__ void __launch_bounds__(kTailThreads)
     }
     if (tid == 0) pre[cnt] = total;
     __syncthreads();
-    // the frontier's edges, flat
+    // the frontier's edges, flat: kTailBatch consecutive edges per thread
+    // per step, each load of the batch issued before any is used
     const uint64_t tail = head + cnt;
-    for (uint32_t e = tid; e < total; e += kTailThreads) {
-      uint32_t lo = 0, hi = cnt;  // largest j with pre[j] <= e
+    for (uint32_t e0 = tid * kTailBatch; e0 < total;
+         e0 += kTailThreads * kTailBatch) {
+      uint32_t u[kTailBatch], idx[kTailBatch], w[kTailBatch], lv[kTailBatch];
+      uint64_t rb[kTailBatch];
+      uint32_t lo = 0, hi = cnt;  // largest j with pre[j] <= e0
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (pre[mid] <= e)
+        if (pre[mid] <= e0)
           lo = mid;
         else
           hi = mid;
       }
-      const uint32_t u = queue[head + lo];
-      const uint32_t w = g.pool[g.off[u] + (e - pre[lo])];
-      if (level[w] == kInf && atomicCAS(&level[w], kInf, L + 1) == kInf) {
-        if (parent) parent[w] = (int32_t)u;
-        const uint32_t pos = atomicAdd(&s_next, 1u);
-        queue[tail + pos] = w;
-        atomicAdd(&s_edges, (unsigned long long)g.deg[w]);
+#pragma unroll
+      for (uint32_t j = 0; j < kTailBatch; j++) {
+        const uint32_t e = e0 + j;
+        if (e < total) {
+          while (pre[lo + 1] <= e) lo++;  // pre[cnt] = total > e
+          u[j] = queue[head + lo];
+          idx[j] = e - pre[lo];
+        }
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kTailBatch; j++)
+        rb[j] = e0 + j < total ? g.off[u[j]] : 0;
+#pragma unroll
+      for (uint32_t j = 0; j < kTailBatch; j++)
+        w[j] = e0 + j < total ? g.pool[rb[j] + idx[j]] : 0u;
+#pragma unroll
+      for (uint32_t j = 0; j < kTailBatch; j++)
+        lv[j] = e0 + j < total ? level[w[j]] : 0u;
+#pragma unroll
+      for (uint32_t j = 0; j < kTailBatch; j++) {
+        if (e0 + j < total && lv[j] == kInf &&
+            atomicCAS(&level[w[j]], kInf, L + 1) == kInf) {
+          if (parent) parent[w[j]] = (int32_t)u[j];
+          const uint32_t pos = atomicAdd(&s_next, 1u);
+          queue[tail + pos] = w[j];
+          atomicAdd(&s_edges, (unsigned long long)g.deg[w[j]]);
+        }
       }
     }
     __syncthreads();
@@ -509,6 +564,11 @@ __global__ void __launch_bounds__(kTailThreads)
   }
 }
 
+// Every level's vertices are appended to the queue (order, or a workspace
+// buffer) whatever the direction, so level L's vertices are
+// queue[level_start[L] .. level_start[L + 1]).  Per level the host reads one
+// small counter block and decides the next step; counters and bitmaps are
+// cleared by the kernels themselves.
 static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                     int32_t *parent, uint32_t *order, bool top_down_only,
                     BfsRun *run) {
@@ -519,16 +579,10 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   SSTG_TRY(graph_degree_prefix(g, &prefix));
   EdgeMapCtx ctx;
   SSTG_TRY(edge_map_ctx(g, &ctx, kWsAlgo));
-  SSTG_LAUNCH(k_bfs_init, blocks(n), kT, 0, s, n, level, parent, src);
   uint32_t *queue = order;
-  uint32_t *qbuf = nullptr;
-  if (!queue) {
-    SSTG_TRY(ws.get(kWsAlgo + 10, (uint64_t)n + 1, &qbuf));
-    queue = qbuf;
-  }
-  // queue holds the frontier of the current level at queue + qs
-  SSTG_HIP(hipMemcpyAsync(queue, &src, sizeof(uint32_t), hipMemcpyHostToDevice,
-                          s));
+  if (!queue) SSTG_TRY(ws.get(kWsAlgo + 10, (uint64_t)n + 1, &queue));
+  SSTG_LAUNCH(k_bfs_start, blocks(std::max<uint64_t>(n, ctx.words)), kT, 0, s,
+              n, level, parent, src, queue, ctx.vis, ctx.words);
   uint32_t src_deg = 0;
   SSTG_HIP(hipMemcpyAsync(&src_deg, g->deg + src, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
@@ -536,6 +590,7 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   uint64_t qs = 0, nf = 1, mf = src_deg;
   uint64_t explored = src_deg;  // edges of visited vertices
   bool bottom_up = false;
+  bool vis_stale = false;  // bottom-up steps do not maintain the filter
   const double alpha = g->bfs_alpha, beta = g->bfs_beta;
   uint64_t prev_nf = 0;
   if (run) {  // level 0 is queue[0, 1); each step appends the end of the next
@@ -549,7 +604,6 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   if (!top_down_only) SSTG_TRY(graph_is_symmetric(g, &symmetric));
   top_down_only = top_down_only || !symmetric;
   uint32_t L = 0;
-  uint8_t *flags = nullptr;
   const bool verbose = std::getenv("SSTG_VERBOSE") != nullptr;
   auto t_level = std::chrono::steady_clock::now();
   while (nf > 0) {
@@ -560,7 +614,14 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                    (bottom_up ? (double)nf >= (double)n / beta
                               : nf > prev_nf &&
                                     (double)mf > (double)unexplored / alpha);
-    if (!want_bu && nf <= kTailFrontier && mf <= kTailEdges) {
+    if (!want_bu && mf == ~0ull) {
+      // a bottom-up step left the frontier's degree sum open: its prefix,
+      // which the push below reuses
+      SSTG_TRY(frontier_prefix(g, ctx, queue + qs, nf, &mf));
+      explored += mf;
+    }
+    if (!want_bu && nf <= kTailFrontier && mf <= kTailEdges &&
+        (L > 0 || mf <= kTailStartEdges)) {
       // the remaining top-down levels in one block (k_bfs_tail)
       uint64_t *st = nullptr, *ends = nullptr;
       SSTG_TRY(ws.get(kWsAlgo + 8, 8, &st));
@@ -606,36 +667,27 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
       continue;
     }
     EdgeMapResult res;
+    uint32_t *next = queue + qs + nf;
     if (want_bu) {
-      if (!bottom_up) SSTG_TRY(frontier_to_bitmap(ctx, queue + qs, nf, s));
-      BfsPull f{level, parent, L + 1};
-      SSTG_TRY(edge_map_dense(g, ctx, f, &res));
+      SSTG_TRY(edge_map_dense(g, ctx, BfsPull{level, parent, L + 1},
+                              LevelIs{level, L}, next, &res));
       bottom_up = true;
-      // the next top-down step (if any) needs the queue of level L+1
-      bool next_bu = (double)res.count >= (double)n / beta;
-      if (run || !next_bu) {
-        SSTG_TRY(ws.get(kWsAlgo + 11, n, &flags));
-        SSTG_LAUNCH(k_level_flags, blocks(n), kT, 0, s, (const uint32_t *)level,
-                    n, L + 1, flags);
-        uint64_t *tmp64 = nullptr;
-        SSTG_TRY(ws.get(kWsAlgo + 12, n, &tmp64));
-        SSTG_TRY(select_flagged_u64(flags, nullptr, n, tmp64,
-                                    g->dev_ctr + 7, ws, kWsPrim, s));
-        SSTG_TRY(narrow_u64_to_u32(tmp64, res.count, queue + qs + nf, s));
-      }
+      vis_stale = true;
     } else {
-      BfsPush f{level, parent, L + 1};
-      SSTG_TRY(edge_map_sparse(g, ctx, queue + qs, nf, mf, f,
-                               queue + qs + nf, &res));
+      if (vis_stale) {  // the filter again from the levels
+        SSTG_TRY(edge_map_bits(g, ctx, Reached{level}, ctx.vis, nullptr));
+        vis_stale = false;
+      }
+      SSTG_TRY(edge_map_sparse(g, ctx, queue + qs, nf, mf,
+                               BfsPush{level, parent, L + 1}, next, &res));
       bottom_up = false;
     }
-    if (verbose) {  // diagnostics: per-level wall time (synchronises)
-      SSTG_HIP(hipStreamSynchronize(s));
+    if (verbose) {
       auto t1 = std::chrono::steady_clock::now();
       std::fprintf(stderr,
-                   "bfs level %u %s nf=%llu mf=%llu -> next %llu  %.3f ms\n", L,
+                   "bfs level %u %s nf=%llu mf=%lld -> next %llu  %.3f ms\n", L,
                    want_bu ? "pull" : "push", (unsigned long long)nf,
-                   (unsigned long long)mf, (unsigned long long)res.count,
+                   (long long)mf, (unsigned long long)res.count,
                    std::chrono::duration<double, std::milli>(t1 - t_level)
                        .count());
       t_level = t1;
@@ -643,8 +695,8 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
     qs += nf;
     prev_nf = nf;
     nf = res.count;
-    mf = res.degree_sum;
-    explored += mf;
+    mf = res.degree_sum;  // ~0 after a bottom-up step: not computed
+    if (mf != ~0ull) explored += mf;
     L++;
     if (run) run->level_start.push_back(qs + nf);
   }
@@ -680,8 +732,6 @@ static int pagerank_impl(sstg_graph *g, uint32_t iters, double damping,
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
   if (n == 0) return SSTG_OK;
-  EdgeMapCtx ctx;
-  SSTG_TRY(edge_map_ctx(g, &ctx, kWsAlgo));
   DegreeBins bins;
   SSTG_TRY(degree_bins(g, &bins));
   Real *contrib = nullptr, *contrib2 = nullptr;
@@ -870,7 +920,12 @@ static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
   for (uint32_t L = 1; L < run.levels; L++)
     SSTG_TRY(pull(L, SigmaPull8{lev, sigma, L}));
   // backward: dependencies from the children, the deepest level (no
-  // children, delta 0) excluded
+  // children, delta 0) only turned into quotients
+  const uint32_t last = run.levels - 1;
+  const uint64_t lb = st[last * 4], le = st[last * 4 + 3];
+  if (le > lb)
+    SSTG_LAUNCH(k_bc_last_quotients, ceil_div(le - lb, kT), kT, 0, s,
+                (const uint32_t *)(list + lb), le - lb, sigma);
   for (int64_t L = (int64_t)run.levels - 2; L >= 0; L--)
     SSTG_TRY(pull((uint32_t)L, DeltaPull8{lev, sigma, out, (uint32_t)L}));
   SSTG_HIP(hipStreamSynchronize(s));

@@ -4,8 +4,9 @@
 // SparseMatrixV::map_neighbors / map_range, SparseMatrix.hpp:174-315).
 //
 // Functor contracts (device methods):
-//   sparse push  : bool cond(d); bool update_atomic(s, d)  -> d joins the
-//                  output frontier when update_atomic returns true
+//   sparse push  : uint32_t peek(d), static bool open(x) -> d may be
+//                  claimed; uint32_t try_claim(d) returns the value it
+//                  replaced, open(that) -> claimed; void claimed(s, d)
 //   dense pull   : bool cond(d); void update(s, d)         -> d scans its
 //                  neighbours until one is in the input frontier bitmap
 //   vertex pull  : Acc zero(); Acc term(v, u) / add(acc, v, u);
@@ -24,16 +25,21 @@ struct EdgeMapResult {
 };
 
 struct EdgeMapCtx {
-  uint32_t *bm_cur = nullptr;   // input frontier bitmap
-  uint32_t *bm_next = nullptr;  // output frontier bitmap
+  uint32_t *bm = nullptr;   // frontier bitmap of a bottom-up step
+  uint32_t *vis = nullptr;  // visited filter of the top-down steps
   uint64_t words = 0;
-  uint64_t *fpre = nullptr;     // frontier degree prefix (sparse)
-  uint64_t fcap = 0;
+  uint64_t *fpre = nullptr;  // frontier degree prefix (sparse)
+  const uint32_t *fpre_front = nullptr;  // the frontier fpre was built for
+  uint64_t fpre_n = 0;
   int slot = 0;
-  unsigned long long *ctr = nullptr;  // [0] count, [1] degree sum
-  // bottom-up active lists: the bins' survivors back to back in one buffer
-  // (bin b at act_off[b], act_n[b] entries), double-buffered; act_src < 0
-  // until the first bottom-up step, whose lists are the degree bins
+  // step counters, two sets of kStepCtrs (see em::kStepCtrs); a step adds
+  // into set `parity` and clears the other
+  unsigned long long *ctrs = nullptr;
+  int parity = 0;
+  // bottom-up active lists: bin b's survivors at act[.] + act_off[b] (the
+  // bins' offsets in the degree-bin lists, so a bin never outgrows its
+  // region), act_n[b] entries, double-buffered; act_src < 0 until the first
+  // bottom-up step, whose lists are the degree bins
   uint32_t *act[2] = {};
   uint64_t act_off[4] = {0, 0, 0, 0};
   uint64_t act_n[4] = {0, 0, 0, 0};
@@ -45,20 +51,11 @@ namespace {  // internal linkage: this header is included by several files
 
 constexpr int kT = 256;
 constexpr int kChunk = 8;
-constexpr int kBitmapRounds = 8;  // 512 vertices per wave in k_bitmap_from
+constexpr int kBitmapRounds = 8;  // 512 vertices per wave in k_bits
 
 __global__ void k_clear_u32(uint32_t *p, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n) p[i] = 0;
-}
-
-__global__ void k_queue_to_bitmap(const uint32_t *q, uint64_t nq,
-                                  uint32_t *bm) {
-  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (i < nq) {
-    uint32_t v = q[i];
-    atomicOr(&bm[v >> 5], 1u << (v & 31));
-  }
 }
 
 __global__ void k_frontier_degrees(DevGraph g, const uint32_t *q, uint64_t nq,
@@ -73,25 +70,14 @@ __global__ void k_narrow(const uint64_t *in, uint64_t n, uint32_t *out) {
   if (i < n) out[i] = (uint32_t)in[i];
 }
 
-// Wave-aggregated append of d to out; also accumulates count and degree sum.
-__device__ __forceinline__ void append(bool take, uint32_t d, uint32_t dd,
-                                       uint32_t *out,
-                                       unsigned long long *ctr) {
-  uint64_t mask = __ballot(take);
-  if (!mask) return;
-  uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-  uint32_t cnt = (uint32_t)__popcll(mask);
-  unsigned long long dsum = take ? dd : 0;
-  // sum degrees of the taken lanes over the active lanes
-  for (int o = kWave / 2; o > 0; o >>= 1)
-    dsum += __shfl_xor(dsum, o, kWave);
-  unsigned long long base = 0;
-  if (lane_id() == leader) {
-    base = atomicAdd(&ctr[0], (unsigned long long)cnt);
-    atomicAdd(&ctr[1], dsum);
-  }
-  base = __shfl(base, leader, kWave);
-  if (take && out) out[base + mask_rank(mask)] = d;
+// Counters of one BFS step (u64): [0] vertices found, [1] their degree sum,
+// [2 + b] survivors of degree bin b (bottom-up).  Two sets alternate: the
+// kernels of a step add into one and clear the other, which the host has
+// read after the step before, so no step needs a clearing call of its own.
+constexpr int kStepCtrs = 8;
+
+__device__ __forceinline__ void clear_step_ctrs(unsigned long long *c) {
+  if (c && blockIdx.x == 0 && threadIdx.x < kStepCtrs) c[threadIdx.x] = 0;
 }
 
 // Block-wide exclusive scan of one u32 per thread (kT threads); returns the
@@ -114,160 +100,198 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total) 
   return off + incl - v;
 }
 
-// Adds a per-thread count and degree sum to ctr with one atomic pair per
-// block.
-__device__ __forceinline__ void block_count(uint32_t cnt, uint64_t dsum,
-                                            unsigned long long *ctr) {
-  __shared__ unsigned long long bc[2];
+// Per-block result of a bottom-up step, without global atomics (same-address
+// atomics from ~10^5 blocks queue at one memory channel: at rMat scale 24
+// they made the widest bottom-up level 5.8 ms instead of 1 ms).  The block
+// writes its found vertices, compacted, to tmp_f from lbase (the list index
+// of its first vertex) on, its survivors likewise to tmp_k, and
+// (found << 32) | survivors to *pc.  Every thread of the block must call it.
+__device__ __forceinline__ void step_partial(bool take_f, bool take_k,
+                                             uint32_t v, uint64_t lbase,
+                                             uint32_t *tmp_f, uint32_t *tmp_k,
+                                             uint64_t *pc) {
+  // at most kT of each per block: two 16-bit counts in one scan
+  const uint32_t packed = (take_f ? 1u : 0u) | (take_k ? 0x10000u : 0u);
+  uint32_t total;
+  const uint32_t off = block_scan_u32(packed, &total);
+  if (take_f) tmp_f[lbase + (off & 0xFFFFu)] = v;
+  if (take_k) tmp_k[lbase + (off >> 16)] = v;
+  if (threadIdx.x == 0)
+    *pc = ((uint64_t)(total & 0xFFFFu) << 32) | (uint64_t)(total >> 16);
+}
+
+// Pull blocks of one bottom-up step: bin b's kernel numbers its blocks from
+// pbase[b] (pbase[4] = all blocks), its block k covers list indices from
+// lbase[b] + k * per[b] (lbase[b]: the bin's offset in the active lists).
+struct StepBlocks {
+  uint64_t pbase[5];
+  uint64_t lbase[4];
+  uint32_t per[4];
+};
+
+// After the exclusive scan of the packed counts pc[0..pbase[4]]: one wave
+// per pull block moves its found vertices to next (the BFS queue, in block
+// order) and its survivors to its bin's region of keep.  The first block of
+// each bin stores the bin's survivor count in ctr[2 + b], block 0 the found
+// count in ctr[0].
+constexpr int kPlaceT = 64;
+__global__ void __launch_bounds__(kPlaceT)
+    k_step_place(StepBlocks sb, const uint64_t *pc, const uint32_t *tmp_f,
+                 const uint32_t *tmp_k, uint32_t *next, uint32_t *keep,
+                 unsigned long long *ctr) {
+  const uint64_t k = blockIdx.x;
+  int b = 0;
+  while (b < 3 && k >= sb.pbase[b + 1]) b++;
+  const uint64_t p0 = pc[k], p1 = pc[k + 1], pb = pc[sb.pbase[b]];
+  const uint32_t nfound = (uint32_t)(p1 >> 32) - (uint32_t)(p0 >> 32);
+  const uint32_t nkeep = (uint32_t)p1 - (uint32_t)p0;
+  const uint64_t lb = sb.lbase[b] + (k - sb.pbase[b]) * sb.per[b];
+  const uint64_t fdst = p0 >> 32;
+  const uint64_t kdst = sb.lbase[b] + ((uint32_t)p0 - (uint32_t)pb);
+  for (uint32_t t = threadIdx.x; t < nfound; t += kPlaceT)
+    next[fdst + t] = tmp_f[lb + t];
+  for (uint32_t t = threadIdx.x; t < nkeep; t += kPlaceT)
+    keep[kdst + t] = tmp_k[lb + t];
   if (threadIdx.x == 0) {
-    bc[0] = 0;
-    bc[1] = 0;
-  }
-  __syncthreads();
-  uint64_t c = wave_sum((uint64_t)cnt), d = wave_sum(dsum);
-  if (lane_id() == 0 && (c | d)) {
-    atomicAdd(&bc[0], (unsigned long long)c);
-    atomicAdd(&bc[1], (unsigned long long)d);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && (bc[0] | bc[1])) {
-    atomicAdd(&ctr[0], bc[0]);
-    atomicAdd(&ctr[1], bc[1]);
+    if (k == sb.pbase[b])
+      ctr[2 + b] = (uint32_t)pc[sb.pbase[b + 1]] - (uint32_t)pb;
+    if (k == 0) ctr[0] = pc[sb.pbase[4]] >> 32;
   }
 }
 
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;  // ids are below n_rows < 2^32
+
+// Top-down step: thread t expands the kChunk consecutive frontier edges from
+// t * kChunk.  Their row reads are issued together, then their words of the
+// visited filter vis, then the cond loads, then the claims, so a lane keeps
+// kChunk loads in flight instead of walking one dependent chain per edge.
+// vis only filters: a bit is set only for a visited vertex (plain racy ORs
+// may lose bits, never add wrong ones), so a stale or lost bit costs a
+// failed claim, never a wrong result.  The block appends its claims to next
+// with one atomic.
 template <class F>
 __global__ void __launch_bounds__(kT)
     k_sparse_push(DevGraph g, const uint32_t *front, const uint64_t *fpre,
-                  uint64_t nf, uint64_t mf, F f, uint32_t *next,
-                  unsigned long long *ctr) {
-  // each thread expands kChunk consecutive frontier edges, keeps the vertices
-  // it claims in registers, and the block appends them with one atomic
+                  uint64_t nf, uint64_t mf, F f, uint32_t *vis, uint32_t *next,
+                  unsigned long long *ctr, unsigned long long *ctr_clear) {
+  clear_step_ctrs(ctr_clear);
   __shared__ uint32_t got_lds[kT * kChunk];
   uint32_t *got = got_lds + threadIdx.x * kChunk;
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
-  uint64_t end = e + kChunk < mf ? e + kChunk : mf;
+  const uint64_t e0 = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
   uint32_t cnt = 0;
   uint64_t dsum = 0;
-  if (e < end) {
-    uint64_t i = find_segment(fpre, nf, e);
-    for (; e < end; e++) {
-      while (e >= fpre[i + 1]) i++;
-      uint32_t s = front[i];
-      uint32_t d = g.pool[g.off[s] + (e - fpre[i])];
-      if (f.cond(d) && f.update_atomic(s, d)) {
-        got[cnt++] = d;
-        dsum += g.deg[d];
+  // the walk below stays inside fpre[0..nf] only for edges below fpre[nf]
+  if (e0 < mf) mf = min(mf, fpre[nf]);
+  if (e0 < mf) {
+    uint64_t i = find_segment(fpre, nf, e0);
+    uint64_t seg_start = fpre[i], seg_end = fpre[i + 1];
+    uint32_t s = front[i];
+    uint64_t rbase = g.off[s];
+    uint32_t d[kChunk], src[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; j++) {
+      const uint64_t e = e0 + j;
+      d[j] = kNoVertex;
+      src[j] = s;
+      if (e < mf) {
+        while (e >= seg_end) {  // zero-degree frontier vertices skipped too
+          i++;
+          seg_start = seg_end;
+          seg_end = fpre[i + 1];
+          s = front[i];
+          rbase = g.off[s];
+        }
+        src[j] = s;
+        d[j] = g.pool[rbase + (e - seg_start)];
+      }
+    }
+    // each phase's loads are unconditional (a non-candidate reads vertex
+    // 0's entry) or predicated without use, so they are all in flight
+    // before the first result is needed
+    uint32_t w[kChunk], x[kChunk];
+    bool c[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; j++)
+      w[j] = vis[(d[j] != kNoVertex ? d[j] : 0u) >> 5];
+#pragma unroll
+    for (int j = 0; j < kChunk; j++)
+      c[j] = d[j] != kNoVertex && !((w[j] >> (d[j] & 31)) & 1u);
+#pragma unroll
+    for (int j = 0; j < kChunk; j++) x[j] = f.peek(c[j] ? d[j] : 0u);
+#pragma unroll
+    for (int j = 0; j < kChunk; j++) {
+      c[j] = c[j] && F::open(x[j]);
+      x[j] = c[j] ? f.try_claim(d[j]) : 0u;
+    }
+    uint32_t dd[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; j++) {
+      c[j] = c[j] && F::open(x[j]);  // this lane's claim won
+      dd[j] = c[j] ? g.deg[d[j]] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; j++) {
+      if (c[j]) {
+        f.claimed(src[j], d[j]);
+        got[cnt++] = d[j];
+        dsum += dd[j];
+        // the word as read above: a bit set meanwhile by another lane may
+        // be lost, which only weakens the filter
+        vis[d[j] >> 5] = w[j] | (1u << (d[j] & 31));
       }
     }
   }
   uint32_t total;
-  uint32_t off = block_scan_u32(cnt, &total);
+  const uint32_t off = block_scan_u32(cnt, &total);
   __shared__ unsigned long long base;
-  if (threadIdx.x == 0 && total) {
+  if (threadIdx.x == 0 && total)
     base = atomicAdd(&ctr[0], (unsigned long long)total);
-  }
+  __shared__ unsigned long long bsum;
+  if (threadIdx.x == 0) bsum = 0;
   __syncthreads();
-  if (total) {
+  const uint64_t ws = wave_sum(dsum);
+  if (lane_id() == 0 && ws) atomicAdd(&bsum, (unsigned long long)ws);
+  if (total)
     for (uint32_t k = 0; k < cnt; k++) next[base + off + k] = got[k];
-  }
-  block_count(0, dsum, ctr);
-}
-
-// Output frontier bitmap of a dense step: bit v = f.in_output(v); one lane
-// per vertex, a ballot per 64 vertices, no atomics.
-// Per-block count and degree sum stored at part[2 * block] (no atomics:
-// many blocks adding to one address serialise at the L2).
-__device__ __forceinline__ void block_partial(uint64_t cnt, uint64_t dsum,
-                                              unsigned long long *part) {
-  __shared__ unsigned long long bp[2][kT / kWave];
-  cnt = wave_sum(cnt);
-  dsum = wave_sum(dsum);
-  if (lane_id() == 0) {
-    bp[0][threadIdx.x / kWave] = cnt;
-    bp[1][threadIdx.x / kWave] = dsum;
-  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long c = 0, d = 0;
-    for (int w = 0; w < kT / kWave; w++) {
-      c += bp[0][w];
-      d += bp[1][w];
-    }
-    part[2 * (uint64_t)blockIdx.x] = c;
-    part[2 * (uint64_t)blockIdx.x + 1] = d;
-  }
+  if (threadIdx.x == 0 && bsum) atomicAdd(&ctr[1], bsum);
 }
 
-// ctr[0..1] = the sums of the nb partial pairs (one block)
+// bm bit v = p(v) for v < n; one lane per vertex, a ballot per 64 vertices,
+// no atomics.
+template <class P>
 __global__ void __launch_bounds__(kT)
-    k_sum_partials(const unsigned long long *part, uint64_t nb,
-                   unsigned long long *ctr) {
-  uint64_t c = 0, d = 0;
-  for (uint64_t b = threadIdx.x; b < nb; b += kT) {
-    c += part[2 * b];
-    d += part[2 * b + 1];
-  }
-  __shared__ unsigned long long bp[2][kT / kWave];
-  c = wave_sum(c);
-  d = wave_sum(d);
-  if (lane_id() == 0) {
-    bp[0][threadIdx.x / kWave] = c;
-    bp[1][threadIdx.x / kWave] = d;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long tc = 0, td = 0;
-    for (int w = 0; w < kT / kWave; w++) {
-      tc += bp[0][w];
-      td += bp[1][w];
-    }
-    ctr[0] = tc;
-    ctr[1] = td;
-  }
-}
-
-template <class F>
-__global__ void __launch_bounds__(kT)
-    k_bitmap_from(uint32_t n, F f, uint32_t *bm, uint64_t words, DevGraph g,
-                  unsigned long long *part) {
-  // kBitmapRounds words of 64 vertices per wave; the output vertices are
-  // counted (and their degrees summed) per block into part
-  uint64_t cnt = 0, dsum = 0;
+    k_bits(uint32_t n, P p, uint32_t *bm, uint64_t words,
+           unsigned long long *ctr_clear, uint64_t *zero1) {
+  clear_step_ctrs(ctr_clear);
+  if (zero1 && blockIdx.x == 0 && threadIdx.x == 0) *zero1 = 0;
   const uint32_t lane = lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * kT + threadIdx.x) / kWave;
   for (int r = 0; r < kBitmapRounds; r++) {
     const uint64_t v = (wave * kBitmapRounds + r) * kWave + lane;
-    const bool in = v < n && f.in_output((uint32_t)v);
+    const bool in = v < n && p((uint32_t)v);
     const uint64_t m = __ballot(in);
-    if (in) {
-      cnt++;
-      dsum += g.deg[v];
-    }
     const uint64_t w = v >> 5;
     if ((lane & 31) == 0 && w < words)
       bm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
   }
-  block_partial(cnt, dsum, part);
 }
-
-constexpr uint32_t kNoVertex = 0xFFFFFFFFu;  // ids are below n_rows < 2^32
 
 // Bottom-up step over the active vertices list[0..count) with G lanes per
 // vertex (G divides 64).  A vertex whose cond holds tests one neighbour per
 // lane first, then U per lane per round: the U row elements, then their U
 // bitmap words, so each lane has U gathers in flight before it tests any of
 // them.  The first round with a frontier member ends the scan and the lowest
-// such lane's neighbour is the source.  keep[i] = 1 marks a vertex with
-// neighbours that found none: it stays active for the next bottom-up step.
-// No atomics: the counts come from the bitmap pass and the active list from
-// a compaction of keep.
+// such lane's neighbour is the source.  Found vertices and the vertices
+// with neighbours that found none (survivors, active in the next bottom-up
+// step) leave through step_partial; list[i] has list index lbase + i.
 template <class F, int G, int U>
 __global__ void __launch_bounds__(kT)
     k_dense_pull(DevGraph g, const uint32_t *list, uint64_t count, F f,
-                 const uint32_t *bm_cur, uint8_t *keep) {
-  const uint64_t gid =
-      (uint64_t)xcd_block(blockIdx.x, gridDim.x) * kT + threadIdx.x;
+                 const uint32_t *bm_cur, uint64_t lbase, uint32_t *tmp_f,
+                 uint32_t *tmp_k, uint64_t *pc) {
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t gid = (uint64_t)blk * kT + threadIdx.x;
   const uint32_t lig = (uint32_t)(gid % G);
   const uint64_t i = gid / G;
   const uint32_t glane0 = lane_id() & ~(uint32_t)(G - 1);
@@ -301,61 +325,69 @@ __global__ void __launch_bounds__(kT)
         if (m) {
           const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
           const uint32_t su = __shfl(mine, leader, kWave);
-          if (lig == 0) {
-            f.update(su, v);
-            found = true;
-          }
+          if (lig == 0) f.update(su, v);
+          found = true;
           break;
         }
       }
       active = !found && dv > 0;
     }
-    if (lig == 0) keep[i] = active ? 1 : 0;
   }
+  step_partial(lig == 0 && found, lig == 0 && active, v,
+               lbase + (uint64_t)blk * (kT / G), tmp_f, tmp_k, pc + blk);
 }
 
 // Bottom-up step with a whole block per (high-degree) vertex: 256 x U
-// neighbours per round, all loads of a round issued before their tests.
+// neighbours per round, all loads of a round issued before their tests;
+// block i's result goes to pc[i] and slot lbase + i of tmp_f or tmp_k.
+// The first hit of round r goes to hit[r % 3]: thread 0 resets the slot of
+// round r + 1 before round r's barrier, when every thread has read it as
+// round r - 2's slot, so all threads read one value per round.
 template <class F, int U>
 __global__ void __launch_bounds__(kT)
     k_dense_pull_block(DevGraph g, const uint32_t *list, uint64_t count, F f,
-                       const uint32_t *bm_cur, uint8_t *keep) {
-  __shared__ uint32_t first_hit;
+                       const uint32_t *bm_cur, uint64_t lbase, uint32_t *tmp_f,
+                       uint32_t *tmp_k, uint64_t *pc) {
+  __shared__ uint32_t hit[3];
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t v = list[i];
-  if (!f.cond(v)) {  // uniform across the block
-    if (threadIdx.x == 0) keep[i] = 0;
-    return;
-  }
-  const uint32_t dv = g.deg[v];
-  const uint32_t *row = g.pool + g.off[v];
-  if (threadIdx.x == 0) first_hit = kNoVertex;
-  __syncthreads();
-  for (uint32_t base = 0, uu = 1; base < dv; base += kT * uu, uu = U) {
-    uint32_t u[U], w[U];
-#pragma unroll
-    for (int j = 0; j < U; j++) {
-      const uint32_t k = base + threadIdx.x + j * kT;
-      u[j] = ((uint32_t)j < uu && k < dv) ? row[k] : kNoVertex;
-    }
-#pragma unroll
-    for (int j = 0; j < U; j++)
-      w[j] = u[j] != kNoVertex ? bm_cur[u[j] >> 5] : 0u;
-#pragma unroll
-    for (int j = 0; j < U; j++)
-      if (u[j] != kNoVertex && ((w[j] >> (u[j] & 31)) & 1u))
-        atomicMin(&first_hit, base + threadIdx.x + j * kT);
+  bool found = false, active = false;
+  if (f.cond(v)) {  // uniform across the block: only this block writes v
+    const uint32_t dv = g.deg[v];
+    const uint32_t *row = g.pool + g.off[v];
+    if (threadIdx.x < 3) hit[threadIdx.x] = kNoVertex;
     __syncthreads();
-    const uint32_t h = first_hit;
-    if (h != kNoVertex) {
-      if (threadIdx.x == 0) {
-        f.update(row[h], v);
-        keep[i] = 0;
+    uint32_t r = 0;
+    for (uint32_t base = 0, uu = 1; base < dv; base += kT * uu, uu = U, r++) {
+      uint32_t u[U], w[U];
+#pragma unroll
+      for (int j = 0; j < U; j++) {
+        const uint32_t k = base + threadIdx.x + j * kT;
+        u[j] = ((uint32_t)j < uu && k < dv) ? row[k] : kNoVertex;
       }
-      return;
+#pragma unroll
+      for (int j = 0; j < U; j++)
+        w[j] = u[j] != kNoVertex ? bm_cur[u[j] >> 5] : 0u;
+#pragma unroll
+      for (int j = 0; j < U; j++)
+        if (u[j] != kNoVertex && ((w[j] >> (u[j] & 31)) & 1u))
+          atomicMin(&hit[r % 3], base + threadIdx.x + j * kT);
+      if (threadIdx.x == 0) hit[(r + 1) % 3] = kNoVertex;
+      __syncthreads();
+      const uint32_t h = hit[r % 3];
+      if (h != kNoVertex) {
+        if (threadIdx.x == 0) f.update(row[h], v);
+        found = true;
+        break;
+      }
     }
+    active = !found && dv > 0;
   }
-  if (threadIdx.x == 0) keep[i] = dv > 0 ? 1 : 0;
+  if (threadIdx.x == 0) {  // one vertex: its own slot at list index lbase + i
+    if (found) tmp_f[lbase + i] = v;
+    if (active) tmp_k[lbase + i] = v;
+    pc[i] = ((uint64_t)(found ? 1u : 0u) << 32) | (active ? 1u : 0u);
+  }
 }
 
 // ------------------------------------------------------------ vertex pull
@@ -446,22 +478,19 @@ struct OnLevel {
 }  // namespace
 }  // namespace em
 
+// Buffers of a BFS: the frontier bitmap and the visited filter at slot and
+// slot + 1, the frontier degree prefix at slot + 2; both counter sets
+// cleared.
 inline int edge_map_ctx(sstg_graph *g, EdgeMapCtx *ctx, int slot) {
   ctx->words = ((uint64_t)g->n_rows + 31) / 32 + 1;
   ctx->slot = slot;
-  SSTG_TRY(g->ws.get(slot + 0, ctx->words, &ctx->bm_cur));
-  SSTG_TRY(g->ws.get(slot + 1, ctx->words, &ctx->bm_next));
-  ctx->ctr = (unsigned long long *)(g->dev_ctr + 9);
-  return SSTG_OK;
-}
-
-inline int frontier_to_bitmap(EdgeMapCtx &ctx, const uint32_t *q, uint64_t nq,
-                              hipStream_t s) {
-  SSTG_LAUNCH(em::k_clear_u32, ceil_div(ctx.words, em::kT), em::kT, 0, s,
-              ctx.bm_cur, ctx.words);
-  if (nq)
-    SSTG_LAUNCH(em::k_queue_to_bitmap, ceil_div(nq, em::kT), em::kT, 0, s, q,
-                nq, ctx.bm_cur);
+  SSTG_TRY(g->ws.get(slot + 0, ctx->words, &ctx->bm));
+  SSTG_TRY(g->ws.get(slot + 1, ctx->words, &ctx->vis));
+  SSTG_TRY(g->ws.get(kWsBottomUp + 2, 2 * em::kStepCtrs, &ctx->ctrs));
+  ctx->parity = 0;
+  SSTG_HIP(hipMemsetAsync(ctx->ctrs, 0,
+                          2 * em::kStepCtrs * sizeof(unsigned long long),
+                          g->stream));
   return SSTG_OK;
 }
 
@@ -472,34 +501,75 @@ inline int narrow_u64_to_u32(const uint64_t *in, uint64_t n, uint32_t *out,
   return SSTG_OK;
 }
 
-inline int read_result(sstg_graph *g, EdgeMapCtx &ctx, EdgeMapResult *res) {
-  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctx.ctr, 2 * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, g->stream));
+// Bitmap bm of the vertices v < n for which p(v) holds (clearing the counter
+// set ctr_clear when given).
+template <class P>
+int edge_map_bits(sstg_graph *g, EdgeMapCtx &ctx, P p, uint32_t *bm,
+                  unsigned long long *ctr_clear, uint64_t *zero1 = nullptr) {
+  const uint64_t waves = ceil_div(ctx.words * 32, (uint64_t)kWave);
+  const uint64_t nb =
+      ceil_div(waves, (uint64_t)(em::kT / kWave) * em::kBitmapRounds);
+  SSTG_LAUNCH(em::k_bits<P>, nb, em::kT, 0, g->stream, g->n_rows, p, bm,
+              ctx.words, ctr_clear, zero1);
+  return SSTG_OK;
+}
+
+// The counters of the step just launched on the counter set ctx.parity
+// (first k of them) into host_rb, one copy and one wait; then the other set
+// becomes current.
+inline int step_counters(sstg_graph *g, EdgeMapCtx &ctx, int k) {
+  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctx.ctrs + em::kStepCtrs * ctx.parity,
+                          k * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          g->stream));
   SSTG_HIP(hipStreamSynchronize(g->stream));
-  res->count = g->host_rb[0];
-  res->degree_sum = g->host_rb[1];
+  ctx.parity ^= 1;
+  return SSTG_OK;
+}
+
+// Exclusive degree prefix of the frontier front[0..nf) into ctx.fpre
+// (nf + 1 entries); with total given, the degree sum is read back into it
+// (one copy, one wait).
+inline int frontier_prefix(sstg_graph *g, EdgeMapCtx &ctx,
+                           const uint32_t *front, uint64_t nf,
+                           uint64_t *total) {
+  hipStream_t s = g->stream;
+  SSTG_TRY(g->ws.get(ctx.slot + 2, nf + 1, &ctx.fpre));
+  SSTG_LAUNCH(em::k_frontier_degrees, ceil_div(nf + 1, em::kT), em::kT, 0, s,
+              g->view(), front, nf, ctx.fpre);
+  SSTG_TRY(scan_exclusive_u64(ctx.fpre, ctx.fpre, nf + 1, nullptr, g->ws,
+                              kWsPrim2, s));
+  ctx.fpre_front = front;
+  ctx.fpre_n = nf;
+  if (total) {
+    SSTG_HIP(hipMemcpyAsync(g->host_rb + 8, ctx.fpre + nf, sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    SSTG_HIP(hipStreamSynchronize(s));
+    *total = g->host_rb[8];
+  }
   return SSTG_OK;
 }
 
 // Sparse push over the frontier queue front[0..nf) whose degree sum is mf;
-// the output frontier is appended at next.
+// the output frontier is appended at next, and vis takes its bits.
 template <class F>
 int edge_map_sparse(sstg_graph *g, EdgeMapCtx &ctx, const uint32_t *front,
                     uint64_t nf, uint64_t mf, F f, uint32_t *next,
                     EdgeMapResult *res) {
   hipStream_t s = g->stream;
-  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 2 * sizeof(uint64_t), s));
-  if (nf && mf) {
-    SSTG_TRY(g->ws.get(ctx.slot + 2, nf + 1, &ctx.fpre));
-    SSTG_LAUNCH(em::k_frontier_degrees, ceil_div(nf + 1, em::kT), em::kT, 0, s,
-                g->view(), front, nf, ctx.fpre);
-    SSTG_TRY(scan_exclusive_u64(ctx.fpre, ctx.fpre, nf + 1, nullptr, g->ws,
-                                kWsPrim2, s));
-    SSTG_LAUNCH(em::k_sparse_push<F>, ceil_div(ceil_div(mf, em::kChunk), em::kT),
-                em::kT, 0, s, g->view(), front, (const uint64_t *)ctx.fpre, nf,
-                mf, f, next, ctx.ctr);
-  }
-  return read_result(g, ctx, res);
+  res->count = res->degree_sum = 0;
+  if (!nf || !mf) return SSTG_OK;
+  if (ctx.fpre_front != front || ctx.fpre_n != nf)
+    SSTG_TRY(frontier_prefix(g, ctx, front, nf, nullptr));
+  ctx.fpre_front = nullptr;
+  unsigned long long *cur = ctx.ctrs + em::kStepCtrs * ctx.parity;
+  unsigned long long *other = ctx.ctrs + em::kStepCtrs * (1 - ctx.parity);
+  SSTG_LAUNCH(em::k_sparse_push<F>, ceil_div(ceil_div(mf, em::kChunk), em::kT),
+              em::kT, 0, s, g->view(), front, (const uint64_t *)ctx.fpre, nf,
+              mf, f, ctx.vis, next, cur, other);
+  SSTG_TRY(step_counters(g, ctx, 2));
+  res->count = g->host_rb[0];
+  res->degree_sum = g->host_rb[1];
+  return SSTG_OK;
 }
 
 // ----------------------------------------------------------- degree bins
@@ -566,70 +636,17 @@ inline int degree_bins(sstg_graph *g, DegreeBins *out, int set = kBinsPull) {
   return SSTG_OK;
 }
 
-namespace em {
-namespace {
-// Kept flags of each bin's segment of a bottom-up step, summed: counts[b] =
-// sum of keep[seg[b] .. seg[b + 1]).  Grid-stride per-block partials, then
-// one block adds them (no contended atomics).
-constexpr uint32_t kSegBlocks = 1024;
-
-__global__ void __launch_bounds__(kT)
-    k_segment_partials(const uint8_t *keep, const uint64_t *seg,
-                       unsigned long long *part) {
-  const uint64_t s1 = seg[1], s2 = seg[2], s3 = seg[3], total = seg[4];
-  uint64_t c[4] = {0, 0, 0, 0};
-  for (uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * kT) {
-    const uint64_t k = keep[i];
-    const int b = i < s1 ? 0 : i < s2 ? 1 : i < s3 ? 2 : 3;
-    c[b] += k;
-  }
-  __shared__ unsigned long long wc[4][kT / kWave];
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const uint64_t t = wave_sum(c[b]);
-    if (lane_id() == 0) wc[b][threadIdx.x / kWave] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    unsigned long long t = 0;
-    for (int w = 0; w < kT / kWave; w++) t += wc[threadIdx.x][w];
-    part[4 * (uint64_t)blockIdx.x + threadIdx.x] = t;
-  }
-}
-
-__global__ void __launch_bounds__(kT)
-    k_segment_sum(const unsigned long long *part, uint32_t nb,
-                  unsigned long long *counts) {
-  uint64_t c[4] = {0, 0, 0, 0};
-  for (uint32_t i = threadIdx.x; i < nb; i += kT)
-#pragma unroll
-    for (int b = 0; b < 4; b++) c[b] += part[4 * (uint64_t)i + b];
-  __shared__ unsigned long long wc[4][kT / kWave];
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    const uint64_t t = wave_sum(c[b]);
-    if (lane_id() == 0) wc[b][threadIdx.x / kWave] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    unsigned long long t = 0;
-    for (int w = 0; w < kT / kWave; w++) t += wc[threadIdx.x][w];
-    counts[threadIdx.x] = t;
-  }
-}
-}  // namespace
-}  // namespace em
-
-// Dense pull over the active vertices against ctx.bm_cur, split by degree
-// bin (1, 4, 16 lanes or a block per vertex; 16 lanes rather than a wave in
-// the third bin took BFS from the hub at rMat scale 24 from 2.71 to 2.52
-// ms, and 4 rather than 8 in the second to 2.49 ms); the output frontier is
-// ctx.bm_next, after which the two bitmaps swap roles.  The bins' active
-// lists lie back to back, so one compaction of the keep flags carries the
-// survivors of every bin to the next step.
-template <class F>
-int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
+// Dense pull over the active vertices, split by degree bin (1, 4, 16 lanes
+// or a block per vertex; 16 lanes rather than a wave in the third bin took
+// BFS from the hub at rMat scale 24 from 2.71 to 2.52 ms, and 4 rather than
+// 8 in the second to 2.49 ms), against the bitmap of the vertices for which
+// in_front holds.  Found vertices are appended at next (the BFS queue);
+// each bin's survivors go to its region of the other active list.  The
+// found vertices' degree sum is not computed (res->degree_sum = ~0): a
+// following top-down step takes it from frontier_prefix.
+template <class F, class P>
+int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, P in_front,
+                   uint32_t *next, EdgeMapResult *res) {
   hipStream_t s = g->stream;
   DegreeBins bins;
   SSTG_TRY(degree_bins(g, &bins, kBinsBfs));
@@ -639,70 +656,58 @@ int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, EdgeMapResult *res) {
   const int dst = first ? 0 : 1 - ctx.act_src;
   SSTG_TRY(g->ws.get(kWsBottomUp + 0, all + 1, &ctx.act[0]));
   SSTG_TRY(g->ws.get(kWsBottomUp + 1, all + 1, &ctx.act[1]));
-  // the first bottom-up step reads the degree bins; later ones the vertices
-  // the previous step left unvisited
-  const uint32_t *cur = first ? bins.list[0] : ctx.act[ctx.act_src];
-  uint64_t cnt[4], off[5];
-  off[0] = 0;
+  uint32_t *tmp_f = nullptr, *tmp_k = nullptr;
+  SSTG_TRY(g->ws.get(kWsBottomUp + 3, all + 1, &tmp_f));
+  SSTG_TRY(g->ws.get(kWsBottomUp + 8, all + 1, &tmp_k));
+  static constexpr uint32_t kPer[4] = {em::kT / 1, em::kT / 4, em::kT / 16, 1};
+  em::StepBlocks sb;
+  uint64_t cnt[4], nblk[4];
+  const uint32_t *cur[4];
+  sb.pbase[0] = 0;
   for (int b = 0; b < 4; b++) {
+    ctx.act_off[b] = b ? ctx.act_off[b - 1] + bins.count[b - 1] : 0;
     cnt[b] = first ? bins.count[b] : ctx.act_n[b];
-    off[b + 1] = off[b] + cnt[b];
+    cur[b] = first ? bins.list[b] : ctx.act[ctx.act_src] + ctx.act_off[b];
+    nblk[b] = ceil_div(cnt[b], (uint64_t)kPer[b]);
+    sb.lbase[b] = ctx.act_off[b];
+    sb.per[b] = kPer[b];
+    sb.pbase[b + 1] = sb.pbase[b] + nblk[b];
   }
-  const uint64_t total = off[4];
-  // ctr[0..1]: found count and degree sum; ctr[2]: survivors;
-  // ctr[3 + b]: survivors of bin b
-  SSTG_HIP(hipMemsetAsync(ctx.ctr, 0, 7 * sizeof(uint64_t), s));
-  uint8_t *keep = nullptr;
-  uint64_t *seg = nullptr;
-  SSTG_TRY(g->ws.get(kWsBottomUp + 8, all + 1, &keep));
-  SSTG_TRY(g->ws.get(kWsBottomUp + 2, 5, &seg));
-  for (int b = 0; b < 5; b++) g->host_rb[40 + b] = off[b];  // pinned
-  SSTG_HIP(hipMemcpyAsync(seg, g->host_rb + 40, sizeof(off),
-                          hipMemcpyHostToDevice, s));
+  const uint64_t nb_all = sb.pbase[4];
+  uint64_t *pc = nullptr;
+  SSTG_TRY(g->ws.get(kWsBottomUp + 9, nb_all + 1, &pc));
+  unsigned long long *ctr = ctx.ctrs + em::kStepCtrs * ctx.parity;
+  unsigned long long *other = ctx.ctrs + em::kStepCtrs * (1 - ctx.parity);
+  // the frontier bitmap (this launch also clears the next counter set and
+  // the scan's last entry)
+  SSTG_TRY(edge_map_bits(g, ctx, in_front, ctx.bm, other, pc + nb_all));
   DevGraph v = g->view();
-  const uint32_t *bm = ctx.bm_cur;
-  if (cnt[0])
-    SSTG_LAUNCH((em::k_dense_pull<F, 1, 8>), ceil_div(cnt[0], em::kT), em::kT,
-                0, s, v, cur + off[0], cnt[0], f, bm, keep + off[0]);
-  if (cnt[1])
-    SSTG_LAUNCH((em::k_dense_pull<F, 4, 8>), ceil_div(cnt[1] * 4, em::kT),
-                em::kT, 0, s, v, cur + off[1], cnt[1], f, bm, keep + off[1]);
-  if (cnt[2])
-    SSTG_LAUNCH((em::k_dense_pull<F, 16, 8>), ceil_div(cnt[2] * 16, em::kT),
-                em::kT, 0, s, v, cur + off[2], cnt[2], f, bm, keep + off[2]);
-  if (cnt[3])
-    SSTG_LAUNCH((em::k_dense_pull_block<F, 4>), cnt[3], em::kT, 0, s, v,
-                cur + off[3], cnt[3], f, bm, keep + off[3]);
-  if (total) {
-    SSTG_TRY(select_flagged_u32(keep, cur, total, ctx.act[dst],
-                                (uint64_t *)(ctx.ctr + 2), g->ws, kWsPrim, s));
-    const uint32_t nb = (uint32_t)std::min<uint64_t>(
-        ceil_div(total, (uint64_t)em::kT * 16), em::kSegBlocks);
-    unsigned long long *spart = nullptr;
-    SSTG_TRY(g->ws.get(kWsBottomUp + 3, 4 * (uint64_t)em::kSegBlocks, &spart));
-    SSTG_LAUNCH(em::k_segment_partials, nb, em::kT, 0, s, (const uint8_t *)keep,
-                (const uint64_t *)seg, spart);
-    SSTG_LAUNCH(em::k_segment_sum, 1, em::kT, 0, s,
-                (const unsigned long long *)spart, nb, ctx.ctr + 3);
+  const uint32_t *bm = ctx.bm;
+  if (nblk[0])
+    SSTG_LAUNCH((em::k_dense_pull<F, 1, 8>), nblk[0], em::kT, 0, s, v, cur[0],
+                cnt[0], f, bm, sb.lbase[0], tmp_f, tmp_k, pc + sb.pbase[0]);
+  if (nblk[1])
+    SSTG_LAUNCH((em::k_dense_pull<F, 4, 8>), nblk[1], em::kT, 0, s, v, cur[1],
+                cnt[1], f, bm, sb.lbase[1], tmp_f, tmp_k, pc + sb.pbase[1]);
+  if (nblk[2])
+    SSTG_LAUNCH((em::k_dense_pull<F, 16, 8>), nblk[2], em::kT, 0, s, v,
+                cur[2], cnt[2], f, bm, sb.lbase[2], tmp_f, tmp_k,
+                pc + sb.pbase[2]);
+  if (nblk[3])
+    SSTG_LAUNCH((em::k_dense_pull_block<F, 4>), nblk[3], em::kT, 0, s, v,
+                cur[3], cnt[3], f, bm, sb.lbase[3], tmp_f, tmp_k,
+                pc + sb.pbase[3]);
+  if (nb_all) {
+    SSTG_TRY(scan_exclusive_u64(pc, pc, nb_all + 1, nullptr, g->ws, kWsPrim,
+                                s));
+    SSTG_LAUNCH(em::k_step_place, nb_all, em::kPlaceT, 0, s, sb,
+                (const uint64_t *)pc, (const uint32_t *)tmp_f,
+                (const uint32_t *)tmp_k, next, ctx.act[dst], ctr);
   }
-  const uint64_t bm_waves = ceil_div(ctx.words * 32, (uint64_t)kWave);
-  const uint64_t bm_blocks =
-      ceil_div(bm_waves, (uint64_t)(em::kT / kWave) * em::kBitmapRounds);
-  unsigned long long *part = nullptr;
-  SSTG_TRY(g->ws.get(kWsBottomUp + 9, 2 * bm_blocks, &part));
-  SSTG_LAUNCH(em::k_bitmap_from<F>, bm_blocks, em::kT, 0, s, g->n_rows, f,
-              ctx.bm_next, ctx.words, v, part);
-  SSTG_LAUNCH(em::k_sum_partials, 1, em::kT, 0, s,
-              (const unsigned long long *)part, bm_blocks, ctx.ctr);
-  uint32_t *t = ctx.bm_cur;
-  ctx.bm_cur = ctx.bm_next;
-  ctx.bm_next = t;
-  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctx.ctr, 7 * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-  SSTG_HIP(hipStreamSynchronize(s));
+  SSTG_TRY(step_counters(g, ctx, 6));
   res->count = g->host_rb[0];
-  res->degree_sum = g->host_rb[1];
-  for (int b = 0; b < 4; b++) ctx.act_n[b] = g->host_rb[3 + b];
+  res->degree_sum = ~0ull;
+  for (int b = 0; b < 4; b++) ctx.act_n[b] = g->host_rb[2 + b];
   ctx.act_src = dst;
   return SSTG_OK;
 }

@@ -192,9 +192,7 @@ enum : int {
   kWsPrim = 48,      // 48..63 primitives (scan recursion etc.)
   kWsPrim2 = 64,     // 64..79 second primitive range
   kWsBottomUp = 112,  // 112, 113 active lists of the bottom-up steps,
-                      // 114 their bin offsets, 115 per-bin partial
-                      // counts, 120 keep flags, 121
-                      // per-block partial sums
+                      // 114 the BFS step counters
   kWsMultiOff = 116,  // multi store: batch offsets of a batched update
   kWsMultiBase = 117, // multi store: vertex base and size of each batch
   kWsBfsBins = 122,   // degree bin lists of the BFS set, back to back
