@@ -583,10 +583,19 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   if (!queue) SSTG_TRY(ws.get(kWsAlgo + 10, (uint64_t)n + 1, &queue));
   SSTG_LAUNCH(k_bfs_start, blocks(std::max<uint64_t>(n, ctx.words)), kT, 0, s,
               n, level, parent, src, queue, ctx.vis, ctx.words);
+  // the source's degree, and the degree mass of the upper half of the ids
+  // against the whole (rows are sorted by id, so when the high ids carry
+  // more edges -- as in the reference's rMat, whose hubs have mostly 1
+  // bits -- a frontier vertex is met sooner from a row's end)
   uint32_t src_deg = 0;
   SSTG_HIP(hipMemcpyAsync(&src_deg, g->deg + src, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipMemcpyAsync(g->host_rb + 8, prefix + n / 2, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+  SSTG_HIP(hipMemcpyAsync(g->host_rb + 9, prefix + n, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
   SSTG_HIP(hipStreamSynchronize(s));
+  ctx.rev = 2 * g->host_rb[8] < g->host_rb[9];
   uint64_t qs = 0, nf = 1, mf = src_deg;
   uint64_t explored = src_deg;  // edges of visited vertices
   bool bottom_up = false;

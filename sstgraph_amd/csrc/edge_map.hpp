@@ -44,6 +44,8 @@ struct EdgeMapCtx {
   uint64_t act_off[4] = {0, 0, 0, 0};
   uint64_t act_n[4] = {0, 0, 0, 0};
   int act_src = -1;
+  // bottom-up steps read rows from their end (set by the caller)
+  int rev = 0;
 };
 
 namespace em {
@@ -129,32 +131,32 @@ struct StepBlocks {
   uint32_t per[4];
 };
 
-// After the exclusive scan of the packed counts pc[0..pbase[4]]: one wave
-// per pull block moves its found vertices to next (the BFS queue, in block
-// order) and its survivors to its bin's region of keep.  The first block of
-// each bin stores the bin's survivor count in ctr[2 + b], block 0 the found
-// count in ctr[0].
+// After the exclusive scan of the packed counts pc[0..pbase[4]]: LPB lanes
+// per pull block of bin b (sized to the block's at most per[b] vertices)
+// move its found vertices to next (the BFS queue, in block order) and its
+// survivors to the bin's region of keep.  The first block of the bin stores
+// the bin's survivor count in ctr[2 + b], block 0 the found count in
+// ctr[0].
 constexpr int kPlaceT = 64;
+template <int LPB>
 __global__ void __launch_bounds__(kPlaceT)
-    k_step_place(StepBlocks sb, const uint64_t *pc, const uint32_t *tmp_f,
-                 const uint32_t *tmp_k, uint32_t *next, uint32_t *keep,
-                 unsigned long long *ctr) {
-  const uint64_t k = blockIdx.x;
-  int b = 0;
-  while (b < 3 && k >= sb.pbase[b + 1]) b++;
+    k_step_place(StepBlocks sb, int b, const uint64_t *pc,
+                 const uint32_t *tmp_f, const uint32_t *tmp_k, uint32_t *next,
+                 uint32_t *keep, unsigned long long *ctr) {
+  const uint64_t gk = ((uint64_t)blockIdx.x * kPlaceT + threadIdx.x) / LPB;
+  const uint32_t lig = threadIdx.x % LPB;
+  if (gk >= sb.pbase[b + 1] - sb.pbase[b]) return;
+  const uint64_t k = sb.pbase[b] + gk;
   const uint64_t p0 = pc[k], p1 = pc[k + 1], pb = pc[sb.pbase[b]];
   const uint32_t nfound = (uint32_t)(p1 >> 32) - (uint32_t)(p0 >> 32);
   const uint32_t nkeep = (uint32_t)p1 - (uint32_t)p0;
-  const uint64_t lb = sb.lbase[b] + (k - sb.pbase[b]) * sb.per[b];
+  const uint64_t lb = sb.lbase[b] + gk * sb.per[b];
   const uint64_t fdst = p0 >> 32;
   const uint64_t kdst = sb.lbase[b] + ((uint32_t)p0 - (uint32_t)pb);
-  for (uint32_t t = threadIdx.x; t < nfound; t += kPlaceT)
-    next[fdst + t] = tmp_f[lb + t];
-  for (uint32_t t = threadIdx.x; t < nkeep; t += kPlaceT)
-    keep[kdst + t] = tmp_k[lb + t];
-  if (threadIdx.x == 0) {
-    if (k == sb.pbase[b])
-      ctr[2 + b] = (uint32_t)pc[sb.pbase[b + 1]] - (uint32_t)pb;
+  for (uint32_t t = lig; t < nfound; t += LPB) next[fdst + t] = tmp_f[lb + t];
+  for (uint32_t t = lig; t < nkeep; t += LPB) keep[kdst + t] = tmp_k[lb + t];
+  if (lig == 0) {
+    if (gk == 0) ctr[2 + b] = (uint32_t)pc[sb.pbase[b + 1]] - (uint32_t)pb;
     if (k == 0) ctr[0] = pc[sb.pbase[4]] >> 32;
   }
 }
@@ -282,14 +284,17 @@ __global__ void __launch_bounds__(kT)
 // lane first, then U per lane per round: the U row elements, then their U
 // bitmap words, so each lane has U gathers in flight before it tests any of
 // them.  The first round with a frontier member ends the scan and the lowest
-// such lane's neighbour is the source.  Found vertices and the vertices
-// with neighbours that found none (survivors, active in the next bottom-up
-// step) leave through step_partial; list[i] has list index lbase + i.
+// such lane's neighbour is the source.  With rev the rows are read from
+// their end (largest ids first): any frontier neighbour is a valid parent,
+// so the order only decides how soon one is met.  Found vertices and the
+// vertices with neighbours that found none (survivors, active in the next
+// bottom-up step) leave through step_partial; list[i] has list index
+// lbase + i.
 template <class F, int G, int U>
 __global__ void __launch_bounds__(kT)
     k_dense_pull(DevGraph g, const uint32_t *list, uint64_t count, F f,
-                 const uint32_t *bm_cur, uint64_t lbase, uint32_t *tmp_f,
-                 uint32_t *tmp_k, uint64_t *pc) {
+                 const uint32_t *bm_cur, int rev, uint64_t lbase,
+                 uint32_t *tmp_f, uint32_t *tmp_k, uint64_t *pc) {
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t gid = (uint64_t)blk * kT + threadIdx.x;
   const uint32_t lig = (uint32_t)(gid % G);
@@ -312,7 +317,8 @@ __global__ void __launch_bounds__(kT)
 #pragma unroll
         for (int j = 0; j < U; j++) {
           const uint32_t k = base + lig + j * G;
-          u[j] = ((uint32_t)j < uu && k < dv) ? row[k] : kNoVertex;
+          u[j] = ((uint32_t)j < uu && k < dv) ? row[rev ? dv - 1 - k : k]
+                                              : kNoVertex;
         }
 #pragma unroll
         for (int j = 0; j < U; j++)
@@ -346,8 +352,8 @@ __global__ void __launch_bounds__(kT)
 template <class F, int U>
 __global__ void __launch_bounds__(kT)
     k_dense_pull_block(DevGraph g, const uint32_t *list, uint64_t count, F f,
-                       const uint32_t *bm_cur, uint64_t lbase, uint32_t *tmp_f,
-                       uint32_t *tmp_k, uint64_t *pc) {
+                       const uint32_t *bm_cur, int rev, uint64_t lbase,
+                       uint32_t *tmp_f, uint32_t *tmp_k, uint64_t *pc) {
   __shared__ uint32_t hit[3];
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t v = list[i];
@@ -363,7 +369,8 @@ __global__ void __launch_bounds__(kT)
 #pragma unroll
       for (int j = 0; j < U; j++) {
         const uint32_t k = base + threadIdx.x + j * kT;
-        u[j] = ((uint32_t)j < uu && k < dv) ? row[k] : kNoVertex;
+        u[j] = ((uint32_t)j < uu && k < dv) ? row[rev ? dv - 1 - k : k]
+                                            : kNoVertex;
       }
 #pragma unroll
       for (int j = 0; j < U; j++)
@@ -376,7 +383,7 @@ __global__ void __launch_bounds__(kT)
       __syncthreads();
       const uint32_t h = hit[r % 3];
       if (h != kNoVertex) {
-        if (threadIdx.x == 0) f.update(row[h], v);
+        if (threadIdx.x == 0) f.update(row[rev ? dv - 1 - h : h], v);
         found = true;
         break;
       }
@@ -685,24 +692,36 @@ int edge_map_dense(sstg_graph *g, EdgeMapCtx &ctx, F f, P in_front,
   const uint32_t *bm = ctx.bm;
   if (nblk[0])
     SSTG_LAUNCH((em::k_dense_pull<F, 1, 8>), nblk[0], em::kT, 0, s, v, cur[0],
-                cnt[0], f, bm, sb.lbase[0], tmp_f, tmp_k, pc + sb.pbase[0]);
+                cnt[0], f, bm, ctx.rev, sb.lbase[0], tmp_f, tmp_k, pc + sb.pbase[0]);
   if (nblk[1])
     SSTG_LAUNCH((em::k_dense_pull<F, 4, 8>), nblk[1], em::kT, 0, s, v, cur[1],
-                cnt[1], f, bm, sb.lbase[1], tmp_f, tmp_k, pc + sb.pbase[1]);
+                cnt[1], f, bm, ctx.rev, sb.lbase[1], tmp_f, tmp_k, pc + sb.pbase[1]);
   if (nblk[2])
     SSTG_LAUNCH((em::k_dense_pull<F, 16, 8>), nblk[2], em::kT, 0, s, v,
-                cur[2], cnt[2], f, bm, sb.lbase[2], tmp_f, tmp_k,
+                cur[2], cnt[2], f, bm, ctx.rev, sb.lbase[2], tmp_f, tmp_k,
                 pc + sb.pbase[2]);
   if (nblk[3])
     SSTG_LAUNCH((em::k_dense_pull_block<F, 4>), nblk[3], em::kT, 0, s, v,
-                cur[3], cnt[3], f, bm, sb.lbase[3], tmp_f, tmp_k,
+                cur[3], cnt[3], f, bm, ctx.rev, sb.lbase[3], tmp_f, tmp_k,
                 pc + sb.pbase[3]);
   if (nb_all) {
     SSTG_TRY(scan_exclusive_u64(pc, pc, nb_all + 1, nullptr, g->ws, kWsPrim,
                                 s));
-    SSTG_LAUNCH(em::k_step_place, nb_all, em::kPlaceT, 0, s, sb,
-                (const uint64_t *)pc, (const uint32_t *)tmp_f,
-                (const uint32_t *)tmp_k, next, ctx.act[dst], ctr);
+    const uint32_t *tf = tmp_f, *tk = tmp_k;
+    uint32_t *keep = ctx.act[dst];
+    const uint64_t pt = em::kPlaceT;
+    if (nblk[0])
+      SSTG_LAUNCH(em::k_step_place<64>, ceil_div(nblk[0] * 64, pt), pt, 0, s,
+                  sb, 0, (const uint64_t *)pc, tf, tk, next, keep, ctr);
+    if (nblk[1])
+      SSTG_LAUNCH(em::k_step_place<16>, ceil_div(nblk[1] * 16, pt), pt, 0, s,
+                  sb, 1, (const uint64_t *)pc, tf, tk, next, keep, ctr);
+    if (nblk[2])
+      SSTG_LAUNCH(em::k_step_place<4>, ceil_div(nblk[2] * 4, pt), pt, 0, s,
+                  sb, 2, (const uint64_t *)pc, tf, tk, next, keep, ctr);
+    if (nblk[3])
+      SSTG_LAUNCH(em::k_step_place<1>, ceil_div(nblk[3], pt), pt, 0, s, sb, 3,
+                  (const uint64_t *)pc, tf, tk, next, keep, ctr);
   }
   SSTG_TRY(step_counters(g, ctx, 6));
   res->count = g->host_rb[0];
