@@ -252,19 +252,25 @@ struct DeltaPull {
   }
 };
 
-// Brandes pulls over one BFS level's vertices with the levels as bytes (a
-// graph of fewer than 64 levels): a 16 MB level array at rMat scale 24
-// instead of 64 MB, so the per-edge level gather mostly hits the cache;
-// sigma and delta are gathered only for the parents (forward) and children
-// (backward) it selects.
-struct SigmaPull8 {
-  const uint8_t *lev;
+// Brandes pulls over one BFS level's vertices (a graph of fewer than 64
+// levels) against a bitmap per level: the per-edge test reads a 2 MB bitmap
+// at rMat scale 24, which stays in an XCD's L2, where a byte per vertex
+// (16 MB) or the u32 levels (64 MB) miss it; sigma and the quotients are
+// gathered only for the parents (forward) and children (backward) it
+// selects.  (Keeping the values of each level densely by rank within the
+// level, one more prefix load per selected edge, measured slower: 0.64
+// against 0.57 ms for the 16-lane sigma pull of a level at scale 24.)
+struct SigmaPullBm {
+  const uint32_t *prev;  // bitmap of level L - 1
   double *sigma;
-  uint32_t L;  // level of the vertices being computed
   using Acc = double;
   __device__ __forceinline__ Acc zero() const { return 0.0; }
-  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t u) const {
-    if ((uint32_t)lev[u] + 1 == L) a += sigma[u];
+  // branch-free: a non-parent reads v's own (cached) slot, so the gathers
+  // of the unrolled neighbours can all be in flight at once
+  __device__ __forceinline__ void add(Acc &a, uint32_t v, uint32_t u) const {
+    const bool p = (prev[u >> 5] >> (u & 31)) & 1u;
+    const double x = sigma[p ? u : v];
+    a += p ? x : 0.0;
   }
   __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
   __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
@@ -280,15 +286,16 @@ struct SigmaPull8 {
 // (1 + delta) / sigma, which its parents' pull (one level up, the next
 // launch) gathers: one 8-byte value and no divide per child.  Within a
 // launch only level-L slots are written and only level-(L+1) slots read.
-struct DeltaPull8 {
-  const uint8_t *lev;
+struct DeltaPullBm {
+  const uint32_t *next;  // bitmap of level L + 1
   double *q;  // sigma below level L+1, (1 + delta) / sigma from L+1 down
   double *delta;
-  uint32_t L;
   using Acc = double;
   __device__ __forceinline__ Acc zero() const { return 0.0; }
-  __device__ __forceinline__ void add(Acc &a, uint32_t, uint32_t w) const {
-    if ((uint32_t)lev[w] == L + 1) a += q[w];
+  __device__ __forceinline__ void add(Acc &a, uint32_t v, uint32_t w) const {
+    const bool c = (next[w >> 5] >> (w & 31)) & 1u;
+    const double x = q[c ? w : v];
+    a += c ? x : 0.0;
   }
   __device__ __forceinline__ Acc merge(Acc a, Acc b) const { return a + b; }
   __device__ static __forceinline__ Acc shfl_xor(Acc a, int o) {
@@ -308,20 +315,33 @@ __global__ void k_bc_last_quotients(const uint32_t *list, uint64_t count,
   if (i < count) q[list[i]] = 1.0 / q[list[i]];
 }
 
-__global__ void k_level_bytes(const uint32_t *level, uint32_t n, uint8_t *lev) {
+// bm + l * words: the bitmap of level l, for l < levels; one lane per
+// vertex, a ballot per level and 64 vertices (no atomics)
+__global__ void k_level_bitmaps(const uint32_t *level, uint32_t n,
+                                uint32_t levels, uint32_t *bm,
+                                uint64_t words) {
+  const uint32_t lane = lane_id();
   const uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (v < n) lev[v] = level[v] < 255u ? (uint8_t)level[v] : (uint8_t)255;
+  const uint32_t lv = v < n ? level[v] : kInf;
+  const uint64_t w = v >> 5;
+  for (uint32_t l = 0; l < levels; l++) {
+    const uint64_t m = __ballot(lv == l);
+    if ((lane & 31) == 0 && w < words)
+      bm[l * words + w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+  }
 }
 
 // Sort key of the i-th reached vertex: (level * 4 + lane class) << 32 | v,
-// class 0: degree <= t1 (4 lanes), 1: <= t2 (16 lanes), 2: larger (a block)
+// the classes of the reductions' degree bins: 0 degree <= t0 (1 lane),
+// 1 <= t1 (8 lanes), 2 <= t2 (16 lanes), 3 larger (a block)
 __global__ void k_bc_keys(const uint32_t *order, uint64_t count,
                           const uint32_t *level, const uint32_t *deg,
-                          uint32_t t1, uint32_t t2, uint64_t *key) {
+                          uint32_t t0, uint32_t t1, uint32_t t2,
+                          uint64_t *key) {
   const uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= count) return;
   const uint32_t v = order[i], d = deg[v];
-  const uint32_t c = d <= t1 ? 0u : d <= t2 ? 1u : 2u;
+  const uint32_t c = d <= t0 ? 0u : d <= t1 ? 1u : d <= t2 ? 2u : 3u;
   key[i] = ((uint64_t)(level[v] * 4u + c) << 32) | v;
 }
 
@@ -876,8 +896,9 @@ int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
 // ====================================================================== BC
 // Brandes from src on a graph of fewer than 64 BFS levels: the reached
 // vertices grouped by (level, lane class) with one radix pass, so each
-// level's pulls run over exactly its vertices (4, 16 lanes or a block by
-// degree) instead of whole degree bins filtered by level.
+// level's pulls run over exactly its vertices (1, 8, 16 lanes or a block by
+// the reductions' degree bins) instead of whole degree bins filtered by
+// level.
 static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
                       const uint32_t *order, const BfsRun &run, double *sigma,
                       double *out) {
@@ -886,15 +907,18 @@ static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
   const uint64_t reached = run.reached;
   const uint32_t nb = run.levels * 4;
   uint64_t *key = nullptr, *tmp = nullptr, *start = nullptr;
-  uint8_t *lev = nullptr;
+  uint32_t *bm = nullptr;
+  const uint64_t words = ((uint64_t)n + 31) / 32 + 1;
   SSTG_TRY(g->ws.get(kWsAlgo + 16, reached + 1, &key));
   SSTG_TRY(g->ws.get(kWsAlgo + 17, reached + 1, &tmp));
   SSTG_TRY(g->ws.get(kWsAlgo + 18, (uint64_t)nb + 1, &start));
-  SSTG_TRY(g->ws.get(kWsAlgo + 19, n, &lev));
-  const uint32_t t1 = g->bin_hi[kBinsPull][1], t2 = g->bin_hi[kBinsPull][2];
-  SSTG_LAUNCH(k_level_bytes, blocks(n), kT, 0, s, level, n, lev);
+  SSTG_TRY(g->ws.get(kWsAlgo + 19, run.levels * words, &bm));
+  const uint32_t t0 = g->bin_hi[kBinsPull][0], t1 = g->bin_hi[kBinsPull][1],
+                 t2 = g->bin_hi[kBinsPull][2];
+  SSTG_LAUNCH(k_level_bitmaps, blocks(words * 32), kT, 0, s, level, n,
+              run.levels, bm, words);
   SSTG_LAUNCH(k_bc_keys, blocks(reached), kT, 0, s, order, reached, level,
-              (const uint32_t *)g->deg, t1, t2, key);
+              (const uint32_t *)g->deg, t0, t1, t2, key);
   uint64_t *sorted = key;
   const int shifts[1] = {32};
   SSTG_TRY(radix_sort_u64(key, tmp, reached, shifts, 1, g->ws, kWsPrim, s,
@@ -910,33 +934,37 @@ static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
   SSTG_HIP(hipStreamSynchronize(s));
   const DevGraph v = g->view();
   auto pull = [&](uint32_t L, auto f) -> int {
-    const uint64_t b0 = st[L * 4], b1 = st[L * 4 + 1], b2 = st[L * 4 + 2],
-                   b3 = st[L * 4 + 3];
-    if (b1 > b0)
-      SSTG_LAUNCH((em::k_vertex_pull<decltype(f), 4>), ceil_div((b1 - b0) * 4, kT),
-                  kT, 0, s, v, (const uint32_t *)(list + b0), b1 - b0, f);
-    if (b2 > b1)
-      SSTG_LAUNCH((em::k_vertex_pull<decltype(f), 16>),
-                  ceil_div((b2 - b1) * 16, kT), kT, 0, s, v,
-                  (const uint32_t *)(list + b1), b2 - b1, f);
-    if (b3 > b2)
-      SSTG_LAUNCH((em::k_vertex_pull<decltype(f), 256>), b3 - b2, kT, 0, s, v,
-                  (const uint32_t *)(list + b2), b3 - b2, f);
+    using Fn = decltype(f);
+    const uint64_t *b = st.data() + L * 4;  // b[0..4]: the level's classes
+    const uint32_t *lst = list;
+    if (b[1] > b[0])
+      SSTG_LAUNCH((em::k_vertex_pull<Fn, 1>), ceil_div(b[1] - b[0], kT), kT, 0,
+                  s, v, lst + b[0], b[1] - b[0], f);
+    if (b[2] > b[1])
+      SSTG_LAUNCH((em::k_vertex_pull<Fn, 8>), ceil_div((b[2] - b[1]) * 8, kT),
+                  kT, 0, s, v, lst + b[1], b[2] - b[1], f);
+    if (b[3] > b[2])
+      SSTG_LAUNCH((em::k_vertex_pull<Fn, 16>),
+                  ceil_div((b[3] - b[2]) * 16, kT), kT, 0, s, v, lst + b[2],
+                  b[3] - b[2], f);
+    if (b[4] > b[3])
+      SSTG_LAUNCH((em::k_vertex_pull<Fn, 256>), b[4] - b[3], kT, 0, s, v,
+                  lst + b[3], b[4] - b[3], f);
     return SSTG_OK;
   };
   SSTG_LAUNCH(k_bc_init, blocks(n), kT, 0, s, n, sigma, out, src);
   // forward: sigma from the parents, level by level
   for (uint32_t L = 1; L < run.levels; L++)
-    SSTG_TRY(pull(L, SigmaPull8{lev, sigma, L}));
+    SSTG_TRY(pull(L, SigmaPullBm{bm + (L - 1) * words, sigma}));
   // backward: dependencies from the children, the deepest level (no
   // children, delta 0) only turned into quotients
   const uint32_t last = run.levels - 1;
-  const uint64_t lb = st[last * 4], le = st[last * 4 + 3];
+  const uint64_t lb = st[last * 4], le = st[last * 4 + 4];
   if (le > lb)
     SSTG_LAUNCH(k_bc_last_quotients, ceil_div(le - lb, kT), kT, 0, s,
                 (const uint32_t *)(list + lb), le - lb, sigma);
   for (int64_t L = (int64_t)run.levels - 2; L >= 0; L--)
-    SSTG_TRY(pull((uint32_t)L, DeltaPull8{lev, sigma, out, (uint32_t)L}));
+    SSTG_TRY(pull((uint32_t)L, DeltaPullBm{bm + (L + 1) * words, sigma, out}));
   SSTG_HIP(hipStreamSynchronize(s));
   return SSTG_OK;
 }
