@@ -1037,7 +1037,7 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   for (uint32_t b = 0; b < cb; b += 8) shifts[np++] = (int)b;
   for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
   uint64_t *sorted = nullptr;
-  SSTG_TRY(radix_sort_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  SSTG_TRY(sort_keys_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
   uint32_t *pos = nullptr, *B_pos = nullptr;
   SSTG_TRY(ws.get(kWsMarkPos, n, &pos));
   SSTG_TRY(ws.get(kWsBPos, n, &B_pos));
@@ -1245,7 +1245,7 @@ int graph_effective_keys(sstg_graph *g, const uint32_t *src,
   for (uint32_t b = 0; b < cb; b += 8) shifts[np++] = (int)b;
   for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
   uint64_t *sorted = nullptr;
-  SSTG_TRY(radix_sort_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  SSTG_TRY(sort_keys_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
   SSTG_LAUNCH(k_mark, blocks(n), kT, 0, s, (const uint64_t *)sorted, n,
               g->view(), insert ? 1 : 0, skip_self ? 1 : 0, flags, pos);
   SSTG_TRY(select_flagged_u64(flags, sorted, n, out, ctr + kCtrB, ws, kWsPrim,
