@@ -118,7 +118,9 @@ __global__ void k_lower_degree(DevGraph g, uint32_t *lower) {
   uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (v >= g.n) return;
   uint32_t d = g.deg[v];
-  lower[v] = d ? lower_bound_8(g.pool + g.off[v], d, (uint32_t)v) : 0;
+  lower[v] = d ? (uint32_t)lower_bound_dev(g.pool + g.off[v], (uint64_t)d,
+                                           (uint32_t)v)
+               : 0;
 }
 
 // |a[0..na) n b[0..nb)| for sorted unique arrays: binary-search the shorter
@@ -393,7 +395,7 @@ __global__ void k_symmetry(DevGraph g, const uint64_t *prefix, uint64_t m,
     if (v < g.n) {
       uint32_t dv = g.deg[v];
       const uint32_t *row = g.pool + g.off[v];
-      uint64_t p = dv ? lower_bound_8(row, dv, (uint32_t)u) : 0;
+      uint64_t p = dv ? lower_bound_dev(row, (uint64_t)dv, (uint32_t)u) : 0;
       ok = p < dv && row[p] == (uint32_t)u;
     }
     if (!ok) *flag = 0;

@@ -143,41 +143,6 @@ __device__ __forceinline__ uint64_t lower_bound_dev(const T *a, uint64_t len,
   return lo;
 }
 
-// lower_bound over a sorted row a[0..len) of ids (first index with
-// a[i] >= x): each round issues seven probes together and keeps the eighth
-// of the range they bracket, and the last at most eight candidates are read
-// together, so a row of d ids costs about log8(d) + 1 dependent rounds of
-// loads instead of log2(d).
-__device__ __forceinline__ uint32_t lower_bound_8(const uint32_t *a,
-                                                  uint32_t len, uint32_t x) {
-  uint32_t lo = 0, hi = len;  // a[0..lo) < x <= a[hi..len)
-  while (hi - lo > 8) {
-    const uint64_t span = hi - lo;
-    uint32_t p[7], v[7];
-#pragma unroll
-    for (int j = 0; j < 7; j++) p[j] = lo + (uint32_t)((span * (j + 1)) >> 3);
-#pragma unroll
-    for (int j = 0; j < 7; j++) v[j] = a[p[j]];
-    uint32_t nlo = lo, nhi = hi;
-#pragma unroll
-    for (int j = 0; j < 7; j++)  // the probes below x are a prefix
-      if (v[j] < x) nlo = p[j] + 1;
-#pragma unroll
-    for (int j = 6; j >= 0; j--)
-      if (v[j] >= x) nhi = p[j];
-    lo = nlo;
-    hi = nhi;
-  }
-  if (lo == hi) return lo;
-  uint32_t below = 0;
-#pragma unroll
-  for (uint32_t j = 0; j < 8; j++) {
-    const uint32_t i = lo + j < hi ? lo + j : hi - 1;  // in bounds
-    below += (lo + j < hi && a[i] < x) ? 1u : 0u;
-  }
-  return lo + below;
-}
-
 // index of the segment containing flat position x, given exclusive offsets
 // off[0..k) (off[0] == 0, non-decreasing): largest s with off[s] <= x.
 __device__ __forceinline__ uint64_t find_segment(const uint64_t *off,

@@ -190,7 +190,7 @@ __global__ void k_wt_get(DevGraph g, WtTable t, int has_table,
   if (r < g.n) {
     const uint32_t d = g.deg[r];
     const uint32_t *row = g.pool + g.off[r];
-    const uint64_t p = lower_bound_8(row, d, c);
+    const uint64_t p = lower_bound_dev(row, (uint64_t)d, c);
     if (p < d && row[p] == c) {
       w = 1;
       if (has_table) {
