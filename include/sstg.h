@@ -335,11 +335,11 @@ int sstg_statistics(sstg_graph *g, sstg_stats *out);
 /* ---- degree thresholds and their autotuner ------------------------------
  * No reference counterpart (the reference's tinyset thresholds are fixed
  * by its PMA layout, TinySet_small.hpp:216-293).  The vertex-pull kernels
- * give a vertex 1 lane if its degree is <= t[0], 8 lanes (4 in the BFS steps,
- * BC's levels and the batched store's BFS) if <= t[1], 16 lanes (a wave in
- * TC's orientation) if
- * <= t[2], else a 256-thread block (defaults 8, 64, 2048; t strictly
- * increasing).  Two sets per graph: the reductions (PageRank, BC, TC's
+ * give a vertex 1 lane if its degree is <= t[0], 8 lanes (4 in the BFS steps
+ * and the batched store's BFS) if <= t[1], 16 lanes (a wave in TC's
+ * orientation) if
+ * <= t[2], else a 256-thread block (defaults 8, 64, 256 for the reductions
+ * and 32, 128, 2048 for BFS; t strictly increasing).  Two sets per graph: the reductions (PageRank, BC, TC's
  * orientation, the batched store's BFS levels) and BFS (bottom-up steps).  Mutating
  * calls. */
 int sstg_set_degree_thresholds(sstg_graph *g, const uint32_t t[3]);

@@ -581,9 +581,10 @@ int edge_map_sparse(sstg_graph *g, EdgeMapCtx &ctx, const uint32_t *front,
 
 // ----------------------------------------------------------- degree bins
 // Vertices split by degree (cached per graph version and threshold set):
-// bin 0 deg <= t[0] (one lane), bin 1 <= t[1] (8 lanes), bin 2 <= t[2] (a
-// wave), bin 3 larger (a block), t = bin_hi[set]; the boundaries default to
-// 8 / 64 / 2048 and are set per graph by the autotuner.
+// bin 0 deg <= t[0] (one lane), bin 1 <= t[1] (8 lanes), bin 2 <= t[2] (16
+// lanes), bin 3 larger (a block), t = bin_hi[set]; the boundaries default to
+// 8 / 64 / 256 for the reductions and 32 / 128 / 2048 for the BFS steps and
+// are set per graph by the autotuner.
 struct DegreeBins {
   uint32_t *list[4] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t count[4] = {0, 0, 0, 0};

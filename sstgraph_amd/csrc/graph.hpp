@@ -36,10 +36,12 @@ struct sstg_graph {
   // the vertex-pull reductions (PageRank, BC, TC orientation, the batched
   // store's BFS levels), [kBinsBfs] for the single-source BFS steps.  Bin b holds degrees in
   // (bin_hi[set][b-1], bin_hi[set][b]], the last bin the rest; tune.hip sets
-  // the boundaries, tune_epoch counts the changes
+  // the boundaries, tune_epoch counts the changes.  The defaults are what
+  // the tuner picks on rMat at scales 22 and 24 (both sets, both seeds
+  // measured)
   uint64_t bins_version[2] = {~0ull, ~0ull};
   uint64_t bins_count[2][4] = {};
-  uint32_t bin_hi[2][3] = {{8, 64, 2048}, {8, 64, 2048}};
+  uint32_t bin_hi[2][3] = {{8, 64, 256}, {32, 128, 2048}};
   uint64_t tune_epoch = 0;
   // BFS direction switch (Beamer): top-down -> bottom-up when the frontier
   // grows and its edges exceed unexplored / alpha; back when it holds fewer

@@ -1,16 +1,17 @@
 // Autotuner for the per-graph launch settings.
 //
 // The vertex-pull kernels give each vertex 1 lane, 8 lanes (4 in the BFS
-// steps, BC's levels and the batched store's BFS levels), 16 lanes (a wave
-// in TC's orientation) or a whole
+// steps and the batched store's BFS levels), 16 lanes (a wave in TC's
+// orientation) or a whole
 // 256-thread block by its degree, with one set of degree boundaries for the
 // reductions (PageRank, BC, the TC orientation, the batched store's BFS
 // levels) and one for single-source BFS (the bottom-up steps), and BFS
 // switches between top-down and bottom-up steps by two ratios.  Where these
 // pay off depends on the degree distribution and on how the rows sit in
-// HBM, so they are per-graph settings (degree boundaries 8 / 64 / 2048,
-// alpha 15, beta 18 by default) that the tuner picks by timing the graph's
-// own algorithms:
+// HBM, so they are per-graph settings (degree boundaries 8 / 64 / 256 for
+// the reductions and 32 / 128 / 2048 for BFS, alpha 15, beta 18 by default:
+// the tuner's picks on rMat at scales 22 and 24) that the tuner picks by
+// timing the graph's own algorithms:
 //  * the vertex-pull set is tuned on PageRank iterations (SSTG_TUNE_PR),
 //    the BFS set and the direction switch on one BFS from a given source
 //    (SSTG_TUNE_BFS), each timing the minimum of three runs after an

@@ -45,10 +45,15 @@ def _check_all(g, off, cols, what):
     assert ok, f"{what}: PR rel err {err}"
 
 
+# the library's defaults (the tuner's picks on rMat at scales 22 and 24)
+PULL_DEFAULT = [8, 64, 256]
+BFS_DEFAULT = [32, 128, 2048]
+
+
 def test_threshold_settings_keep_results_exact(g14):
     g, off, cols = g14
-    assert g.degree_thresholds() == [8, 64, 2048]
-    assert g.degree_thresholds(bfs=True) == [8, 64, 2048]
+    assert g.degree_thresholds() == PULL_DEFAULT
+    assert g.degree_thresholds(bfs=True) == BFS_DEFAULT
     sets = ([1, 16, 256], [32, 512, 65536], [4, 32, 1024], [8, 64, 2048])
     for t, tb in zip(sets, sets[::-1]):
         say("thresholds", t, "bfs", tb)
@@ -57,7 +62,8 @@ def test_threshold_settings_keep_results_exact(g14):
         assert g.degree_thresholds() == t
         assert g.degree_thresholds(bfs=True) == tb
         _check_all(g, off, cols, f"{t} / bfs {tb}")
-    g.set_degree_thresholds([8, 64, 2048], bfs=True)
+    g.set_degree_thresholds(PULL_DEFAULT)
+    g.set_degree_thresholds(BFS_DEFAULT, bfs=True)
 
 
 def test_bfs_switch_settings_keep_levels_exact(g14):
@@ -92,8 +98,8 @@ def test_invalid_thresholds_rejected(g14):
             g.set_degree_thresholds(t)
         with pytest.raises(S.SSTGError):
             g.set_degree_thresholds(t, bfs=True)
-    assert g.degree_thresholds() == [8, 64, 2048]
-    assert g.degree_thresholds(bfs=True) == [8, 64, 2048]
+    assert g.degree_thresholds() == PULL_DEFAULT
+    assert g.degree_thresholds(bfs=True) == BFS_DEFAULT
     with pytest.raises(S.SSTGError):
         g.autotune(pr=False, bfs_source=None)
     with pytest.raises(S.IdOutOfRange):
@@ -109,8 +115,8 @@ def test_autotune_picks_a_valid_faster_setting(g14):
     assert t[0] < t[1] < t[2] and tb[0] < tb[1] < tb[2]
     assert g.degree_thresholds() == t
     assert g.degree_thresholds(bfs=True) == tb
-    assert rep["start_thresholds"] == [8, 64, 2048]
-    assert rep["start_bfs_thresholds"] == [8, 64, 2048]
+    assert rep["start_thresholds"] == PULL_DEFAULT
+    assert rep["start_bfs_thresholds"] == BFS_DEFAULT
     assert (rep["start_alpha"], rep["start_beta"]) == (15.0, 18.0)
     assert g.bfs_switch() == (rep["alpha"], rep["beta"])
     assert rep["evaluations"] >= 20
@@ -120,8 +126,8 @@ def test_autotune_picks_a_valid_faster_setting(g14):
     # a PageRank-only tuning leaves the BFS settings alone
     rep2 = g.autotune(pr=True, bfs_source=None)
     assert rep2["bfs_thresholds"] == tb and rep2["bfs_start_ms"] == 0
-    g.set_degree_thresholds([8, 64, 2048])
-    g.set_degree_thresholds([8, 64, 2048], bfs=True)
+    g.set_degree_thresholds(PULL_DEFAULT)
+    g.set_degree_thresholds(BFS_DEFAULT, bfs=True)
     g.set_bfs_switch(15, 18)
 
 
