@@ -759,6 +759,20 @@ def main():
                                gbs_model=(4 * m_r + 12 * n) / t / 1e9)
             log(f"{name} (src {src_v}): {1e3 * t:.3f} ms, reached {int(reached.sum())}, "
                 f"{m_r / t / 1e9:.2f} GTEPS")
+        # the same BFS from the hub replayed from its recording (opt-in,
+        # SSTG_GRAPH_CAPTURE=1): the first call records, the timed calls replay
+        os.environ["SSTG_GRAPH_CAPTURE"] = "1"
+        try:
+            EMV.BFS(g, hub, out=(d_par, d_lev))
+            S.synchronize()
+            tb = [timed(lambda: EMV.BFS(g, hub, out=(d_par, d_lev))) for _ in range(5)]
+        finally:
+            del os.environ["SSTG_GRAPH_CAPTURE"]
+        t = float(np.median(tb))
+        ok = bool(np.array_equal(d_lev.to_host(), level))
+        extra["bfs_hub_replayed"] = dict(src=hub, ms=1e3 * t, levels_match=ok,
+                                         gteps=extra["bfs_hub"]["m_reached"] / t / 1e9)
+        log(f"bfs_hub replayed: {1e3 * t:.3f} ms (levels {'match' if ok else 'DIFFER'})")
         d_pr = S.DeviceArray(n, np.float64)
         EMV.PR_S(g, args.pr_iters, out=d_pr)
         S.synchronize()
@@ -793,8 +807,8 @@ def main():
                 f"alpha {rep['alpha']:g} beta {rep['beta']:g} ({rep['evaluations']} "
                 f"settings): BFS hub {1e3 * tb:.3f} ms, PR x{args.pr_iters} "
                 f"{1e3 * tp:.2f} ms")
-            g.set_degree_thresholds([8, 64, 2048])
-            g.set_degree_thresholds([8, 64, 2048], bfs=True)
+            g.set_degree_thresholds([8, 64, 256])
+            g.set_degree_thresholds([32, 128, 2048], bfs=True)
             g.set_bfs_switch(15, 18)
         for d in (d_par, d_lev, d_pr):
             d.free()

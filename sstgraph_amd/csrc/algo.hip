@@ -75,17 +75,34 @@ struct Reached {
 };
 
 // Start of a BFS from src: levels and parents reset, the queue holding src,
-// the visited filter holding only src (threads cover max(n, words)).
+// the visited filter holding only src (threads cover max(n, words)); info
+// gets src's degree and the degree prefix at n / 2 and at n.
 __global__ void k_bfs_start(uint32_t n, uint32_t *level, int32_t *parent,
                             uint32_t src, uint32_t *queue, uint32_t *vis,
-                            uint64_t words) {
+                            uint64_t words, const uint32_t *deg,
+                            const uint64_t *prefix,
+                            unsigned long long *info) {
   const uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (v < n) {
     level[v] = (v == src) ? 0u : kInf;
     if (parent) parent[v] = (v == src) ? (int32_t)src : -1;
   }
   if (v < words) vis[v] = (v == (src >> 5)) ? 1u << (src & 31) : 0u;
-  if (v == 0) queue[0] = src;
+  if (v == 0) {
+    queue[0] = src;
+    info[0] = deg[src];
+    info[1] = prefix[n / 2];
+    info[2] = prefix[n];
+  }
+}
+
+// The tail kernel's starting state, written by a kernel so that a captured
+// launch sequence holds the values themselves.
+struct TailState {
+  uint64_t w[7];
+};
+__global__ void k_tail_state(uint64_t *st, TailState t) {
+  if (threadIdx.x < 7) st[threadIdx.x] = t.w[threadIdx.x];
 }
 
 __global__ void k_level_dist(const uint32_t *level, uint32_t n, int32_t *out) {
@@ -591,7 +608,7 @@ __global__ void __launch_bounds__(kTailThreads)
 // cleared by the kernels themselves.
 static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
                     int32_t *parent, uint32_t *order, bool top_down_only,
-                    BfsRun *run) {
+                    BfsRun *run, ReadLog *log = nullptr) {
   hipStream_t s = g->stream;
   Workspace &ws = g->ws;
   const uint32_t n = g->n_rows;
@@ -599,23 +616,20 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   SSTG_TRY(graph_degree_prefix(g, &prefix));
   EdgeMapCtx ctx;
   SSTG_TRY(edge_map_ctx(g, &ctx, kWsAlgo));
+  ctx.log = log;
   uint32_t *queue = order;
   if (!queue) SSTG_TRY(ws.get(kWsAlgo + 10, (uint64_t)n + 1, &queue));
+  unsigned long long *info = ctx.ctrs + 2 * em::kStepCtrs;
   SSTG_LAUNCH(k_bfs_start, blocks(std::max<uint64_t>(n, ctx.words)), kT, 0, s,
-              n, level, parent, src, queue, ctx.vis, ctx.words);
+              n, level, parent, src, queue, ctx.vis, ctx.words,
+              (const uint32_t *)g->deg, prefix, info);
   // the source's degree, and the degree mass of the upper half of the ids
   // against the whole (rows are sorted by id, so when the high ids carry
   // more edges -- as in the reference's rMat, whose hubs have mostly 1
   // bits -- a frontier vertex is met sooner from a row's end)
-  uint32_t src_deg = 0;
-  SSTG_HIP(hipMemcpyAsync(&src_deg, g->deg + src, sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-  SSTG_HIP(hipMemcpyAsync(g->host_rb + 8, prefix + n / 2, sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-  SSTG_HIP(hipMemcpyAsync(g->host_rb + 9, prefix + n, sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-  SSTG_HIP(hipStreamSynchronize(s));
-  ctx.rev = 2 * g->host_rb[8] < g->host_rb[9];
+  SSTG_TRY(read_back(g, log, g->host_rb + 8, info, 3 * sizeof(uint64_t)));
+  const uint32_t src_deg = (uint32_t)g->host_rb[8];
+  ctx.rev = 2 * g->host_rb[9] < g->host_rb[10];
   uint64_t qs = 0, nf = 1, mf = src_deg;
   uint64_t explored = src_deg;  // edges of visited vertices
   bool bottom_up = false;
@@ -656,25 +670,16 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
       SSTG_TRY(ws.get(kWsAlgo + 8, 8, &st));
       SSTG_TRY(ws.get(kWsAlgo + 9, kTailLevels, &ends));
       uint64_t *h = g->host_rb + 48;  // pinned staging
-      h[0] = qs;
-      h[1] = nf;
-      h[2] = L;
-      h[3] = 0;
-      h[4] = 0;
-      h[5] = prev_nf;
-      h[6] = mf;
-      SSTG_HIP(hipMemcpyAsync(st, h, 7 * sizeof(uint64_t), hipMemcpyHostToDevice,
-                              s));
+      const TailState t0{{qs, nf, L, 0, 0, prev_nf, mf}};
+      SSTG_LAUNCH(k_tail_state, 1, kWave, 0, s, st, t0);
       SSTG_LAUNCH(k_bfs_tail, 1, kTailThreads, 0, s, g->view(), level, parent,
                   queue, st, ends);
-      SSTG_HIP(hipMemcpyAsync(h, st, 7 * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                              s));
-      SSTG_HIP(hipStreamSynchronize(s));
+      SSTG_TRY(read_back(g, log, h, st, 7 * sizeof(uint64_t)));
       const uint64_t levels_run = h[3];
       if (run && levels_run) {
         std::vector<uint64_t> e(levels_run);
-        SSTG_HIP(hipMemcpy(e.data(), ends, levels_run * sizeof(uint64_t),
-                           hipMemcpyDeviceToHost));
+        SSTG_TRY(read_back(g, log, e.data(), ends,
+                           levels_run * sizeof(uint64_t)));
         for (uint64_t x : e) run->level_start.push_back(x);
       }
       if (verbose) {
@@ -736,9 +741,88 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   return SSTG_OK;
 }
 
+void bfs_release(sstg_graph *g) {
+  if (g->bfs_exec) hipGraphExecDestroy((hipGraphExec_t)g->bfs_exec);
+  if (g->bfs_graph) hipGraphDestroy((hipGraph_t)g->bfs_graph);
+  g->bfs_exec = g->bfs_graph = nullptr;
+  delete static_cast<BfsKey *>(g->bfs_key);
+  g->bfs_key = nullptr;
+}
+
+static BfsKey bfs_key_of(const sstg_graph *g, uint32_t src, const int32_t *parent,
+                         const uint32_t *lv) {
+  BfsKey k{};
+  k.version = g->version;
+  k.tune_epoch = g->tune_epoch;
+  k.ws_generation = g->ws.generation;
+  k.src = src;
+  k.n = g->n_rows;
+  const void *p[5] = {lv, parent, g->off, g->deg, g->pool};
+  for (int i = 0; i < 5; i++) k.ptr[i] = p[i];
+  k.alpha = g->bfs_alpha;
+  k.beta = g->bfs_beta;
+  return k;
+}
+
+// A BFS from a given source on a given graph version takes the same steps
+// every time: every level's frontier is the same set, so the counts the
+// host reads back, and with them every launch's shape, repeat (only the
+// order of the vertices inside a top-down level's queue differs).  The
+// first call runs plainly, logging each read-back; the same run is then
+// captured with the logged values standing in for the reads (no waits),
+// and later calls with the same key replay the recording in one launch.
+static int bfs_replayed(sstg_graph *g, uint32_t src, int32_t *parent,
+                        uint32_t *lv) {
+  hipStream_t s = g->stream;
+  BfsKey key = bfs_key_of(g, src, parent, lv);
+  const BfsKey *have = static_cast<const BfsKey *>(g->bfs_key);
+  if (g->bfs_exec && have && std::memcmp(have, &key, sizeof(key)) == 0) {
+    SSTG_HIP(hipGraphLaunch((hipGraphExec_t)g->bfs_exec, s));
+    SSTG_HIP(hipStreamSynchronize(s));
+    return SSTG_OK;
+  }
+  bfs_release(g);
+  ReadLog log;
+  log.mode = 1;
+  SSTG_TRY(bfs_core(g, src, lv, parent, nullptr, false, nullptr, &log));
+  SSTG_HIP(hipStreamSynchronize(s));
+  // after the run: it may have allocated workspace or built cached data
+  key = bfs_key_of(g, src, parent, lv);
+  log.mode = 2;
+  log.pos = 0;
+  Profiler &prof = profiler();
+  const bool prof_on = prof.on;
+  prof.on = false;  // no event records inside the capture
+  SSTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = bfs_core(g, src, lv, parent, nullptr, false, nullptr, &log);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  prof.on = prof_on;
+  // keep the recording only if it consumed exactly the logged reads and
+  // allocated nothing (the results are those of the plain run either way)
+  if (rc != SSTG_OK || e != hipSuccess || log.pos != log.words.size() ||
+      g->ws.generation != key.ws_generation) {
+    if (graph) hipGraphDestroy(graph);
+    if (rc != SSTG_OK) return rc;
+    SSTG_HIP(e);
+    return SSTG_OK;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    hipGraphDestroy(graph);
+    SSTG_HIP(ei);
+  }
+  g->bfs_graph = graph;
+  g->bfs_exec = exec;
+  g->bfs_key = new BfsKey(key);
+  return SSTG_OK;
+}
+
 int algo_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level) {
   uint32_t *lv = level;
   if (!lv) SSTG_TRY(g->ws.get(kWsAlgo + 13, g->n_rows, &lv));
+  if (iteration_capture_enabled(g)) return bfs_replayed(g, src, parent, lv);
   SSTG_TRY(bfs_core(g, src, lv, parent, nullptr, false, nullptr));
   SSTG_HIP(hipStreamSynchronize(g->stream));
   return SSTG_OK;

@@ -31,6 +31,17 @@ struct IterPairKey {
   double damping;
 };
 bool iteration_capture_enabled(const sstg_graph *g);
+// A recorded BFS (algo.hip): with the same switch, a BFS from the source
+// of the last recording on the same graph version, settings, outputs and
+// workspace replays that recording (hipGraph) instead of launching level by
+// level with a host read-back per level.
+struct BfsKey {
+  uint64_t version, tune_epoch, ws_generation;
+  uint32_t src, n;
+  const void *ptr[5];  // levels, parents, and the store's off, deg, pool
+  double alpha, beta;
+};
+void bfs_release(sstg_graph *g);
 int replay_iteration_pairs(sstg_graph *g, const IterPairKey &key,
                            uint32_t pairs, const std::function<int()> &record);
 int pagerank_segmented(sstg_graph *g, uint32_t iters, double damping,

@@ -109,6 +109,7 @@ int Workspace::get(int slot, size_t bytes, void **out) {
       a.p = arena + arena_used;
       a.bytes = want;
       arena_used += want;
+      generation++;
     }
     *out = a.p;
     return SSTG_OK;
@@ -139,6 +140,7 @@ int Workspace::get(int slot, size_t bytes, void **out) {
     s.p = p;
     s.bytes = want;
     total += want;
+    generation++;
   }
   *out = s.p;
   return SSTG_OK;
@@ -150,6 +152,7 @@ void Workspace::release_all() {
     if (s.p) hipFree(s.p);
   slots.clear();
   total = 0;
+  generation++;
 }
 
 static bool device_ok(int *dev) {

@@ -93,6 +93,9 @@ struct Workspace {
   // are lost, as when an allocated slot grows).
   uint8_t *arena = nullptr;
   size_t arena_bytes = 0, arena_used = 0;
+  // bumped whenever a slot gets new memory (or all are released): a
+  // recorded launch sequence that holds slot addresses is stale after it
+  uint64_t generation = 0;
   std::vector<Slot> arena_slots;
   void begin_arena(void *base, size_t bytes) {
     arena = static_cast<uint8_t *>(base);

@@ -24,6 +24,35 @@ struct EdgeMapResult {
   uint64_t degree_sum = 0;  // sum of their degrees
 };
 
+// Device read-backs of a traversal.  Recording: each read is made and its
+// words kept; replaying (while the stream is captured): the words come from
+// the log and nothing is read, so the same launches are enqueued without a
+// wait.
+struct ReadLog {
+  int mode = 0;  // 0 plain, 1 record, 2 replay
+  std::vector<uint64_t> words;
+  size_t pos = 0;
+};
+
+inline int read_back(sstg_graph *g, ReadLog *log, void *dst, const void *src,
+                     size_t bytes) {
+  const size_t nw = (bytes + 7) / 8;
+  if (log && log->mode == 2) {
+    if (log->pos + nw > log->words.size()) return SSTG_E_INVAL;
+    std::memcpy(dst, log->words.data() + log->pos, bytes);
+    log->pos += nw;
+    return SSTG_OK;
+  }
+  SSTG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream));
+  SSTG_HIP(hipStreamSynchronize(g->stream));
+  if (log && log->mode == 1) {
+    const size_t at = log->words.size();
+    log->words.resize(at + nw, 0);
+    std::memcpy(log->words.data() + at, dst, bytes);
+  }
+  return SSTG_OK;
+}
+
 struct EdgeMapCtx {
   uint32_t *bm = nullptr;   // frontier bitmap of a bottom-up step
   uint32_t *vis = nullptr;  // visited filter of the top-down steps
@@ -46,6 +75,7 @@ struct EdgeMapCtx {
   int act_src = -1;
   // bottom-up steps read rows from their end (set by the caller)
   int rev = 0;
+  ReadLog *log = nullptr;  // read-backs, recorded or replayed (may be null)
 };
 
 namespace em {
@@ -56,6 +86,11 @@ constexpr int kChunk = 8;
 constexpr int kBitmapRounds = 8;  // 512 vertices per wave in k_bits
 
 __global__ void k_clear_u32(uint32_t *p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_clear_u64(unsigned long long *p, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
   if (i < n) p[i] = 0;
 }
@@ -486,18 +521,20 @@ struct OnLevel {
 }  // namespace em
 
 // Buffers of a BFS: the frontier bitmap and the visited filter at slot and
-// slot + 1, the frontier degree prefix at slot + 2; both counter sets
-// cleared.
+// slot + 1, the frontier degree prefix at slot + 2; both counter sets (and
+// kCtxExtra words after them) cleared by a kernel, which a captured launch
+// sequence can hold.
+constexpr int kCtxExtra = 8;
 inline int edge_map_ctx(sstg_graph *g, EdgeMapCtx *ctx, int slot) {
   ctx->words = ((uint64_t)g->n_rows + 31) / 32 + 1;
   ctx->slot = slot;
   SSTG_TRY(g->ws.get(slot + 0, ctx->words, &ctx->bm));
   SSTG_TRY(g->ws.get(slot + 1, ctx->words, &ctx->vis));
-  SSTG_TRY(g->ws.get(kWsBottomUp + 2, 2 * em::kStepCtrs, &ctx->ctrs));
+  SSTG_TRY(g->ws.get(kWsBottomUp + 2, 2 * em::kStepCtrs + kCtxExtra,
+                     &ctx->ctrs));
   ctx->parity = 0;
-  SSTG_HIP(hipMemsetAsync(ctx->ctrs, 0,
-                          2 * em::kStepCtrs * sizeof(unsigned long long),
-                          g->stream));
+  SSTG_LAUNCH(em::k_clear_u64, 1, em::kT, 0, g->stream, ctx->ctrs,
+              (uint64_t)(2 * em::kStepCtrs + kCtxExtra));
   return SSTG_OK;
 }
 
@@ -525,10 +562,9 @@ int edge_map_bits(sstg_graph *g, EdgeMapCtx &ctx, P p, uint32_t *bm,
 // (first k of them) into host_rb, one copy and one wait; then the other set
 // becomes current.
 inline int step_counters(sstg_graph *g, EdgeMapCtx &ctx, int k) {
-  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctx.ctrs + em::kStepCtrs * ctx.parity,
-                          k * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                          g->stream));
-  SSTG_HIP(hipStreamSynchronize(g->stream));
+  SSTG_TRY(read_back(g, ctx.log, g->host_rb,
+                     ctx.ctrs + em::kStepCtrs * ctx.parity,
+                     k * sizeof(uint64_t)));
   ctx.parity ^= 1;
   return SSTG_OK;
 }
@@ -548,9 +584,8 @@ inline int frontier_prefix(sstg_graph *g, EdgeMapCtx &ctx,
   ctx.fpre_front = front;
   ctx.fpre_n = nf;
   if (total) {
-    SSTG_HIP(hipMemcpyAsync(g->host_rb + 8, ctx.fpre + nf, sizeof(uint64_t),
-                            hipMemcpyDeviceToHost, s));
-    SSTG_HIP(hipStreamSynchronize(s));
+    SSTG_TRY(read_back(g, ctx.log, g->host_rb + 8, ctx.fpre + nf,
+                       sizeof(uint64_t)));
     *total = g->host_rb[8];
   }
   return SSTG_OK;

@@ -358,12 +358,14 @@ int pr_iteration(sstg_graph *g, const SegLayout &L, const double *contrib,
 
 }  // namespace
 
+// Frees the graph's recorded launch sequences (PageRank pairs and BFS).
 void pagerank_release(sstg_graph *g) {
   if (g->pr_exec) hipGraphExecDestroy((hipGraphExec_t)g->pr_exec);
   if (g->pr_graph) hipGraphDestroy((hipGraph_t)g->pr_graph);
   g->pr_exec = g->pr_graph = nullptr;
   delete static_cast<IterPairKey *>(g->pr_key);
   g->pr_key = nullptr;
+  bfs_release(g);
 }
 
 bool iteration_capture_enabled(const sstg_graph *g) {

@@ -342,3 +342,42 @@ def test_pagerank_replayed_pairs_stay_exact(c1, graph_capture):
     ok, w = rel_close(EMV.PR_F32K(g, 10).astype(np.float64),
                       O.pagerank(o_off, o_cols, 10, 0.85), 1e-5)
     assert ok, w
+
+
+def test_bfs_replayed_runs_match_oracle(c1, graph_capture):
+    """With recording on, the first BFS from a source records its launch
+    sequence and later calls with the same source, outputs and graph
+    version replay it.  Every call, replayed or not, gives the oracle's
+    levels and a valid parent tree: repeated calls with other device work
+    between them, a change of source (records again), a threshold or
+    direction-switch change, and an update (a new graph version)."""
+    n, _, off, cols = c1
+    s = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
+    g = S.SparseMatrixV(n)
+    g.insert_batch(s, cols)
+    hub = int(np.argmax(np.diff(off)))
+    for src in (hub, hub, 0, 12345, 12345, hub):
+        say("replayed bfs", src)
+        _check_bfs(g, off, cols, src)
+    for k in range(3):
+        EMV.PR_S(g, 2)
+        EMV.CC(g)
+        S.synchronize()
+        _check_bfs(g, off, cols, hub)
+    g.set_bfs_switch(4.0, 40.0)
+    _check_bfs(g, off, cols, hub)
+    _check_bfs(g, off, cols, hub)
+    g.set_bfs_switch(15.0, 18.0)
+    g.set_degree_thresholds([8, 64, 512], bfs=True)
+    _check_bfs(g, off, cols, hub)
+    _check_bfs(g, off, cols, hub)
+    rng = np.random.default_rng(9)
+    bs = rng.integers(0, n, 4000).astype(np.uint32)
+    bd = rng.integers(0, n, 4000).astype(np.uint32)
+    g.insert_batch(np.concatenate([bs, bd]), np.concatenate([bd, bs]))
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+    og.insert_batch(np.concatenate([bs, bd]), np.concatenate([bd, bs]))
+    o_off, o_cols = og.csr()
+    for src in (hub, hub, 7, 7):
+        _check_bfs(g, o_off, o_cols, src)
