@@ -1,0 +1,144 @@
+// One-workgroup sort of up to kBlockSortMax 64-bit keys held in LDS.
+//
+// The keys carry no payload, so equal keys need no stable order and the
+// sort compares whole keys.  Two steps:
+//   1. runs of 64: each wave sorts its runs across its lanes with a bitonic
+//      network (21 compare-exchange steps over lane shuffles), all of its
+//      runs side by side;
+//   2. the runs are merged pairwise by rank, doubling the run length per
+//      round: a key's place in the merged pair is its index in its own run
+//      plus the number of keys of the other run before it (strictly smaller
+//      keys for a key of the left run, smaller or equal for the right run,
+//      so equal keys keep left before right and every place is taken once).
+//      A thread owns up to four consecutive keys of one run: the first and
+//      the last are ranked by a fixed-step search over the other run (a
+//      power of two long), the ones between by a search inside the range
+//      those two ranks bound (nearly always a few keys).
+// The length is padded to a power of two (at least 64) with all-ones keys.
+// They sort to the end, and since they equal any all-ones key of the input,
+// the first n keys afterwards are the input sorted whatever it holds.
+// Replaces an 8-bit LSD radix sort of six passes (four barriers and a
+// sixteen-step column scan each) for these sizes.
+#pragma once
+
+#include "common.hpp"
+
+namespace sstg {
+
+constexpr uint32_t kBlockSortMax = 4096;
+
+// number of a[0..len) before x: a[i] < x, or a[i] <= x when !strict;
+// a nondecreasing, len a power of two
+__device__ __forceinline__ uint32_t rank_in_run(const uint64_t *a, uint32_t len,
+                                                uint64_t x, bool strict) {
+  uint32_t b = 0;
+  for (uint32_t h = len >> 1; h > 0; h >>= 1) {
+    const uint64_t v = a[b + h - 1];
+    if (v < x || (!strict && v == x)) b += h;
+  }
+  const uint64_t v = a[b];
+  return b + (v < x || (!strict && v == x));
+}
+
+// the same count, known to lie in [lo, hi]
+__device__ __forceinline__ uint32_t rank_between(const uint64_t *a,
+                                                 uint32_t lo, uint32_t hi,
+                                                 uint64_t x, bool strict) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint64_t v = a[mid];
+    if (v < x || (!strict && v == x))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Sorts s[0, n) ascending, n <= kBlockSortMax, with all T threads of the
+// block (every thread must call it).  s must have room for the padded
+// length: the next power of two of max(n, 64).  Ends with a barrier.
+template <int T>
+__device__ void block_sort_u64(uint64_t *s, uint32_t n) {
+  static_assert(T % kWave == 0 && kBlockSortMax % T == 0, "block shape");
+  constexpr int kWaves = T / kWave;
+  constexpr int kItems = kBlockSortMax / T;          // keys per thread, at most
+  constexpr int kRuns = kBlockSortMax / kWave / kWaves;  // 64-key runs per wave
+  uint32_t np = kWave;
+  while (np < n) np <<= 1;
+  for (uint32_t i = n + threadIdx.x; i < np; i += T) s[i] = ~0ull;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  // 1. a wave's runs (w, w + kWaves, ...) go through the network together,
+  // so their shuffles overlap
+  {
+    const uint32_t runs = np / kWave;
+    uint64_t x[kRuns];
+#pragma unroll
+    for (int q = 0; q < kRuns; q++) {
+      const uint32_t r = w + (uint32_t)q * kWaves;
+      x[q] = r < runs ? s[r * kWave + lane] : ~0ull;
+    }
+#pragma unroll
+    for (uint32_t k = 2; k <= (uint32_t)kWave; k <<= 1) {
+#pragma unroll
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        const bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
+#pragma unroll
+        for (int q = 0; q < kRuns; q++) {
+          const uint64_t y = __shfl_xor(x[q], (int)j, kWave);
+          x[q] = take_min ? (y < x[q] ? y : x[q]) : (y > x[q] ? y : x[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kRuns; q++) {
+      const uint32_t r = w + (uint32_t)q * kWaves;
+      if (r < runs) s[r * kWave + lane] = x[q];
+    }
+  }
+  __syncthreads();
+  // 2. thread t owns the `per` consecutive keys [t * per, +per), all in one
+  // run: the first and last are ranked by a full search of the other run,
+  // the ones between inside the range those two bound
+  const uint32_t per = np > (uint32_t)T ? np / T : 1;
+  const uint32_t i0 = threadIdx.x * per;
+  for (uint32_t len = kWave; len < np; len <<= 1) {
+    uint64_t x[kItems];
+    uint32_t at[kItems];
+    if (i0 < np) {
+      const uint32_t pair = i0 & ~(2 * len - 1);
+      const bool left = (i0 & len) == 0;
+      const uint64_t *other = s + (left ? pair + len : pair);
+      // a key's place: its index less len for the right run, plus its rank
+      const uint32_t own = i0 - (left ? 0 : len);
+#pragma unroll
+      for (int j = 0; j < kItems; j++)
+        if ((uint32_t)j < per) x[j] = s[i0 + j];
+      uint64_t xl = x[0];  // the last key (no dynamic register index)
+#pragma unroll
+      for (int j = 1; j < kItems; j++)
+        if ((uint32_t)j == per - 1) xl = x[j];
+      const uint32_t c0 = rank_in_run(other, len, x[0], left);
+      const uint32_t c1 = per > 1 ? rank_in_run(other, len, xl, left) : c0;
+#pragma unroll
+      for (int j = 0; j < kItems; j++) {
+        if ((uint32_t)j >= per) break;
+        const uint32_t c = j == 0 ? c0
+                           : (uint32_t)j == per - 1
+                               ? c1
+                               : rank_between(other, c0, c1, x[j], left);
+        at[j] = own + (uint32_t)j + c;
+      }
+    }
+    __syncthreads();
+    if (i0 < np) {
+#pragma unroll
+      for (int j = 0; j < kItems; j++)
+        if ((uint32_t)j < per) s[at[j]] = x[j];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace sstg

@@ -27,6 +27,12 @@ struct sstg_graph {
   bool noalloc = false;
   // small pinned host block for read-backs
   uint64_t *host_rb = nullptr;
+  // small pinned host block mapped into the device's address space
+  // (fine-grained: kernels write it directly, and the host reads it after
+  // its wait with no copy launched) and its device address; null where a
+  // handle has none (the copy through host_rb is used then)
+  uint64_t *host_ctr = nullptr;
+  uint64_t *host_ctr_dev = nullptr;
   // device-side counters block (see update.hip)
   uint64_t *dev_ctr = nullptr;
   // cached derived data: exclusive prefix of deg (n_rows + 1 entries)

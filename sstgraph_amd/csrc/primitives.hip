@@ -1,5 +1,6 @@
 // Scans, stream compaction and LSD radix sort for gfx950 (wave64).
 #include "primitives.hpp"
+#include "block_sort.hpp"
 #include "graph.hpp"
 
 namespace sstg {
@@ -423,17 +424,30 @@ __global__ void __launch_bounds__(kSsThreads)
 }
 
 // ---------------------------------------------------------- merge sort
-// Keys of 4K < n <= kMergeSortMax: k_sort_small sorts runs of kMsChunk
-// keys (one block each, all in parallel), then each k_merge_pass doubles
-// the sorted runs in one launch.  A 10^5-key batch takes 6 launches where
-// the radix sort takes 30 (6 digit passes of histogram, three scan launches
-// and scatter).  One block sorts 16K keys in 90 us, 4K keys in a fraction
-// of that, against 8 us per merge pass at 10^5 keys.
+constexpr uint32_t kMsChunk = kBlockSortMax;
+
+// Block b sorts keys[b * kMsChunk, +kMsChunk) (the last chunk may be short)
+// by whole-key comparison (block_sort.hpp).
+__global__ void __launch_bounds__(kSsThreads)
+    k_sort_chunks(uint64_t *keys, uint64_t n) {
+  __shared__ uint64_t s[kMsChunk];
+  keys += (uint64_t)blockIdx.x * kMsChunk;
+  const uint32_t m = (uint32_t)min(n - (uint64_t)blockIdx.x * kMsChunk,
+                                   (uint64_t)kMsChunk);
+  for (uint32_t i = threadIdx.x; i < m; i += kSsThreads) s[i] = keys[i];
+  block_sort_u64<kSsThreads>(s, m);  // begins and ends with a barrier
+  for (uint32_t i = threadIdx.x; i < m; i += kSsThreads) keys[i] = s[i];
+}
+
+// Keys of n <= kMergeSortMax: k_sort_chunks sorts runs of kMsChunk keys
+// (one block each, all in parallel), then each k_merge_pass doubles the
+// sorted runs in one launch.  A 10^5-key batch takes 6 launches where the
+// radix sort takes 30 (6 digit passes of histogram, three scan launches and
+// scatter).
 constexpr int kMpThreads = 256;
 constexpr int kMpItems = 8;
 constexpr uint32_t kMpOut = kMpThreads * kMpItems;  // outputs per block
 constexpr uint64_t kMergeSortMax = 1ull << 21;
-constexpr uint32_t kMsChunk = 4096;
 
 // Number of a[0..la) among the first d outputs of the merge of a and b
 // (ties from a first): the smallest i with a[i] > b[d - 1 - i], or the
@@ -618,14 +632,12 @@ int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
 int sort_keys_u64(uint64_t *keys, uint64_t *tmp, uint64_t n, const int *shifts,
                   int npass, Workspace &ws, int ws_base, hipStream_t s,
                   uint64_t **result) {
-  if (n <= kMsChunk || n > kMergeSortMax || npass == 0)
+  *result = keys;
+  if (n <= 1 || npass == 0) return SSTG_OK;
+  if (n > kMergeSortMax)
     return radix_sort_u64(keys, tmp, n, shifts, npass, ws, ws_base, s, result);
-  if (npass > 16) return SSTG_E_INVAL;
-  SortShifts sh{};
-  for (int p = 0; p < npass; p++) sh.shift[p] = shifts[p];
-  sh.passes = npass;
-  SSTG_LAUNCH(k_sort_small, (uint32_t)ceil_div(n, (uint64_t)kMsChunk),
-              kSsThreads, 0, s, keys, (uint32_t)n, sh, kMsChunk);
+  SSTG_LAUNCH(k_sort_chunks, (uint32_t)ceil_div(n, (uint64_t)kMsChunk),
+              kSsThreads, 0, s, keys, n);
   uint64_t *src = keys, *dst = tmp;
   for (uint64_t run = kMsChunk; run < n; run *= 2) {
     const uint64_t pairs = ceil_div(n, 2 * run);

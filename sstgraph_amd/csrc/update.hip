@@ -212,6 +212,7 @@ struct SegScan {
   uint32_t blocks;
   uint64_t *big_idx, *big_stage, *big_work, *tile_idx, *tile_len;
   uint64_t *ctr;
+  uint64_t *hctr;  // mapped host copy of the counter block, or null
 };
 
 __device__ __forceinline__ void seg_sums(const SegScan &a, uint64_t s,
@@ -362,6 +363,9 @@ __global__ void __launch_bounds__(kT) k_seg_scan(SegScan a) {
     a.big_stage[carry[1]] = carry[3];
     a.big_work[carry[1]] = carry[4];
     a.tile_len[carry[2]] = carry[5];
+    // the whole block, earlier launches' counters included, for the host
+    if (a.hctr)
+      for (int k = 0; k < kCtrCount; k++) a.hctr[k] = a.ctr[k];
   }
 }
 
@@ -922,6 +926,9 @@ int graph_alloc(sstg_graph *g) {
   SSTG_HIP(hipMalloc(&g->dev_ctr, kCtrCount * sizeof(uint64_t)));
   SSTG_HIP(hipHostMalloc(&g->host_rb, 64 * sizeof(uint64_t),
                          hipHostMallocDefault));
+  SSTG_HIP(hipHostMalloc(&g->host_ctr, 16 * sizeof(uint64_t),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+  SSTG_HIP(hipHostGetDevicePointer((void **)&g->host_ctr_dev, g->host_ctr, 0));
   SSTG_HIP(hipStreamSynchronize(g->stream));
   return SSTG_OK;
 }
@@ -937,10 +944,12 @@ void graph_free(sstg_graph *g) {
   hipFree(g->dev_ctr);
   hipFree(g->deg_prefix);
   if (g->host_rb) hipHostFree(g->host_rb);
+  if (g->host_ctr) hipHostFree(g->host_ctr);
   g->ws.release_all();
   g->off = nullptr;
   g->deg = g->cap = g->pool = nullptr;
   g->dev_ctr = g->deg_prefix = g->host_rb = nullptr;
+  g->host_ctr = g->host_ctr_dev = nullptr;
 }
 
 // Moves every row into a fresh pool laid out in row order (the root-level
@@ -1094,6 +1103,7 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
     sc.tile_idx = tile_idx;
     sc.tile_len = tile_len;
     sc.ctr = ctr;
+    sc.hctr = g->host_ctr_dev;
     const uint64_t tiles = ceil_div(sc.n, kSegTile);
     if (tiles <= kSegOneBlock) {
       sc.per = sc.n;
@@ -1109,10 +1119,13 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
       SSTG_LAUNCH(k_seg_scan, sc.blocks, kT, 0, s, sc);
     }
   }
-  SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, kCtrCount * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
+  const uint64_t *h = g->host_ctr;
+  if (!g->host_ctr_dev) {
+    SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, kCtrCount * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    h = g->host_rb;
+  }
   SSTG_HIP(hipStreamSynchronize(s));
-  const uint64_t *h = g->host_rb;
   if (h[kCtrErr]) return SSTG_E_RANGE;
   const uint64_t b = h[kCtrB], t = h[kCtrT];
   const uint64_t stage_tot = h[kCtrStage], alloc_tot = h[kCtrAlloc],

@@ -10,7 +10,7 @@
 // touched rows (step 7), one row per workgroup at a time:
 //
 //   1. load and range-check the pairs, pack them as row << 32 | col keys;
-//   2. LSD radix sort in LDS (8-bit digits over the column then row bits);
+//   2. sort in LDS (64-key bitonic runs merged by rank, block_sort.hpp);
 //   3. keep the first copy of each key, drop self-loops on insert, and keep
 //      keys absent (insert) or present (remove) in their row, found by a
 //      binary search of the row in HBM, which also gives each key's place;
@@ -31,6 +31,7 @@
 // The layout rules are the general pipeline's (slot capacities from
 // slot_capacity, relocated slots left as garbage), so the two paths can
 // follow each other on one graph.
+#include "block_sort.hpp"
 #include "graph.hpp"
 
 namespace sstg {
@@ -40,7 +41,6 @@ constexpr int kSmT = 1024;
 constexpr int kSmWaves = kSmT / kWave;
 constexpr uint32_t kSmMax = kSmallBatch;
 constexpr int kSmItems = kSmMax / kSmT;  // keys per thread
-constexpr int kSmBins = 256;
 constexpr uint64_t kSmMaxMove = 1u << 15;  // longer moves: general pipeline
 constexpr int kSmMergeBlocks = 1024;      // workgroups of the merge launch
 
@@ -54,11 +54,6 @@ enum {
   kSmGarbage = 5,   // elements of slots left behind
   kSmSym = 6,       // symmetric after the update (if it was before)
   kSmCount = 7,
-};
-
-struct SmallShifts {
-  int shift[8];
-  int passes;
 };
 
 // exclusive block-wide prefix of v; *total gets the block's sum
@@ -121,42 +116,83 @@ __device__ __forceinline__ bool lds_has(const uint64_t *a, uint32_t n,
   return lo < n && a[lo] == x;
 }
 
-// lower bound of x in the sorted row a[0..len) with four probes per round,
-// issued together: a round trip to HBM per round, about log5(len) rounds
-// instead of log2(len) for a lone key whose search is all latency
-__device__ __forceinline__ uint32_t lower_bound_4(const uint32_t *a,
-                                                  uint32_t len, uint32_t x) {
-  uint32_t lo = 0, hi = len;  // answer in [lo, hi]
-  while (hi - lo > 4) {
-    const uint32_t step = (hi - lo) / 5;
-    const uint32_t p1 = lo + step, p2 = p1 + step, p3 = p2 + step,
-                   p4 = p3 + step;
-    const uint32_t v1 = a[p1], v2 = a[p2], v3 = a[p3], v4 = a[p4];
-    if (v4 < x)
-      lo = p4 + 1;
-    else if (v3 < x) {
-      lo = p3 + 1;
-      hi = p4;
-    } else if (v2 < x) {
-      lo = p2 + 1;
-      hi = p3;
-    } else if (v1 < x) {
-      lo = p1 + 1;
-      hi = p2;
-    } else {
-      hi = p1;
-    }
+// The row searches of the keep step.  Each round issues kFan probes per key
+// and cuts the interval kFan + 1 ways, and a thread's keys are searched
+// side by side: every round issues the probes of all of them before it
+// waits.  The value at the interval's upper end is kept, so the membership
+// test needs no load after the last round.  The whole step runs on the one
+// CU of the plan's workgroup, whose memory pipeline takes about one cache
+// line per clock: more probes per round (fewer rounds, more lines) measured
+// slower from 4 up, one probe per round slower at 1000 keys (keep step at
+// scale 24, 1000 / 4000 keys: 1 probe 14.0 / 42.5 us, 2 probes 12.7 / 45.2,
+// 4 probes 13.4 / 51.2, 8 probes 16.4 / 60.6; keys one after another with
+// 4 probes and a membership load 14.7 / 45.7).
+constexpr int kFan = 2;
+
+template <int K>
+__device__ __forceinline__ void search_rows(const uint32_t *const row[K],
+                                            const uint32_t len[K],
+                                            const uint32_t x[K],
+                                            const bool want[K], uint32_t p[K],
+                                            bool found[K]) {
+  // answer (first index with row[i] >= x) in [lo, hi]; row[hi] == vhi
+  // when hi < len
+  uint32_t lo[K], hi[K], vhi[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    lo[j] = 0;
+    hi[j] = want[j] ? len[j] : 0;
+    vhi[j] = 0;
   }
-  // at most four candidates left: one more round of loads together
-  const uint32_t w0 = lo < hi ? a[lo] : 0u, w1 = lo + 1 < hi ? a[lo + 1] : 0u,
-                 w2 = lo + 2 < hi ? a[lo + 2] : 0u,
-                 w3 = lo + 3 < hi ? a[lo + 3] : 0u;
-  uint32_t below = 0;
-  below += lo < hi && w0 < x;
-  below += lo + 1 < hi && w1 < x;
-  below += lo + 2 < hi && w2 < x;
-  below += lo + 3 < hi && w3 < x;
-  return lo + below;
+  for (;;) {
+    uint32_t v[K][kFan];
+    bool more = false;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      if (hi[j] - lo[j] > (uint32_t)kFan) {
+        const uint32_t step = (hi[j] - lo[j]) / (kFan + 1);
+#pragma unroll
+        for (int m = 0; m < kFan; m++) v[j][m] = row[j][lo[j] + step * (m + 1)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      if (hi[j] - lo[j] > (uint32_t)kFan) {
+        more = true;
+        const uint32_t step = (hi[j] - lo[j]) / (kFan + 1);
+        uint32_t below = 0, vnext = 0;
+#pragma unroll
+        for (int m = 0; m < kFan; m++) below += v[j][m] < x[j];
+#pragma unroll
+        for (int m = 0; m < kFan; m++) vnext = below == (uint32_t)m ? v[j][m] : vnext;
+        const uint32_t base = lo[j];
+        if (below > 0) lo[j] = base + step * below + 1;
+        if (below < (uint32_t)kFan) {
+          hi[j] = base + step * (below + 1);
+          vhi[j] = vnext;
+        }
+      }
+    }
+    if (!more) break;
+  }
+  // at most kFan candidates left: one more round of loads together
+  uint32_t w[K][kFan];
+#pragma unroll
+  for (int j = 0; j < K; j++)
+#pragma unroll
+    for (int m = 0; m < kFan; m++)
+      w[j][m] = lo[j] + m < hi[j] ? row[j][lo[j] + m] : 0u;
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    uint32_t below = 0, at = 0;
+#pragma unroll
+    for (int m = 0; m < kFan; m++) below += lo[j] + m < hi[j] && w[j][m] < x[j];
+#pragma unroll
+    for (int m = 0; m < kFan; m++) at = below == (uint32_t)m ? w[j][m] : at;
+    p[j] = lo[j] + below;
+    found[j] = want[j] && (p[j] < hi[j] ? at == x[j]
+                                        : (hi[j] < len[j] && vhi[j] == x[j]));
+  }
 }
 
 // One row's merge by the whole workgroup (thread me of width): the batch
@@ -210,6 +246,21 @@ __device__ __forceinline__ void merge_row(uint32_t *row, uint32_t *dst,
   }
 }
 
+// The counter block, to device memory (read by the merge launch) and, when
+// hctr is not null, to mapped host memory, which the host reads after its
+// wait with no copy launched.
+__device__ __forceinline__ void put_counts(uint64_t *ctr, uint64_t *hctr,
+                                           uint64_t err, uint64_t fallback,
+                                           uint64_t b, uint64_t t, uint64_t a,
+                                           uint64_t garbage, uint64_t sym) {
+  const uint64_t v[kSmCount] = {err, fallback, b, t, a, garbage, sym};
+#pragma unroll
+  for (int k = 0; k < kSmCount; k++) {
+    ctr[k] = v[k];
+    if (hctr) hctr[k] = v[k];
+  }
+}
+
 // The plan of a small update; each array holds up to kSmMax (+1) entries.
 struct SmallPlan {
   uint64_t *key;     // kept keys, sorted
@@ -220,18 +271,18 @@ struct SmallPlan {
 
 __global__ void __launch_bounds__(kSmT)
     k_update_small_plan(const uint32_t *src, const uint32_t *dst, uint32_t n,
-                        DevGraph g, uint32_t n_cols, SmallShifts sh,
+                        DevGraph g, uint32_t n_cols,
                         int insert, int skip_self, int track_sym,
                         uint64_t pool_tail, uint64_t pool_cap, SmallPlan plan,
-                        uint64_t *ctr) {
+                        uint64_t *ctr, uint64_t *hctr) {
   __shared__ uint64_t key[kSmMax];
   __shared__ uint32_t pos[kSmMax];
   __shared__ uint32_t rstart[kSmMax + 1];
   __shared__ uint32_t rdeg[kSmMax];  // touched rows' degree and capacity
   __shared__ uint32_t rcap[kSmMax];
+  __shared__ uint32_t kdeg[kSmMax];  // each key's row degree and capacity
+  __shared__ uint32_t kcap[kSmMax];
   __shared__ uint8_t kflag[kSmMax];  // the keep decision of each sorted key
-  __shared__ uint32_t wcnt[kSmWaves][kSmBins];
-  __shared__ uint32_t tstart[kSmBins];
   __shared__ uint32_t wsum32[kSmWaves];
   __shared__ uint64_t wsum64[kSmWaves];
   __shared__ uint32_t s_bad, s_sym;
@@ -251,99 +302,56 @@ __global__ void __launch_bounds__(kSmT)
   __syncthreads();
   if (s_bad) {
     if (tid == 0) {
-      ctr[kSmErr] = 1;
-      ctr[kSmFallback] = 0;
+      put_counts(ctr, hctr, 1, 0, 0, 0, 0, 0, 0);
     }
     return;
   }
 
-  // 2. LSD radix sort: each wave owns `per` consecutive 64-key items
-  {
-    const uint32_t per = (((n + kWave - 1) / kWave) + kSmWaves - 1) / kSmWaves;
-    const uint32_t base = w * per * kWave;
-    for (int p = 0; p < sh.passes; p++) {
-      const int shift = sh.shift[p];
-      for (int i = tid; i < kSmWaves * kSmBins; i += kSmT) (&wcnt[0][0])[i] = 0;
-      __syncthreads();
-      uint64_t kv[kSmItems];
-      uint32_t rank[kSmItems];
-#pragma unroll
-      for (int j = 0; j < kSmItems; j++) {
-        if ((uint32_t)j >= per) break;  // uniform across the block
-        const uint32_t i = base + (uint32_t)j * kWave + lane;
-        const bool valid = i < n;
-        kv[j] = valid ? key[i] : 0;
-        const uint32_t d = (uint32_t)(kv[j] >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-          const uint64_t m = __ballot((d >> b) & 1u);
-          peers &= ((d >> b) & 1u) ? m : ~m;
-        }
-        const uint32_t before = wcnt[w][d];
-        const uint32_t r = mask_rank(peers);
-        rank[j] = before + r;
-        if (valid && r == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
-      }
-      __syncthreads();
-      uint32_t dtot = 0;
-      if (tid < kSmBins) {
-        uint32_t run = 0;
-#pragma unroll
-        for (int ww = 0; ww < kSmWaves; ww++) {
-          const uint32_t c = wcnt[ww][tid];
-          wcnt[ww][tid] = run;
-          run += c;
-        }
-        dtot = run;
-      }
-      uint32_t tot = 0;
-      const uint32_t ex = block_scan(dtot, wsum32, &tot);
-      if (tid < kSmBins) tstart[tid] = ex;
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < kSmItems; j++) {
-        if ((uint32_t)j >= per) break;
-        const uint32_t i = base + (uint32_t)j * kWave + lane;
-        if (i < n) {
-          const uint32_t d = (uint32_t)(kv[j] >> shift) & 255u;
-          key[tstart[d] + wcnt[w][d] + rank[j]] = kv[j];
-        }
-      }
-      // every wave has placed its keys before wcnt and tstart are reset
-      __syncthreads();
-    }
-  }
+  // 2. sort in LDS (block_sort.hpp; the keys carry nothing else)
+  block_sort_u64<kSmT>(key, n);
 
   // 3. keep: first copy, no self-loop on insert, absent (insert) or
-  // present (remove).  The row searches are latency chains, so key i is
-  // searched by thread i mod kSmT (one chain per thread for up to kSmT
-  // keys); the decisions go through LDS to the compaction, whose threads
-  // own contiguous keys [kSmItems * tid, +kSmItems)
-  for (uint32_t i = tid; i < n; i += kSmT) {
-    const uint64_t kk = key[i];
-    bool k = i == 0 || key[i - 1] != kk;
-    const uint32_t r = (uint32_t)(kk >> 32), c = (uint32_t)kk;
-    if (insert && skip_self && r == c) k = false;
-    uint32_t p = 0;
-    if (k) {
-      const uint32_t d = g.deg[r];
-      const uint64_t o = g.off[r];
-      bool exists = false;
-      if (d) {
-        const uint32_t *row = g.pool + o;
-        p = lower_bound_4(row, d, c);
-        exists = p < d && row[p] == c;
+  // present (remove), found by searching the row in HBM.  Thread tid
+  // searches keys tid, tid + kSmT, ... side by side (search_rows); the
+  // decisions, places and the rows' degree and capacity go through LDS to
+  // the compaction, whose threads own contiguous keys [kSmItems * tid, +kSmItems)
+  {
+    const uint32_t *row[kSmItems];
+    uint32_t len[kSmItems], x[kSmItems], p[kSmItems];
+    bool want[kSmItems], found[kSmItems];
+#pragma unroll
+    for (int j = 0; j < kSmItems; j++) {
+      const uint32_t i = tid + (uint32_t)j * kSmT;
+      row[j] = g.pool;
+      len[j] = 0;
+      x[j] = 0;
+      want[j] = false;
+      if (i < n) {
+        const uint64_t kk = key[i];
+        const uint32_t r = (uint32_t)(kk >> 32);
+        x[j] = (uint32_t)kk;
+        want[j] = (i == 0 || key[i - 1] != kk) && !(insert && skip_self && r == x[j]);
+        if (want[j]) {
+          len[j] = g.deg[r];
+          kcap[i] = g.cap[r];
+          row[j] = g.pool + g.off[r];
+        }
       }
-      k = insert ? !exists : exists;
     }
-    kflag[i] = k;
-    pos[i] = p;
+    search_rows<kSmItems>(row, len, x, want, p, found);
+#pragma unroll
+    for (int j = 0; j < kSmItems; j++) {
+      const uint32_t i = tid + (uint32_t)j * kSmT;
+      if (i < n) {
+        kflag[i] = want[j] && (insert ? !found[j] : found[j]);
+        pos[i] = p[j];
+        kdeg[i] = len[j];
+      }
+    }
   }
   __syncthreads();
   uint64_t kv[kSmItems];
-  uint32_t kp[kSmItems];
+  uint32_t kp[kSmItems], kd[kSmItems], kc[kSmItems];
   bool keep[kSmItems];
   uint32_t cnt = 0;
 #pragma unroll
@@ -351,7 +359,9 @@ __global__ void __launch_bounds__(kSmT)
     const uint32_t i = tid * kSmItems + j;
     keep[j] = i < n && kflag[i];
     kv[j] = i < n ? key[i] : 0;
-    kp[j] = i < n ? pos[i] : 0;
+    kp[j] = keep[j] ? pos[i] : 0;
+    kd[j] = keep[j] ? kdeg[i] : 0;
+    kc[j] = keep[j] ? kcap[i] : 0;
     cnt += keep[j];
   }
   // 4. compact the kept keys in place (every thread holds its keys)
@@ -362,18 +372,14 @@ __global__ void __launch_bounds__(kSmT)
     if (keep[j]) {
       key[at] = kv[j];
       pos[at] = kp[j];
+      kdeg[at] = kd[j];
+      kcap[at] = kc[j];
       at++;
     }
   __syncthreads();
   if (b == 0) {
     if (tid == 0) {
-      ctr[kSmErr] = 0;
-      ctr[kSmFallback] = 0;
-      ctr[kSmB] = 0;
-      ctr[kSmT_] = 0;
-      ctr[kSmAlloc] = 0;
-      ctr[kSmGarbage] = 0;
-      ctr[kSmSym] = 1;
+      put_counts(ctr, hctr, 0, 0, 0, 0, 0, 0, 1);
     }
     return;
   }
@@ -389,17 +395,13 @@ __global__ void __launch_bounds__(kSmT)
 #pragma unroll
   for (int j = 0; j < kSmItems; j++) {
     const uint32_t i = tid * kSmItems + j;
-    if (i < b && (i == 0 || (key[i] >> 32) != (key[i - 1] >> 32)))
+    if (i < b && (i == 0 || (key[i] >> 32) != (key[i - 1] >> 32))) {
+      rdeg[r_at] = kdeg[i];  // the row's degree and capacity, from its search
+      rcap[r_at] = kcap[i];
       rstart[r_at++] = i;
+    }
   }
   if (tid == 0) rstart[t] = b;
-  __syncthreads();
-  // each touched row's degree and capacity, loaded row r by thread r mod kSmT
-  for (uint32_t r = tid; r < t; r += kSmT) {
-    const uint32_t v = (uint32_t)(key[rstart[r]] >> 32);
-    rdeg[r] = g.deg[v];
-    rcap[r] = g.cap[v];
-  }
   __syncthreads();
 
   // 5. per row: slot, pool offset, garbage, length of its move
@@ -449,8 +451,7 @@ __global__ void __launch_bounds__(kSmT)
   for (int i = 0; i < kSmWaves; i++) M = wsum64[i] > M ? wsum64[i] : M;
   if ((insert && pool_tail + A > pool_cap) || M > kSmMaxMove) {
     if (tid == 0) {
-      ctr[kSmErr] = 0;
-      ctr[kSmFallback] = 1;
+      put_counts(ctr, hctr, 0, 1, 0, 0, 0, 0, 0);
     }
     return;  // nothing written
   }
@@ -478,13 +479,7 @@ __global__ void __launch_bounds__(kSmT)
   for (uint32_t r = tid; r <= t; r += kSmT) plan.rstart[r] = rstart[r];
   __syncthreads();  // s_sym is final
   if (tid == 0) {
-    ctr[kSmErr] = 0;
-    ctr[kSmFallback] = 0;
-    ctr[kSmB] = b;
-    ctr[kSmT_] = t;
-    ctr[kSmAlloc] = A;
-    ctr[kSmGarbage] = G;
-    ctr[kSmSym] = s_sym;
+    put_counts(ctr, hctr, 0, 0, b, t, A, G, s_sym);
   }
 }
 
@@ -522,11 +517,6 @@ int update_small(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                  uint32_t n, bool insert, bool skip_self, bool *fallback) {
   *fallback = false;
   hipStream_t s = g->stream;
-  SmallShifts sh{};
-  const uint32_t cb = bit_width_u32(g->n_cols ? g->n_cols - 1 : 0);
-  const uint32_t rb = bit_width_u32(g->n_rows ? g->n_rows - 1 : 0);
-  for (uint32_t b = 0; b < cb; b += 8) sh.shift[sh.passes++] = (int)b;
-  for (uint32_t b = 0; b < rb; b += 8) sh.shift[sh.passes++] = (int)(32 + b);
   const bool track = g->sym_version == g->version && g->symmetric;
   SmallPlan plan;
   SSTG_TRY(g->ws.get(kWsSmallKeys, kSmMax, &plan.key));
@@ -534,15 +524,18 @@ int update_small(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   SSTG_TRY(g->ws.get(kWsSmallRows, kSmMax + 1, &plan.rstart));
   SSTG_TRY(g->ws.get(kWsSmallAlloc, kSmMax + 1, &plan.aoff));
   SSTG_LAUNCH(k_update_small_plan, 1, kSmT, 0, s, src, dst, n, g->view(),
-              g->n_cols, sh, insert ? 1 : 0, skip_self ? 1 : 0, track ? 1 : 0,
-              g->pool_tail, g->pool_cap, plan, g->dev_ctr);
+              g->n_cols, insert ? 1 : 0, skip_self ? 1 : 0, track ? 1 : 0,
+              g->pool_tail, g->pool_cap, plan, g->dev_ctr, g->host_ctr_dev);
   SSTG_LAUNCH(k_update_small_merge, kSmMergeBlocks, kSmT, 0, s, g->view(),
               insert ? 1 : 0, g->pool_tail, plan,
               (const uint64_t *)g->dev_ctr);
-  SSTG_HIP(hipMemcpyAsync(g->host_rb, g->dev_ctr, kSmCount * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
+  const uint64_t *h = g->host_ctr;
+  if (!g->host_ctr_dev) {
+    SSTG_HIP(hipMemcpyAsync(g->host_rb, g->dev_ctr, kSmCount * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, s));
+    h = g->host_rb;
+  }
   SSTG_HIP(hipStreamSynchronize(s));
-  const uint64_t *h = g->host_rb;
   if (h[kSmErr]) return SSTG_E_RANGE;
   if (h[kSmFallback]) {
     *fallback = true;
