@@ -55,46 +55,79 @@ __device__ __forceinline__ uint32_t rank_between(const uint64_t *a,
   return lo;
 }
 
-// Sorts s[0, n) ascending, n <= kBlockSortMax, with all T threads of the
-// block (every thread must call it).  s must have room for the padded
-// length: the next power of two of max(n, 64).  Ends with a barrier.
-template <int T>
-__device__ void block_sort_u64(uint64_t *s, uint32_t n) {
+// The bitonic network over the 64 lanes of a wave for the first R of a
+// wave's runs (one key per lane each), all R side by side.  With kPay the
+// payload px follows its key: a lane takes its partner's key and payload
+// only when the partner's key is strictly smaller (larger) than its own,
+// so equal keys stay put and the pairs remain a permutation.
+template <int R, bool kPay, int N>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&x)[N], uint32_t (&px)[N],
+                                             uint32_t lane) {
+#pragma unroll
+  for (uint32_t k = 2; k <= (uint32_t)kWave; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        const uint64_t y = __shfl_xor(x[q], (int)j, kWave);
+        const bool take = take_min ? y < x[q] : y > x[q];
+        if (kPay) {
+          const uint32_t py = __shfl_xor(px[q], (int)j, kWave);
+          px[q] = take ? py : px[q];
+        }
+        x[q] = take ? y : x[q];
+      }
+    }
+  }
+}
+
+template <int T, bool kPay>
+__device__ void block_sort_impl(uint64_t *s, uint16_t *v, uint32_t n) {
   static_assert(T % kWave == 0 && kBlockSortMax % T == 0, "block shape");
   constexpr int kWaves = T / kWave;
   constexpr int kItems = kBlockSortMax / T;          // keys per thread, at most
   constexpr int kRuns = kBlockSortMax / kWave / kWaves;  // 64-key runs per wave
   uint32_t np = kWave;
   while (np < n) np <<= 1;
-  for (uint32_t i = n + threadIdx.x; i < np; i += T) s[i] = ~0ull;
+  for (uint32_t i = n + threadIdx.x; i < np; i += T) {
+    s[i] = ~0ull;
+    if (kPay) v[i] = 0;
+  }
   __syncthreads();
   const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   // 1. a wave's runs (w, w + kWaves, ...) go through the network together,
-  // so their shuffles overlap
+  // so their shuffles overlap.  The shuffles (two lane permutes per step
+  // for a 64-bit key) bound this step, so runs a wave does not own are
+  // skipped (a branch uniform across the wave)
   {
     const uint32_t runs = np / kWave;
+    const uint32_t mine = __builtin_amdgcn_readfirstlane(
+        runs > w ? (runs - w + kWaves - 1) / kWaves : 0);
     uint64_t x[kRuns];
+    uint32_t px[kRuns];
 #pragma unroll
     for (int q = 0; q < kRuns; q++) {
       const uint32_t r = w + (uint32_t)q * kWaves;
       x[q] = r < runs ? s[r * kWave + lane] : ~0ull;
+      px[q] = kPay && r < runs ? v[r * kWave + lane] : 0;
     }
-#pragma unroll
-    for (uint32_t k = 2; k <= (uint32_t)kWave; k <<= 1) {
-#pragma unroll
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        const bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
-#pragma unroll
-        for (int q = 0; q < kRuns; q++) {
-          const uint64_t y = __shfl_xor(x[q], (int)j, kWave);
-          x[q] = take_min ? (y < x[q] ? y : x[q]) : (y > x[q] ? y : x[q]);
-        }
-      }
-    }
+    static_assert(kRuns >= 1 && kRuns <= 4, "runs per wave");
+    if (mine >= 4)
+      wave_bitonic<kRuns, kPay>(x, px, lane);
+    else if (mine == 3)
+      wave_bitonic<kRuns < 3 ? kRuns : 3, kPay>(x, px, lane);
+    else if (mine == 2)
+      wave_bitonic<kRuns < 2 ? kRuns : 2, kPay>(x, px, lane);
+    else if (mine == 1)
+      wave_bitonic<1, kPay>(x, px, lane);
 #pragma unroll
     for (int q = 0; q < kRuns; q++) {
       const uint32_t r = w + (uint32_t)q * kWaves;
-      if (r < runs) s[r * kWave + lane] = x[q];
+      if (r < runs) {
+        s[r * kWave + lane] = x[q];
+        if (kPay) v[r * kWave + lane] = (uint16_t)px[q];
+      }
     }
   }
   __syncthreads();
@@ -105,6 +138,7 @@ __device__ void block_sort_u64(uint64_t *s, uint32_t n) {
   const uint32_t i0 = threadIdx.x * per;
   for (uint32_t len = kWave; len < np; len <<= 1) {
     uint64_t x[kItems];
+    uint16_t px[kItems];
     uint32_t at[kItems];
     if (i0 < np) {
       const uint32_t pair = i0 & ~(2 * len - 1);
@@ -114,7 +148,10 @@ __device__ void block_sort_u64(uint64_t *s, uint32_t n) {
       const uint32_t own = i0 - (left ? 0 : len);
 #pragma unroll
       for (int j = 0; j < kItems; j++)
-        if ((uint32_t)j < per) x[j] = s[i0 + j];
+        if ((uint32_t)j < per) {
+          x[j] = s[i0 + j];
+          if (kPay) px[j] = v[i0 + j];
+        }
       uint64_t xl = x[0];  // the last key (no dynamic register index)
 #pragma unroll
       for (int j = 1; j < kItems; j++)
@@ -135,10 +172,28 @@ __device__ void block_sort_u64(uint64_t *s, uint32_t n) {
     if (i0 < np) {
 #pragma unroll
       for (int j = 0; j < kItems; j++)
-        if ((uint32_t)j < per) s[at[j]] = x[j];
+        if ((uint32_t)j < per) {
+          s[at[j]] = x[j];
+          if (kPay) v[at[j]] = px[j];
+        }
     }
     __syncthreads();
   }
+}
+
+// Sorts s[0, n) ascending, n <= kBlockSortMax, with all T threads of the
+// block (every thread must call it).  s must have room for the padded
+// length: the next power of two of max(n, 64).  Ends with a barrier.
+template <int T>
+__device__ void block_sort_u64(uint64_t *s, uint32_t n) {
+  block_sort_impl<T, false>(s, nullptr, n);
+}
+
+// The same, moving the 16-bit payload v[i] with key s[i] (v has the same
+// room; padding entries get payload 0).
+template <int T>
+__device__ void block_sort_u64_u16(uint64_t *s, uint16_t *v, uint32_t n) {
+  block_sort_impl<T, true>(s, v, n);
 }
 
 }  // namespace sstg

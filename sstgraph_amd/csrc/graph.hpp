@@ -215,7 +215,7 @@ enum : int {
   kWsSmallKeys = 145, // small updates: the plan (update_small.hip)
   kWsSmallPos = 146,
   kWsSmallRows = 147,
-  kWsSmallAlloc = 148,
+  kWsSmallFound = 149,  // small updates: each pair's row search
 };
 
 // degree threshold sets of a graph (sstg_graph::bin_hi): the reductions

@@ -5,15 +5,20 @@
 // The general pipeline (update.hip) sizes its grids from a read-back in the
 // middle of the update and runs about fifteen launches; for a batch of a few
 // thousand pairs those launches and the two host waits are the whole cost.
-// Here two launches do every step: one workgroup plans the update with the
-// batch held in LDS (steps 1-6), then a fixed grid of workgroups merges the
-// touched rows (step 7), one row per workgroup at a time:
+// Here three launches do every step: the row searches, spread over many
+// CUs (step 0); one workgroup that plans the update with the batch held in
+// LDS (steps 1-6); then a fixed grid of workgroups that merges the touched
+// rows (step 7), one row per workgroup at a time:
 //
+//   0. (a launch of its own, one thread per pair over many CUs) search
+//      each pair's row in HBM: present or not, its place, the row's degree,
+//      capacity and pool offset, a 32-byte record per pair;
 //   1. load and range-check the pairs, pack them as row << 32 | col keys;
-//   2. sort in LDS (64-key bitonic runs merged by rank, block_sort.hpp);
+//   2. sort in LDS (64-key bitonic runs merged by rank, block_sort.hpp),
+//      each key carrying its pair's index;
 //   3. keep the first copy of each key, drop self-loops on insert, and keep
-//      keys absent (insert) or present (remove) in their row, found by a
-//      binary search of the row in HBM, which also gives each key's place;
+//      keys absent (insert) or present (remove) in their row, from the
+//      key's search record;
 //   4. compact the kept keys; the rows they touch are the runs of equal row;
 //   5. per row: stays in its slot, or (insert past the slot's capacity)
 //      moves to a fresh slot at the pool tail -- offsets by a block scan;
@@ -103,31 +108,16 @@ __device__ __forceinline__ uint32_t count_lt(const uint32_t *a, uint32_t k,
   return lo;
 }
 
-__device__ __forceinline__ bool lds_has(const uint64_t *a, uint32_t n,
-                                        uint64_t x) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] < x)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo < n && a[lo] == x;
-}
-
-// The row searches of the keep step.  Each round issues kFan probes per key
-// and cuts the interval kFan + 1 ways, and a thread's keys are searched
-// side by side: every round issues the probes of all of them before it
-// waits.  The value at the interval's upper end is kept, so the membership
-// test needs no load after the last round.  The whole step runs on the one
-// CU of the plan's workgroup, whose memory pipeline takes about one cache
-// line per clock: more probes per round (fewer rounds, more lines) measured
-// slower from 4 up, one probe per round slower at 1000 keys (keep step at
-// scale 24, 1000 / 4000 keys: 1 probe 14.0 / 42.5 us, 2 probes 12.7 / 45.2,
-// 4 probes 13.4 / 51.2, 8 probes 16.4 / 60.6; keys one after another with
-// 4 probes and a membership load 14.7 / 45.7).
-constexpr int kFan = 2;
+// Row searches: each round issues kFan probes per key and cuts the
+// interval kFan + 1 ways, and a thread's K keys are searched side by side:
+// every round issues the probes of all of them before it waits.  The value
+// at the interval's upper end is kept, so the membership test needs no load
+// after the last round.  A lone key's search is a chain of round trips, so
+// wide rounds pay where the searches are spread over many CUs; run inside
+// the plan's one workgroup, the searches were bound by that CU's memory
+// pipeline (about one cache line per clock) and took 12.7-16.4 us for 1000
+// keys and 42-61 us for 4000 at scale 24, whatever the fan-out.
+constexpr int kFan = 8;
 
 template <int K>
 __device__ __forceinline__ void search_rows(const uint32_t *const row[K],
@@ -261,28 +251,65 @@ __device__ __forceinline__ void put_counts(uint64_t *ctr, uint64_t *hctr,
   }
 }
 
-// The plan of a small update; each array holds up to kSmMax (+1) entries.
+// Step 0: pair i's row searched for its column (one thread per pair, any
+// order, duplicates and self-loops included).  found[2i] = (place, degree,
+// capacity, present), found[2i + 1] = the row's pool offset (low, high
+// word); pairs out of range get zeros (the plan rejects the batch).
+constexpr int kSmSearchT = 64;  // small workgroups: the searches spread over CUs
+
+__global__ void __launch_bounds__(kSmSearchT)
+    k_update_small_search(const uint32_t *src, const uint32_t *dst, uint32_t n,
+                          DevGraph g, uint32_t n_cols, uint4 *found) {
+  const uint32_t i = blockIdx.x * kSmSearchT + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = src[i], c = dst[i];
+  const uint32_t *row[1] = {g.pool};
+  uint32_t len[1] = {0}, x[1] = {c}, p[1];
+  bool want[1] = {r < g.n && c < n_cols}, hit[1];
+  uint32_t cp = 0;
+  uint64_t o = 0;
+  if (want[0]) {
+    len[0] = g.deg[r];
+    cp = g.cap[r];
+    o = g.off[r];
+    row[0] = g.pool + o;
+  }
+  search_rows<1>(row, len, x, want, p, hit);
+  found[2 * i] = make_uint4(p[0], len[0], cp, hit[0] ? 1u : 0u);
+  found[2 * i + 1] = make_uint4((uint32_t)o, (uint32_t)(o >> 32), 0, 0);
+}
+
+// One touched row of a small update, as the merge launch reads it (one
+// record, so a merge workgroup's first loads do not wait on one another).
+struct SmallRow {
+  uint64_t from, to;  // pool offset of the old slot and of the new one
+  uint32_t v, od;     // row, its degree before the update
+  uint32_t bs, k;     // its kept keys: plan key/pos [bs, bs + k)
+  uint32_t cap, pad;  // capacity of a new slot
+};
+
+// The plan of a small update; each array holds up to kSmMax entries.
 struct SmallPlan {
   uint64_t *key;     // kept keys, sorted
   uint32_t *pos;     // their places in the old rows
-  uint32_t *rstart;  // first kept key of each touched row, then b
-  uint64_t *aoff;    // pool offset of each relocated row's slot, then total
+  SmallRow *rows;    // the touched rows
 };
 
 __global__ void __launch_bounds__(kSmT)
     k_update_small_plan(const uint32_t *src, const uint32_t *dst, uint32_t n,
                         DevGraph g, uint32_t n_cols,
                         int insert, int skip_self, int track_sym,
-                        uint64_t pool_tail, uint64_t pool_cap, SmallPlan plan,
-                        uint64_t *ctr, uint64_t *hctr) {
+                        uint64_t pool_tail, uint64_t pool_cap,
+                        const uint4 *found, SmallPlan plan, uint64_t *ctr,
+                        uint64_t *hctr) {
   __shared__ uint64_t key[kSmMax];
   __shared__ uint32_t pos[kSmMax];
   __shared__ uint32_t rstart[kSmMax + 1];
-  __shared__ uint32_t rdeg[kSmMax];  // touched rows' degree and capacity
-  __shared__ uint32_t rcap[kSmMax];
-  __shared__ uint32_t kdeg[kSmMax];  // each key's row degree and capacity
-  __shared__ uint32_t kcap[kSmMax];
   __shared__ uint8_t kflag[kSmMax];  // the keep decision of each sorted key
+  __shared__ uint16_t kidx[kSmMax];  // each sorted key's pair index
+  __shared__ uint32_t kdeg[kSmMax];  // each kept key's row: degree,
+  __shared__ uint32_t kcap[kSmMax];  // capacity and pool offset
+  __shared__ uint64_t koff[kSmMax];
   __shared__ uint32_t wsum32[kSmWaves];
   __shared__ uint64_t wsum64[kSmWaves];
   __shared__ uint32_t s_bad, s_sym;
@@ -298,6 +325,7 @@ __global__ void __launch_bounds__(kSmT)
     const uint32_t a = src[i], b = dst[i];
     if (a >= g.n || b >= n_cols) s_bad = 1;
     key[i] = ((uint64_t)a << 32) | b;
+    kidx[i] = (uint16_t)i;
   }
   __syncthreads();
   if (s_bad) {
@@ -307,50 +335,32 @@ __global__ void __launch_bounds__(kSmT)
     return;
   }
 
-  // 2. sort in LDS (block_sort.hpp; the keys carry nothing else)
-  block_sort_u64<kSmT>(key, n);
+  // 2. sort in LDS (block_sort.hpp), each key with its pair index
+  block_sort_u64_u16<kSmT>(key, kidx, n);
 
   // 3. keep: first copy, no self-loop on insert, absent (insert) or
-  // present (remove), found by searching the row in HBM.  Thread tid
-  // searches keys tid, tid + kSmT, ... side by side (search_rows); the
-  // decisions, places and the rows' degree and capacity go through LDS to
-  // the compaction, whose threads own contiguous keys [kSmItems * tid, +kSmItems)
-  {
-    const uint32_t *row[kSmItems];
-    uint32_t len[kSmItems], x[kSmItems], p[kSmItems];
-    bool want[kSmItems], found[kSmItems];
-#pragma unroll
-    for (int j = 0; j < kSmItems; j++) {
-      const uint32_t i = tid + (uint32_t)j * kSmT;
-      row[j] = g.pool;
-      len[j] = 0;
-      x[j] = 0;
-      want[j] = false;
-      if (i < n) {
-        const uint64_t kk = key[i];
-        const uint32_t r = (uint32_t)(kk >> 32);
-        x[j] = (uint32_t)kk;
-        want[j] = (i == 0 || key[i - 1] != kk) && !(insert && skip_self && r == x[j]);
-        if (want[j]) {
-          len[j] = g.deg[r];
-          kcap[i] = g.cap[r];
-          row[j] = g.pool + g.off[r];
-        }
-      }
+  // present (remove), from the key's search record (written by other CUs,
+  // so read once, here); the decisions, places and the rows' degree,
+  // capacity and offset go through LDS to the compaction, whose threads own
+  // contiguous keys [kSmItems * tid, +kSmItems)
+  for (uint32_t i = tid; i < n; i += kSmT) {
+    const uint64_t kk = key[i];
+    const uint32_t r = (uint32_t)(kk >> 32), c = (uint32_t)kk;
+    const bool want =
+        (i == 0 || key[i - 1] != kk) && !(insert && skip_self && r == c);
+    uint4 f = make_uint4(0, 0, 0, 0), fo = f;
+    if (want) {
+      f = found[2 * kidx[i]];
+      fo = found[2 * kidx[i] + 1];
     }
-    search_rows<kSmItems>(row, len, x, want, p, found);
-#pragma unroll
-    for (int j = 0; j < kSmItems; j++) {
-      const uint32_t i = tid + (uint32_t)j * kSmT;
-      if (i < n) {
-        kflag[i] = want[j] && (insert ? !found[j] : found[j]);
-        pos[i] = p[j];
-        kdeg[i] = len[j];
-      }
-    }
+    kflag[i] = want && (insert ? f.w == 0 : f.w != 0);
+    pos[i] = f.x;
+    kdeg[i] = f.y;
+    kcap[i] = f.z;
+    koff[i] = ((uint64_t)fo.y << 32) | fo.x;
   }
   __syncthreads();
-  uint64_t kv[kSmItems];
+  uint64_t kv[kSmItems], ko[kSmItems];
   uint32_t kp[kSmItems], kd[kSmItems], kc[kSmItems];
   bool keep[kSmItems];
   uint32_t cnt = 0;
@@ -362,6 +372,7 @@ __global__ void __launch_bounds__(kSmT)
     kp[j] = keep[j] ? pos[i] : 0;
     kd[j] = keep[j] ? kdeg[i] : 0;
     kc[j] = keep[j] ? kcap[i] : 0;
+    ko[j] = keep[j] ? koff[i] : 0;
     cnt += keep[j];
   }
   // 4. compact the kept keys in place (every thread holds its keys)
@@ -374,6 +385,7 @@ __global__ void __launch_bounds__(kSmT)
       pos[at] = kp[j];
       kdeg[at] = kd[j];
       kcap[at] = kc[j];
+      koff[at] = ko[j];
       at++;
     }
   __syncthreads();
@@ -395,17 +407,24 @@ __global__ void __launch_bounds__(kSmT)
 #pragma unroll
   for (int j = 0; j < kSmItems; j++) {
     const uint32_t i = tid * kSmItems + j;
-    if (i < b && (i == 0 || (key[i] >> 32) != (key[i - 1] >> 32))) {
-      rdeg[r_at] = kdeg[i];  // the row's degree and capacity, from its search
-      rcap[r_at] = kcap[i];
+    if (i < b && (i == 0 || (key[i] >> 32) != (key[i - 1] >> 32)))
       rstart[r_at++] = i;
-    }
   }
   if (tid == 0) rstart[t] = b;
   __syncthreads();
 
   // 5. per row: slot, pool offset, garbage, length of its move
-  uint64_t alloc[kSmItems];
+  // (the row's degree, capacity and pool offset: those of its first key)
+  uint64_t alloc[kSmItems], from[kSmItems];
+  uint32_t rdeg[kSmItems], rcap[kSmItems];
+#pragma unroll
+  for (int j = 0; j < kSmItems; j++) {
+    const uint32_t r = tid * kSmItems + j;
+    const uint32_t i = r < t ? rstart[r] : 0;
+    rdeg[j] = r < t ? kdeg[i] : 0;
+    rcap[j] = r < t ? kcap[i] : 0;
+    from[j] = r < t ? koff[i] : 0;
+  }
   uint64_t asum = 0, garb = 0, worst = 0;
 #pragma unroll
   for (int j = 0; j < kSmItems; j++) {
@@ -413,7 +432,7 @@ __global__ void __launch_bounds__(kSmT)
     alloc[j] = 0;
     if (r >= t) continue;
     const uint32_t bs = rstart[r], blen = rstart[r + 1] - bs;
-    const uint32_t od = rdeg[r], cp = rcap[r];
+    const uint32_t od = rdeg[j], cp = rcap[j];
     uint64_t move;
     if (insert && (uint64_t)od + blen > cp) {
       alloc[j] = slot_capacity((uint64_t)od + blen, n_cols);
@@ -431,11 +450,19 @@ __global__ void __launch_bounds__(kSmT)
   for (int j = 0; j < kSmItems; j++) {
     const uint32_t r = tid * kSmItems + j;
     if (r < t) {
-      plan.aoff[r] = aex;
+      SmallRow d;
+      d.from = from[j];
+      d.to = alloc[j] ? pool_tail + aex : from[j];
+      d.v = (uint32_t)(key[rstart[r]] >> 32);
+      d.od = rdeg[j];
+      d.bs = rstart[r];
+      d.k = rstart[r + 1] - rstart[r];
+      d.cap = (uint32_t)alloc[j];
+      d.pad = 0;
+      plan.rows[r] = d;
       aex += alloc[j];
     }
   }
-  if (tid == 0) plan.aoff[t] = A;
   (void)block_scan(garb, wsum64, &G);
   // the longest move, by a max over the waves (wsum64 is free again)
   uint64_t wm = worst;
@@ -460,23 +487,38 @@ __global__ void __launch_bounds__(kSmT)
   // graph, where (v, u) was present exactly when (u, v) was: absent for a
   // kept insert key, present for a kept remove key.  So the graph stays
   // symmetric iff every kept key's reverse is kept too.
+  // Keys tid, tid + kSmT, ... per thread (every thread busy for small b),
+  // searched side by side with fixed power-of-two steps.
   if (track_sym) {
+    uint64_t rev[kSmItems];
+    uint32_t at[kSmItems];
+#pragma unroll
+    for (int j = 0; j < kSmItems; j++) {
+      const uint32_t i = tid + (uint32_t)j * kSmT;
+      const uint64_t kk = i < b ? key[i] : 0;
+      rev[j] = (kk << 32) | (kk >> 32);
+      at[j] = 0;
+    }
+    uint32_t top = 1;
+    while (top < b) top <<= 1;
+    for (uint32_t h = top >> 1; h > 0; h >>= 1) {
+#pragma unroll
+      for (int j = 0; j < kSmItems; j++)
+        if (at[j] + h <= b && key[at[j] + h - 1] < rev[j]) at[j] += h;
+    }
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < kSmItems; j++) {
-      const uint32_t i = tid * kSmItems + j;
-      if (i >= b) continue;
-      const uint32_t u = (uint32_t)(key[i] >> 32), v = (uint32_t)key[i];
-      ok &= lds_has(key, b, ((uint64_t)v << 32) | u);
+      const uint32_t i = tid + (uint32_t)j * kSmT;
+      if (i < b) ok &= at[j] < b && key[at[j]] == rev[j];
     }
     if (!ok) s_sym = 0;
   }
-  // the plan for the merge launch
+  // the plan for the merge launch (the rows are written above)
   for (uint32_t i = tid; i < b; i += kSmT) {
     plan.key[i] = key[i];
     plan.pos[i] = pos[i];
   }
-  for (uint32_t r = tid; r <= t; r += kSmT) plan.rstart[r] = rstart[r];
   __syncthreads();  // s_sym is final
   if (tid == 0) {
     put_counts(ctr, hctr, 0, 0, b, t, A, G, s_sym);
@@ -484,28 +526,29 @@ __global__ void __launch_bounds__(kSmT)
 }
 
 // 7. one touched row at a time per workgroup; nothing to do when the plan
-// stopped (error, fallback) or changed nothing
+// stopped (error, fallback) or changed nothing.  A workgroup's first row
+// record is loaded beside the counters (the grid never exceeds kSmMax rows,
+// so the load is in bounds; the record is used only if the row exists), and
+// the row's key places are staged in LDS for the shift searches.
 __global__ void __launch_bounds__(kSmT)
-    k_update_small_merge(DevGraph g, int insert, uint64_t pool_tail,
-                         SmallPlan plan, const uint64_t *ctr) {
+    k_update_small_merge(DevGraph g, int insert, SmallPlan plan,
+                         const uint64_t *ctr) {
+  __shared__ uint32_t spos[kSmMax];
+  SmallRow d = plan.rows[blockIdx.x];
   if (ctr[kSmErr] || ctr[kSmFallback]) return;
   const uint32_t t = (uint32_t)ctr[kSmT_];
   for (uint32_t r = blockIdx.x; r < t; r += gridDim.x) {
-    const uint32_t bs = plan.rstart[r], k = plan.rstart[r + 1] - bs;
-    const uint32_t v = (uint32_t)(plan.key[bs] >> 32);
-    const uint32_t od = g.deg[v];
-    const uint64_t a0 = plan.aoff[r], a1 = plan.aoff[r + 1];
-    const bool reloc = insert && a1 > a0;
-    uint32_t *row = g.pool + g.off[v];
-    uint32_t *to = reloc ? g.pool + pool_tail + a0 : row;
-    merge_row(row, to, od, plan.key + bs, plan.pos + bs, k, insert != 0,
-              threadIdx.x, kSmT);
-    __syncthreads();  // every thread has read deg[v] and off[v]
+    if (r != blockIdx.x) d = plan.rows[r];
+    for (uint32_t i = threadIdx.x; i < d.k; i += kSmT) spos[i] = plan.pos[d.bs + i];
+    __syncthreads();
+    merge_row(g.pool + d.from, g.pool + d.to, d.od, plan.key + d.bs, spos, d.k,
+              insert != 0, threadIdx.x, kSmT);
+    __syncthreads();  // spos is read by every thread before the next row
     if (threadIdx.x == 0) {
-      g.deg[v] = insert ? od + k : od - k;
-      if (reloc) {
-        g.off[v] = pool_tail + a0;
-        g.cap[v] = (uint32_t)(a1 - a0);
+      g.deg[d.v] = insert ? d.od + d.k : d.od - d.k;
+      if (d.to != d.from) {
+        g.off[d.v] = d.to;
+        g.cap[d.v] = d.cap;
       }
     }
   }
@@ -519,16 +562,21 @@ int update_small(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   hipStream_t s = g->stream;
   const bool track = g->sym_version == g->version && g->symmetric;
   SmallPlan plan;
+  uint4 *found = nullptr;
+  SSTG_TRY(g->ws.get(kWsSmallFound, 2 * kSmMax, &found));
   SSTG_TRY(g->ws.get(kWsSmallKeys, kSmMax, &plan.key));
   SSTG_TRY(g->ws.get(kWsSmallPos, kSmMax, &plan.pos));
-  SSTG_TRY(g->ws.get(kWsSmallRows, kSmMax + 1, &plan.rstart));
-  SSTG_TRY(g->ws.get(kWsSmallAlloc, kSmMax + 1, &plan.aoff));
+  SSTG_TRY(g->ws.get(kWsSmallRows, kSmMax, &plan.rows));
+  if (n)
+    SSTG_LAUNCH(k_update_small_search, ceil_div(n, (uint32_t)kSmSearchT),
+                kSmSearchT, 0, s, src, dst, n, g->view(), g->n_cols, found);
   SSTG_LAUNCH(k_update_small_plan, 1, kSmT, 0, s, src, dst, n, g->view(),
               g->n_cols, insert ? 1 : 0, skip_self ? 1 : 0, track ? 1 : 0,
-              g->pool_tail, g->pool_cap, plan, g->dev_ctr, g->host_ctr_dev);
+              g->pool_tail, g->pool_cap, (const uint4 *)found, plan,
+              g->dev_ctr, g->host_ctr_dev);
+  static_assert(kSmMergeBlocks <= kSmMax, "first row records in bounds");
   SSTG_LAUNCH(k_update_small_merge, kSmMergeBlocks, kSmT, 0, s, g->view(),
-              insert ? 1 : 0, g->pool_tail, plan,
-              (const uint64_t *)g->dev_ctr);
+              insert ? 1 : 0, plan, (const uint64_t *)g->dev_ctr);
   const uint64_t *h = g->host_ctr;
   if (!g->host_ctr_dev) {
     SSTG_HIP(hipMemcpyAsync(g->host_rb, g->dev_ctr, kSmCount * sizeof(uint64_t),
