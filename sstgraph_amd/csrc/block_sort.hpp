@@ -2,9 +2,9 @@
 //
 // The keys carry no payload, so equal keys need no stable order and the
 // sort compares whole keys.  Two steps:
-//   1. runs of 64: each wave sorts its runs across its lanes with a bitonic
-//      network (21 compare-exchange steps over lane shuffles), all of its
-//      runs side by side;
+//   1. runs: each wave sorts one run of 64 * R keys with a bitonic network,
+//      R = 1, 2 or 4 consecutive keys per lane (R = n / 1024 rounded up to
+//      a power of two; below 1024 keys, runs of 64 on the first waves);
 //   2. the runs are merged pairwise by rank, doubling the run length per
 //      round: a key's place in the merged pair is its index in its own run
 //      plus the number of keys of the other run before it (strictly smaller
@@ -55,30 +55,119 @@ __device__ __forceinline__ uint32_t rank_between(const uint64_t *a,
   return lo;
 }
 
-// The bitonic network over the 64 lanes of a wave for the first R of a
-// wave's runs (one key per lane each), all R side by side.  With kPay the
-// payload px follows its key: a lane takes its partner's key and payload
-// only when the partner's key is strictly smaller (larger) than its own,
-// so equal keys stay put and the pairs remain a permutation.
-template <int R, bool kPay, int N>
-__device__ __forceinline__ void wave_bitonic(uint64_t (&x)[N], uint32_t (&px)[N],
+// The value of v held by lane (this lane) ^ J, J a power of two below 64,
+// exchanged without LDS: DPP within quads (J = 1, 2) and rows (J = 4, 8: a
+// shift up or down by J, chosen by the lane's bit J), and the gfx950 row
+// and half-wave swaps for J = 16 and 32 (a swap of v with itself leaves
+// each lane's partner value in one of the two results).
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+  static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32,
+                "lane distance");
+  const uint32_t lane = __lane_id();
+  if constexpr (J == 1) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);  // [1,0,3,2]
+  } else if constexpr (J == 2) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]
+  } else if constexpr (J == 4 || J == 8) {
+    const uint32_t up = __builtin_amdgcn_update_dpp(0u, v, 0x100 + J, 0xF, 0xF, false);
+    const uint32_t dn = __builtin_amdgcn_update_dpp(0u, v, 0x110 + J, 0xF, 0xF, false);
+    return (lane & J) ? dn : up;  // row_shl:J reads lane + J, row_shr:J lane - J
+  } else if constexpr (J == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? r[0] : r[1];
+  }
+}
+
+// lane_xor for a distance known once the caller's loops are unrolled
+__device__ __forceinline__ uint32_t lane_xor_at(uint32_t v, uint32_t d) {
+  switch (d) {
+    case 1: return lane_xor<1>(v);
+    case 2: return lane_xor<2>(v);
+    case 4: return lane_xor<4>(v);
+    case 8: return lane_xor<8>(v);
+    case 16: return lane_xor<16>(v);
+    default: return lane_xor<32>(v);
+  }
+}
+
+__device__ __forceinline__ uint64_t lane_xor_at(uint64_t v, uint32_t d) {
+  const uint32_t lo = lane_xor_at((uint32_t)v, d);
+  const uint32_t hi = lane_xor_at((uint32_t)(v >> 32), d);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// A wave's run of 64 * R keys, R consecutive keys per lane (lane l holds
+// elements R*l .. R*l + R - 1), through a bitonic network.  Steps whose
+// partners are less than R apart stay within a lane's registers; the rest
+// pair lane l with lane l ^ (j / R), one exchange per register (lane_xor,
+// no LDS).  So a run of 256 keys costs the exchanges of four runs of 64 and
+// saves two merge rounds.
+// With kPay the payload px follows its key: a pair is exchanged only when
+// its keys are out of order, so equal keys stay put and the pairs remain a
+// permutation.
+template <int R, bool kPay>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&x)[R], uint32_t (&px)[R],
                                              uint32_t lane) {
 #pragma unroll
-  for (uint32_t k = 2; k <= (uint32_t)kWave; k <<= 1) {
+  for (uint32_t k = 2; k <= (uint32_t)(kWave * R); k <<= 1) {
 #pragma unroll
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      const bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
+      if (j < (uint32_t)R) {
 #pragma unroll
-      for (int q = 0; q < R; q++) {
-        const uint64_t y = __shfl_xor(x[q], (int)j, kWave);
-        const bool take = take_min ? y < x[q] : y > x[q];
-        if (kPay) {
-          const uint32_t py = __shfl_xor(px[q], (int)j, kWave);
-          px[q] = take ? py : px[q];
+        for (int q = 0; q < R; q++) {
+          if ((q & j) != 0) continue;
+          const int q2 = q | (int)j;
+          const uint32_t e = lane * R + (uint32_t)q;
+          const bool up = (e & k) == 0;
+          const bool swap = up ? x[q2] < x[q] : x[q] < x[q2];
+          const uint64_t a = x[q], b = x[q2];
+          x[q] = swap ? b : a;
+          x[q2] = swap ? a : b;
+          if (kPay) {
+            const uint32_t pa = px[q], pb = px[q2];
+            px[q] = swap ? pb : pa;
+            px[q2] = swap ? pa : pb;
+          }
         }
-        x[q] = take ? y : x[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+          const uint32_t e = lane * R + (uint32_t)q;
+          const bool take_min = ((e & j) == 0) == ((e & k) == 0);
+          const uint64_t y = lane_xor_at(x[q], j / R);
+          const bool take = take_min ? y < x[q] : y > x[q];
+          if (kPay) {
+            const uint32_t py = lane_xor_at(px[q], j / R);
+            px[q] = take ? py : px[q];
+          }
+          x[q] = take ? y : x[q];
+        }
       }
     }
+  }
+}
+
+// Wave w sorts the run s[w * 64R, +64R) (and its payload) in place.
+template <int R, bool kPay>
+__device__ __forceinline__ void wave_sort_run(uint64_t *s, uint16_t *v,
+                                              uint32_t w, uint32_t lane) {
+  uint64_t x[R];
+  uint32_t px[R];
+  const uint32_t base = w * (uint32_t)(kWave * R) + lane * R;
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    x[q] = s[base + q];
+    px[q] = kPay ? v[base + q] : 0;
+  }
+  wave_bitonic<R, kPay>(x, px, lane);
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    s[base + q] = x[q];
+    if (kPay) v[base + q] = (uint16_t)px[q];
   }
 }
 
@@ -86,8 +175,8 @@ template <int T, bool kPay>
 __device__ void block_sort_impl(uint64_t *s, uint16_t *v, uint32_t n) {
   static_assert(T % kWave == 0 && kBlockSortMax % T == 0, "block shape");
   constexpr int kWaves = T / kWave;
-  constexpr int kItems = kBlockSortMax / T;          // keys per thread, at most
-  constexpr int kRuns = kBlockSortMax / kWave / kWaves;  // 64-key runs per wave
+  constexpr int kItems = kBlockSortMax / T;  // keys per thread, at most
+  static_assert(kItems == 1 || kItems == 2 || kItems == 4, "keys per lane");
   uint32_t np = kWave;
   while (np < n) np <<= 1;
   for (uint32_t i = n + threadIdx.x; i < np; i += T) {
@@ -96,47 +185,22 @@ __device__ void block_sort_impl(uint64_t *s, uint16_t *v, uint32_t n) {
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  // 1. a wave's runs (w, w + kWaves, ...) go through the network together,
-  // so their shuffles overlap.  The shuffles (two lane permutes per step
-  // for a 64-bit key) bound this step, so runs a wave does not own are
-  // skipped (a branch uniform across the wave)
-  {
-    const uint32_t runs = np / kWave;
-    const uint32_t mine = __builtin_amdgcn_readfirstlane(
-        runs > w ? (runs - w + kWaves - 1) / kWaves : 0);
-    uint64_t x[kRuns];
-    uint32_t px[kRuns];
-#pragma unroll
-    for (int q = 0; q < kRuns; q++) {
-      const uint32_t r = w + (uint32_t)q * kWaves;
-      x[q] = r < runs ? s[r * kWave + lane] : ~0ull;
-      px[q] = kPay && r < runs ? v[r * kWave + lane] : 0;
-    }
-    static_assert(kRuns >= 1 && kRuns <= 4, "runs per wave");
-    if (mine >= 4)
-      wave_bitonic<kRuns, kPay>(x, px, lane);
-    else if (mine == 3)
-      wave_bitonic<kRuns < 3 ? kRuns : 3, kPay>(x, px, lane);
-    else if (mine == 2)
-      wave_bitonic<kRuns < 2 ? kRuns : 2, kPay>(x, px, lane);
-    else if (mine == 1)
-      wave_bitonic<1, kPay>(x, px, lane);
-#pragma unroll
-    for (int q = 0; q < kRuns; q++) {
-      const uint32_t r = w + (uint32_t)q * kWaves;
-      if (r < runs) {
-        s[r * kWave + lane] = x[q];
-        if (kPay) v[r * kWave + lane] = (uint16_t)px[q];
-      }
-    }
+  // 1. runs: one per wave of np / kWaves keys once np >= T (R keys per
+  // lane), else np / 64 runs of 64 on the first waves
+  const uint32_t per = np > (uint32_t)T ? np / T : 1;
+  if (per == 1) {
+    if (w < np / kWave) wave_sort_run<1, kPay>(s, v, w, lane);
+  } else if (per == 2) {
+    wave_sort_run<kItems >= 2 ? 2 : 1, kPay>(s, v, w, lane);
+  } else {
+    wave_sort_run<kItems, kPay>(s, v, w, lane);
   }
   __syncthreads();
   // 2. thread t owns the `per` consecutive keys [t * per, +per), all in one
   // run: the first and last are ranked by a full search of the other run,
   // the ones between inside the range those two bound
-  const uint32_t per = np > (uint32_t)T ? np / T : 1;
   const uint32_t i0 = threadIdx.x * per;
-  for (uint32_t len = kWave; len < np; len <<= 1) {
+  for (uint32_t len = kWave * per; len < np; len <<= 1) {
     uint64_t x[kItems];
     uint16_t px[kItems];
     uint32_t at[kItems];
