@@ -773,6 +773,29 @@ def main():
         extra["bfs_hub_replayed"] = dict(src=hub, ms=1e3 * t, levels_match=ok,
                                          gteps=extra["bfs_hub"]["m_reached"] / t / 1e9)
         log(f"bfs_hub replayed: {1e3 * t:.3f} ms (levels {'match' if ok else 'DIFFER'})")
+        # BC from the hub, launched plainly and replayed from its recording
+        d_bc = S.DeviceArray(n, np.float64)
+        Lb = S.lib()
+        bc_step = lambda: S._check(Lb.sstg_bc(g.handle, hub, d_bc.ptr,
+                                              S.SSTG_DEVICE_PTRS), "bc")
+        bc_step()
+        S.synchronize()
+        t = float(np.median([timed(bc_step) for _ in range(5)]))
+        bc_plain = d_bc.to_host()
+        extra["bc_hub"] = dict(src=hub, ms=1e3 * t)
+        os.environ["SSTG_GRAPH_CAPTURE"] = "1"
+        try:
+            bc_step()
+            S.synchronize()
+            t = float(np.median([timed(bc_step) for _ in range(5)]))
+        finally:
+            del os.environ["SSTG_GRAPH_CAPTURE"]
+        bc_rep = d_bc.to_host()
+        diff = float(np.max(np.abs(bc_rep - bc_plain) / np.maximum(np.abs(bc_plain), 1e-12)))
+        extra["bc_hub_replayed"] = dict(src=hub, ms=1e3 * t, max_rel_diff_vs_plain=diff)
+        log(f"bc_hub: {extra['bc_hub']['ms']:.3f} ms, replayed {1e3 * t:.3f} ms "
+            f"(max rel diff {diff:.1e})")
+        d_bc.free()
         d_pr = S.DeviceArray(n, np.float64)
         EMV.PR_S(g, args.pr_iters, out=d_pr)
         S.synchronize()

@@ -741,60 +741,73 @@ static int bfs_core(sstg_graph *g, uint32_t src, uint32_t *level,
   return SSTG_OK;
 }
 
-void bfs_release(sstg_graph *g) {
-  if (g->bfs_exec) hipGraphExecDestroy((hipGraphExec_t)g->bfs_exec);
-  if (g->bfs_graph) hipGraphDestroy((hipGraph_t)g->bfs_graph);
-  g->bfs_exec = g->bfs_graph = nullptr;
-  delete static_cast<BfsKey *>(g->bfs_key);
-  g->bfs_key = nullptr;
+// A recorded launch sequence of a graph: hipGraph, its executable and the
+// key (a RecKey) it was recorded for.
+struct RecSlot {
+  void **graph, **exec, **key;
+};
+
+static void rec_release(RecSlot r) {
+  if (*r.exec) hipGraphExecDestroy((hipGraphExec_t)*r.exec);
+  if (*r.graph) hipGraphDestroy((hipGraph_t)*r.graph);
+  *r.exec = *r.graph = nullptr;
+  delete static_cast<RecKey *>(*r.key);
+  *r.key = nullptr;
 }
 
-static BfsKey bfs_key_of(const sstg_graph *g, uint32_t src, const int32_t *parent,
-                         const uint32_t *lv) {
-  BfsKey k{};
+void bfs_release(sstg_graph *g) {
+  rec_release(RecSlot{&g->bfs_graph, &g->bfs_exec, &g->bfs_key});
+  rec_release(RecSlot{&g->bc_graph, &g->bc_exec, &g->bc_key});
+}
+
+static RecKey rec_key_of(const sstg_graph *g, uint32_t src, const void *out0,
+                         const void *out1) {
+  RecKey k{};
   k.version = g->version;
   k.tune_epoch = g->tune_epoch;
   k.ws_generation = g->ws.generation;
   k.src = src;
   k.n = g->n_rows;
-  const void *p[5] = {lv, parent, g->off, g->deg, g->pool};
+  const void *p[5] = {out0, out1, g->off, g->deg, g->pool};
   for (int i = 0; i < 5; i++) k.ptr[i] = p[i];
   k.alpha = g->bfs_alpha;
   k.beta = g->bfs_beta;
   return k;
 }
 
-// A BFS from a given source on a given graph version takes the same steps
-// every time: every level's frontier is the same set, so the counts the
-// host reads back, and with them every launch's shape, repeat (only the
+// A traversal from a given source on a given graph version takes the same
+// steps every time: every level's frontier is the same set, so the counts
+// the host reads back, and with them every launch's shape, repeat (only the
 // order of the vertices inside a top-down level's queue differs).  The
-// first call runs plainly, logging each read-back; the same run is then
-// captured with the logged values standing in for the reads (no waits),
-// and later calls with the same key replay the recording in one launch.
-static int bfs_replayed(sstg_graph *g, uint32_t src, int32_t *parent,
-                        uint32_t *lv) {
+// first call runs plainly (run(log), no final wait), logging each read-back;
+// the same run is then captured with the logged values standing in for the
+// reads (no waits), and later calls with the same key replay the recording
+// in one launch.
+template <class Run>
+static int run_recorded(sstg_graph *g, RecSlot slot, uint32_t src,
+                        const void *out0, const void *out1, Run run) {
   hipStream_t s = g->stream;
-  BfsKey key = bfs_key_of(g, src, parent, lv);
-  const BfsKey *have = static_cast<const BfsKey *>(g->bfs_key);
-  if (g->bfs_exec && have && std::memcmp(have, &key, sizeof(key)) == 0) {
-    SSTG_HIP(hipGraphLaunch((hipGraphExec_t)g->bfs_exec, s));
+  RecKey key = rec_key_of(g, src, out0, out1);
+  const RecKey *have = static_cast<const RecKey *>(*slot.key);
+  if (*slot.exec && have && std::memcmp(have, &key, sizeof(key)) == 0) {
+    SSTG_HIP(hipGraphLaunch((hipGraphExec_t)*slot.exec, s));
     SSTG_HIP(hipStreamSynchronize(s));
     return SSTG_OK;
   }
-  bfs_release(g);
+  rec_release(slot);
   ReadLog log;
   log.mode = 1;
-  SSTG_TRY(bfs_core(g, src, lv, parent, nullptr, false, nullptr, &log));
+  SSTG_TRY(run(&log));
   SSTG_HIP(hipStreamSynchronize(s));
   // after the run: it may have allocated workspace or built cached data
-  key = bfs_key_of(g, src, parent, lv);
+  key = rec_key_of(g, src, out0, out1);
   log.mode = 2;
   log.pos = 0;
   Profiler &prof = profiler();
   const bool prof_on = prof.on;
   prof.on = false;  // no event records inside the capture
   SSTG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  const int rc = bfs_core(g, src, lv, parent, nullptr, false, nullptr, &log);
+  const int rc = run(&log);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &graph);
   prof.on = prof_on;
@@ -813,16 +826,21 @@ static int bfs_replayed(sstg_graph *g, uint32_t src, int32_t *parent,
     hipGraphDestroy(graph);
     SSTG_HIP(ei);
   }
-  g->bfs_graph = graph;
-  g->bfs_exec = exec;
-  g->bfs_key = new BfsKey(key);
+  *slot.graph = graph;
+  *slot.exec = exec;
+  *slot.key = new RecKey(key);
   return SSTG_OK;
 }
 
 int algo_bfs(sstg_graph *g, uint32_t src, int32_t *parent, uint32_t *level) {
   uint32_t *lv = level;
   if (!lv) SSTG_TRY(g->ws.get(kWsAlgo + 13, g->n_rows, &lv));
-  if (iteration_capture_enabled(g)) return bfs_replayed(g, src, parent, lv);
+  if (iteration_capture_enabled(g))
+    return run_recorded(
+        g, RecSlot{&g->bfs_graph, &g->bfs_exec, &g->bfs_key}, src, lv, parent,
+        [&](ReadLog *log) {
+          return bfs_core(g, src, lv, parent, nullptr, false, nullptr, log);
+        });
   SSTG_TRY(bfs_core(g, src, lv, parent, nullptr, false, nullptr));
   SSTG_HIP(hipStreamSynchronize(g->stream));
   return SSTG_OK;
@@ -985,7 +1003,7 @@ int algo_common_neighbors(sstg_graph *g, uint32_t u, uint32_t v,
 // level.
 static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
                       const uint32_t *order, const BfsRun &run, double *sigma,
-                      double *out) {
+                      double *out, ReadLog *log) {
   hipStream_t s = g->stream;
   const uint32_t n = g->n_rows;
   const uint64_t reached = run.reached;
@@ -1013,9 +1031,7 @@ static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
   uint32_t *list = reinterpret_cast<uint32_t *>(sorted == key ? tmp : key);
   SSTG_TRY(narrow_u64_to_u32(sorted, reached, list, s));
   std::vector<uint64_t> st(nb + 1);
-  SSTG_HIP(hipMemcpyAsync(st.data(), start, (nb + 1) * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-  SSTG_HIP(hipStreamSynchronize(s));
+  SSTG_TRY(read_back(g, log, st.data(), start, (nb + 1) * sizeof(uint64_t)));
   const DevGraph v = g->view();
   auto pull = [&](uint32_t L, auto f) -> int {
     using Fn = decltype(f);
@@ -1049,7 +1065,6 @@ static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
                 (const uint32_t *)(list + lb), le - lb, sigma);
   for (int64_t L = (int64_t)run.levels - 2; L >= 0; L--)
     SSTG_TRY(pull((uint32_t)L, DeltaPullBm{bm + (L + 1) * words, sigma, out}));
-  SSTG_HIP(hipStreamSynchronize(s));
   return SSTG_OK;
 }
 
@@ -1061,12 +1076,40 @@ int algo_bc(sstg_graph *g, uint32_t src, double *out) {
   SSTG_TRY(g->ws.get(kWsAlgo + 13, n, &level));
   SSTG_TRY(g->ws.get(kWsAlgo + 14, (uint64_t)n + 1, &order));
   SSTG_TRY(g->ws.get(kWsAlgo + 15, n, &sigma));
-  BfsRun run;
-  SSTG_TRY(bfs_core(g, src, level, nullptr, order, false, &run));
   bool symmetric = false;
   SSTG_TRY(graph_is_symmetric(g, &symmetric));
-  if (symmetric && run.levels < 64)
-    return bc_leveled(g, src, level, order, run, sigma, out);
+  // symmetric graphs of fewer than 64 levels (every benchmark graph): the
+  // leveled Brandes, recorded and replayed like BFS when asked for
+  auto leveled = [&](ReadLog *log, bool *done) -> int {
+    BfsRun run;
+    SSTG_TRY(bfs_core(g, src, level, nullptr, order, false, &run, log));
+    *done = run.levels < 64;
+    if (!*done) return SSTG_OK;
+    return bc_leveled(g, src, level, order, run, sigma, out, log);
+  };
+  if (symmetric && iteration_capture_enabled(g)) {
+    // the level count is fixed for a key, so a recording is always leveled;
+    // a graph of 64 levels or more records nothing and runs plainly below
+    bool deep = false;
+    const int rc = run_recorded(
+        g, RecSlot{&g->bc_graph, &g->bc_exec, &g->bc_key}, src, out, nullptr,
+        [&](ReadLog *log) {
+          bool done = false;
+          SSTG_TRY(leveled(log, &done));
+          if (done) return (int)SSTG_OK;
+          deep = true;
+          return (int)SSTG_E_UNSUPPORTED;
+        });
+    if (!deep) return rc;
+    SSTG_HIP(hipStreamSynchronize(s));
+  }
+  BfsRun run;
+  SSTG_TRY(bfs_core(g, src, level, nullptr, order, false, &run));
+  if (symmetric && run.levels < 64) {
+    SSTG_TRY(bc_leveled(g, src, level, order, run, sigma, out, nullptr));
+    SSTG_HIP(hipStreamSynchronize(s));
+    return SSTG_OK;
+  }
   SSTG_LAUNCH(k_bc_init, blocks(n), kT, 0, s, n, sigma, out, src);
   // forward: sigma level by level (level 0 is the source, sigma = 1); pulled
   // from the parents on symmetric graphs, pushed along out-edges otherwise

@@ -35,10 +35,10 @@ bool iteration_capture_enabled(const sstg_graph *g);
 // of the last recording on the same graph version, settings, outputs and
 // workspace replays that recording (hipGraph) instead of launching level by
 // level with a host read-back per level.
-struct BfsKey {
+struct RecKey {
   uint64_t version, tune_epoch, ws_generation;
   uint32_t src, n;
-  const void *ptr[5];  // levels, parents, and the store's off, deg, pool
+  const void *ptr[5];  // outputs (two), and the store's off, deg, pool
   double alpha, beta;
 };
 void bfs_release(sstg_graph *g);

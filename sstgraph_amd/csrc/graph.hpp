@@ -65,8 +65,10 @@ struct sstg_graph {
   // and what it was recorded for; no_capture: never record (server views,
   // whose threads share the device with synchronous copies)
   void *pr_graph = nullptr, *pr_exec = nullptr, *pr_key = nullptr;
-  // a recorded BFS (one source's whole launch sequence) and its key
+  // a recorded BFS and a recorded BC (one source's whole launch sequence)
+  // and their keys
   void *bfs_graph = nullptr, *bfs_exec = nullptr, *bfs_key = nullptr;
+  void *bc_graph = nullptr, *bc_exec = nullptr, *bc_key = nullptr;
   bool no_capture = false;
   // cached: does every stored edge have its reverse?
   uint64_t sym_version = ~0ull;

@@ -381,3 +381,44 @@ def test_bfs_replayed_runs_match_oracle(c1, graph_capture):
     o_off, o_cols = og.csr()
     for src in (hub, hub, 7, 7):
         _check_bfs(g, o_off, o_cols, src)
+
+
+def test_bc_replayed_runs_match_oracle(c1, graph_capture):
+    """With recording on, BC from a source records the whole launch sequence
+    (its BFS, the grouping by level and the sigma and delta pulls) and later
+    calls with the same source, output and graph version replay it: every
+    call agrees with the oracle within 1e-9, across repeated calls with
+    other device work between them, source changes, a threshold change and
+    an update."""
+    n, _, off, cols = c1
+    s = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off).astype(np.int64))
+    g = S.SparseMatrixV(n)
+    g.insert_batch(s, cols)
+    hub = int(np.argmax(np.diff(off)))
+    want = {v: O.bc(off, cols, v) for v in (hub, 12345)}
+    for src in (hub, hub, 12345, 12345, hub):
+        say("replayed bc", src)
+        ok, w = rel_close(EMV.BC(g, src), want[src], RTOL, atol_floor=1e-12)
+        assert ok, (src, w)
+    for k in range(2):
+        EMV.BFS(g, k, levels=True)
+        EMV.PR_S(g, 2)
+        S.synchronize()
+        ok, w = rel_close(EMV.BC(g, hub), want[hub], RTOL, atol_floor=1e-12)
+        assert ok, (k, w)
+    g.set_degree_thresholds([4, 32, 1024])
+    for _ in range(2):
+        ok, w = rel_close(EMV.BC(g, hub), want[hub], RTOL, atol_floor=1e-12)
+        assert ok, w
+    rng = np.random.default_rng(11)
+    bs = rng.integers(0, n, 4000).astype(np.uint32)
+    bd = rng.integers(0, n, 4000).astype(np.uint32)
+    g.insert_batch(np.concatenate([bs, bd]), np.concatenate([bd, bs]))
+    og = O.Graph(n)
+    og.load_csr(off, cols)
+    og.insert_batch(np.concatenate([bs, bd]), np.concatenate([bd, bs]))
+    o_off, o_cols = og.csr()
+    w2 = O.bc(o_off, o_cols, hub)
+    for _ in range(2):
+        ok, w = rel_close(EMV.BC(g, hub), w2, RTOL, atol_floor=1e-12)
+        assert ok, w
