@@ -223,10 +223,17 @@ __global__ void __launch_bounds__(kRsThreads)
     (&h[0][0])[i] = 0;
   __syncthreads();
   uint64_t base = (uint64_t)blockIdx.x * kRsTile;
-#pragma unroll 4
+  // all of a thread's keys in flight before the first count
+  uint64_t kv[kRsItems];
+#pragma unroll
   for (int j = 0; j < kRsItems; j++) {
     uint64_t i = base + (uint64_t)j * kRsThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    kv[j] = i < n ? keys[i] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    uint64_t i = base + (uint64_t)j * kRsThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[w][(uint32_t)(kv[j] >> shift) & 255u], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < kRsBins; d += kRsThreads) {
@@ -262,11 +269,17 @@ __global__ void __launch_bounds__(kRsThreads)
                         (uint64_t)w * (kWave * kRsItems);
   uint64_t key[kRsItems];
   uint32_t rank[kRsItems];
+  // all of a lane's keys in flight before the ranking (ranking each key as
+  // it arrived waited for every load in turn)
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    uint64_t i = base + (uint64_t)j * kWave + lane;
+    key[j] = i < n ? in[i] : 0;
+  }
 #pragma unroll
   for (int j = 0; j < kRsItems; j++) {
     uint64_t i = base + (uint64_t)j * kWave + lane;
     bool valid = i < n;
-    key[j] = valid ? in[i] : 0;
     uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -434,7 +447,20 @@ __global__ void __launch_bounds__(kSsThreads)
   keys += (uint64_t)blockIdx.x * kMsChunk;
   const uint32_t m = (uint32_t)min(n - (uint64_t)blockIdx.x * kMsChunk,
                                    (uint64_t)kMsChunk);
-  for (uint32_t i = threadIdx.x; i < m; i += kSsThreads) s[i] = keys[i];
+  {  // the chunk's loads in flight together
+    constexpr int kPer = kMsChunk / kSsThreads;
+    uint64_t v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const uint32_t i = threadIdx.x + (uint32_t)j * kSsThreads;
+      v[j] = i < m ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const uint32_t i = threadIdx.x + (uint32_t)j * kSsThreads;
+      if (i < m) s[i] = v[j];
+    }
+  }
   block_sort_u64<kSsThreads>(s, m);  // begins and ends with a barrier
   for (uint32_t i = threadIdx.x; i < m; i += kSsThreads) keys[i] = s[i];
 }
@@ -507,10 +533,21 @@ __global__ void __launch_bounds__(kMpThreads)
   const uint32_t na = (uint32_t)(ia1 - ia0);
   const uint32_t nb = (uint32_t)((pe - ia1) - (pb - ia0));
   const uint64_t ib0 = pb - ia0;
-  for (uint32_t i = threadIdx.x; i < na; i += kMpThreads)
-    sa[i] = src[a0 + ia0 + i];
-  for (uint32_t i = threadIdx.x; i < nb; i += kMpThreads)
-    sb[i] = src[b0 + ib0 + i];
+  {  // both pieces (na + nb <= kMpOut) in flight before the LDS stores
+    uint64_t va[kMpItems], vb[kMpItems];
+#pragma unroll
+    for (int j = 0; j < kMpItems; j++) {
+      const uint32_t i = threadIdx.x + (uint32_t)j * kMpThreads;
+      va[j] = i < na ? src[a0 + ia0 + i] : 0;
+      vb[j] = i < nb ? src[b0 + ib0 + i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kMpItems; j++) {
+      const uint32_t i = threadIdx.x + (uint32_t)j * kMpThreads;
+      if (i < na) sa[i] = va[j];
+      if (i < nb) sb[i] = vb[j];
+    }
+  }
   __syncthreads();
   const uint32_t cnt = (uint32_t)(pe - pb);
   const uint32_t d = threadIdx.x * kMpItems;

@@ -699,31 +699,54 @@ __global__ void __launch_bounds__(kMergeThreads)
     }
     __syncthreads();
   }
+  // Every input's address first, then all of a thread's loads, then their
+  // LDS stores: a thread keeps up to kMergeItems loads in flight (with one
+  // store after each load, the compiler waited for every load in turn).  A
+  // batch input is the low word of its 64-bit key.  Slots past the tile
+  // load the pool's first word (always there when tile rows exist) and are
+  // not stored, so the loads need no branch.
   uint16_t row_of[kMergeItems];
+  const uint32_t *ptr[kMergeItems];
+  const uint32_t *bw = reinterpret_cast<const uint32_t *>(a.B);
+  if (table) {
+#pragma unroll
+    for (int k = 0; k < kMergeItems; k++) {
+      const uint32_t e = threadIdx.x + (uint32_t)k * kMergeThreads;
+      row_of[k] = 0;
+      ptr[k] = g.pool;
+      if (e < W) {
+        const uint32_t r = rmap[e];
+        const uint32_t alen = ralen[r], j = e - rstart[r];
+        row_of[k] = (uint16_t)r;
+        ptr[k] = j < alen ? g.pool + rsrc[r] + j
+                          : bw + 2 * ((uint64_t)rbstart[r] + (j - alen));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kMergeItems; k++) {
+      const uint32_t e = threadIdx.x + (uint32_t)k * kMergeThreads;
+      row_of[k] = 0;
+      ptr[k] = g.pool;
+      if (e < W) {
+        const uint64_t i = find_in(a.tile_off, r0, r1 - 1, e0 + e);
+        uint64_t src, dst;
+        uint32_t alen, bst;
+        tile_row_info(a, g, i, insert, &src, &dst, &alen, &bst);
+        const uint32_t j = (uint32_t)(e0 + e - a.tile_off[i]);
+        row_of[k] = (uint16_t)(i - r0);
+        ptr[k] = j < alen ? g.pool + src + j : bw + 2 * ((uint64_t)bst + (j - alen));
+      }
+    }
+  }
+  uint32_t val[kMergeItems];
+#pragma unroll
+  for (int k = 0; k < kMergeItems; k++)  // device memory: global, not flat, loads
+    val[k] = *(const __attribute__((address_space(1))) uint32_t *)ptr[k];
 #pragma unroll
   for (int k = 0; k < kMergeItems; k++) {
     const uint32_t e = threadIdx.x + (uint32_t)k * kMergeThreads;
-    row_of[k] = 0;
-    if (e < W) {
-      uint32_t r;
-      uint64_t src;
-      uint32_t alen, bst, j;
-      if (table) {
-        r = rmap[e];
-        src = rsrc[r];
-        alen = ralen[r];
-        bst = rbstart[r];
-        j = e - rstart[r];
-      } else {
-        uint64_t i = find_in(a.tile_off, r0, r1 - 1, e0 + e);
-        uint64_t dst;
-        tile_row_info(a, g, i, insert, &src, &dst, &alen, &bst);
-        r = (uint32_t)(i - r0);
-        j = (uint32_t)(e0 + e - a.tile_off[i]);
-      }
-      row_of[k] = (uint16_t)r;
-      buf[e] = j < alen ? g.pool[src + j] : (uint32_t)a.B[bst + (j - alen)];
-    }
+    if (e < W) buf[e] = val[k];
   }
   __syncthreads();
 #pragma unroll

@@ -321,11 +321,23 @@ __global__ void __launch_bounds__(kSmT)
     s_sym = 1;
   }
   __syncthreads();
-  for (uint32_t i = tid; i < n; i += kSmT) {
-    const uint32_t a = src[i], b = dst[i];
-    if (a >= g.n || b >= n_cols) s_bad = 1;
-    key[i] = ((uint64_t)a << 32) | b;
-    kidx[i] = (uint16_t)i;
+  {  // a thread's pairs in flight together
+    uint32_t va[kSmItems], vb[kSmItems];
+#pragma unroll
+    for (int j = 0; j < kSmItems; j++) {
+      const uint32_t i = tid + (uint32_t)j * kSmT;
+      va[j] = i < n ? src[i] : 0;
+      vb[j] = i < n ? dst[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kSmItems; j++) {
+      const uint32_t i = tid + (uint32_t)j * kSmT;
+      if (i < n) {
+        if (va[j] >= g.n || vb[j] >= n_cols) s_bad = 1;
+        key[i] = ((uint64_t)va[j] << 32) | vb[j];
+        kidx[i] = (uint16_t)i;
+      }
+    }
   }
   __syncthreads();
   if (s_bad) {
