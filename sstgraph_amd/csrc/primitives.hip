@@ -184,20 +184,26 @@ __global__ void __launch_bounds__(kScanThreads)
   const uint64_t hi = lo + per < n ? lo + per : n;
   uint64_t carry = offsets ? offsets[blockIdx.x] : 0;
   for (uint64_t c = lo; c < hi; c += kScanTile) {
+    // flags and values loaded together, before the scans (a value loaded
+    // after its scan cost a second round trip per step)
     uint8_t f[kScanItems];
+    V val[kScanItems];
+    uint32_t val2[kScanItems];
 #pragma unroll
     for (int j = 0; j < kScanItems; j++) {
       uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
-      f[j] = (i < hi && flags[i]) ? 1 : 0;
+      const bool in = i < hi;
+      f[j] = (in && flags[i]) ? 1 : 0;
+      val[j] = (in && values) ? values[i] : (V)i;
+      val2[j] = (in && out2) ? values2[i] : 0u;
     }
 #pragma unroll
     for (int j = 0; j < kScanItems; j++) {
-      uint64_t i = c + (uint64_t)j * kScanThreads + threadIdx.x;
       uint64_t tot;
       uint64_t ex = block_exclusive((uint64_t)f[j], wtot, &tot);
       if (f[j]) {
-        out[carry + ex] = values ? values[i] : (V)i;
-        if (out2) out2[carry + ex] = values2[i];
+        out[carry + ex] = val[j];
+        if (out2) out2[carry + ex] = val2[j];
       }
       carry += tot;
     }

@@ -215,15 +215,36 @@ struct SegScan {
   uint64_t *hctr;  // mapped host copy of the counter block, or null
 };
 
-__device__ __forceinline__ void seg_sums(const SegScan &a, uint64_t s,
-                                         uint64_t v[kSegSums]) {
-  const bool big = a.is_big[s], til = a.is_tile[s];
-  v[0] = a.alloc[s];
-  v[1] = big;
-  v[2] = til;
-  v[3] = big ? a.stage[s] : 0;
-  v[4] = big ? a.work[s] : 0;
-  v[5] = til ? a.tile[s] : 0;
+// The six running sums of the K slots s0, s0 + 1, ... below hi (allocation,
+// big and tile row flags, big rows' staging and work, tile inputs; zeros
+// past hi), their loads issued together: the class flags, allocations and
+// tile inputs (most rows are tile rows) in one round, the big rows' staging
+// and work in a second (one slot at a time, the loads of a slot depended on
+// its flags).
+template <int K>
+__device__ __forceinline__ void seg_sums_run(const SegScan &a, uint64_t s0,
+                                             uint64_t hi,
+                                             uint64_t v[K][kSegSums]) {
+  uint8_t big[K], til[K];
+  uint64_t al[K];
+  uint32_t tl[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const bool in = s0 + j < hi;
+    big[j] = in ? a.is_big[s0 + j] : 0;
+    til[j] = in ? a.is_tile[s0 + j] : 0;
+    al[j] = in ? a.alloc[s0 + j] : 0;
+    tl[j] = in ? a.tile[s0 + j] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    v[j][0] = al[j];
+    v[j][1] = big[j] != 0;
+    v[j][2] = til[j] != 0;
+    v[j][3] = big[j] ? a.stage[s0 + j] : 0;
+    v[j][4] = big[j] ? a.work[s0 + j] : 0;
+    v[j][5] = til[j] ? tl[j] : 0;
+  }
 }
 
 // End of a block's range.  Slots past t (ctr[kCtrT]) are zero and only slot
@@ -239,11 +260,13 @@ __global__ void __launch_bounds__(kT) k_seg_reduce(SegScan a) {
   const uint64_t lo = (uint64_t)blockIdx.x * a.per;
   const uint64_t hi = seg_end(a, lo);
   uint64_t acc[kSegSums] = {0, 0, 0, 0, 0, 0};
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += kT) {
-    uint64_t v[kSegSums];
-    seg_sums(a, i, v);
+  for (uint64_t c = lo; c < hi; c += kSegTile) {
+    uint64_t v[kSegItems][kSegSums];
+    seg_sums_run<kSegItems>(a, c + (uint64_t)threadIdx.x * kSegItems, hi, v);
 #pragma unroll
-    for (int k = 0; k < kSegSums; k++) acc[k] += v[k];
+    for (int j = 0; j < kSegItems; j++)
+#pragma unroll
+      for (int k = 0; k < kSegSums; k++) acc[k] += v[j][k];
   }
 #pragma unroll
   for (int k = 0; k < kSegSums; k++) {
@@ -303,14 +326,9 @@ __global__ void __launch_bounds__(kT) k_seg_scan(SegScan a) {
     uint64_t v[kSegItems][kSegSums], ex[kSegSums];
 #pragma unroll
     for (int k = 0; k < kSegSums; k++) ex[k] = 0;
+    seg_sums_run<kSegItems>(a, i0, hi, v);
 #pragma unroll
     for (int j = 0; j < kSegItems; j++) {
-      if (i0 + j < hi) {
-        seg_sums(a, i0 + j, v[j]);
-      } else {
-#pragma unroll
-        for (int k = 0; k < kSegSums; k++) v[j][k] = 0;
-      }
 #pragma unroll
       for (int k = 0; k < kSegSums; k++) ex[k] += v[j][k];
     }
