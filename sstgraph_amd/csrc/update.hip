@@ -481,12 +481,30 @@ __global__ void __launch_bounds__(kT)
   if (e >= total) return;
   uint64_t end = e + kChunk < total ? e + kChunk : total;
   uint64_t i = find_in(a.stage_off, s_lo, s_hi, e);
-  for (; e < end; e++) {
-    if (e >= a.stage_off[i + 1]) i = find_in(a.stage_off, i, s_hi, e);
-    uint64_t s = a.big_idx[i];
-    uint32_t v = a.seg_v[s];
-    a.scratch[e] = g.pool[g.off[v] + a.seg_p0[s] + (e - a.stage_off[i])];
+  // the row's source offset once per row (big rows are long, so a thread's
+  // elements nearly always share it), then every load before any store
+  auto src_of = [&](uint64_t r) {
+    const uint64_t s = a.big_idx[r];
+    return g.off[a.seg_v[s]] + a.seg_p0[s] - a.stage_off[r];
+  };
+  uint64_t next = a.stage_off[i + 1], base = src_of(i);
+  uint32_t val[kChunk];
+#pragma unroll
+  for (int k = 0; k < kChunk; k++) {
+    const uint64_t ek = e + k;
+    val[k] = 0;
+    if (ek < end) {
+      if (ek >= next) {
+        i = find_in(a.stage_off, i, s_hi, ek);
+        next = a.stage_off[i + 1];
+        base = src_of(i);
+      }
+      val[k] = g.pool[base + ek];
+    }
   }
+#pragma unroll
+  for (int k = 0; k < kChunk; k++)
+    if (e + k < end) a.scratch[e + k] = val[k];
 }
 
 struct MergeSeg {
