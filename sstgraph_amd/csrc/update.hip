@@ -658,7 +658,8 @@ static_assert(kTileRows <= 256, "tile rows are numbered in bytes");
 // 128-row table and the 3 KB row map take 18.5 KB, so eight tiles (the wave
 // limit) are in flight per CU; tiles with more rows read their row
 // descriptors from HBM and search them.  The block shape was chosen by
-// timing the 1e7-edge insert at scale 24 (input range / longest tile row:
+// timing the 1e7-edge insert at scale 24, before the input loads were
+// issued together (input range / longest tile row:
 // 1536 / 1536 merge 1.046 ms, insert 2.50 ms; 1024 / 2048 1.09 / 2.54;
 // 2048 / 1024 1.015 / 2.66 and 1792 / 1280 1.01 / 2.60, whose longer rows
 // go to the slower big-row path; 2048 / 1536 1.09 / 2.54 and 1536 / 2048
