@@ -174,7 +174,6 @@ __device__ __forceinline__ void wave_sort_run(uint64_t *s, uint16_t *v,
 template <int T, bool kPay>
 __device__ void block_sort_impl(uint64_t *s, uint16_t *v, uint32_t n) {
   static_assert(T % kWave == 0 && kBlockSortMax % T == 0, "block shape");
-  constexpr int kWaves = T / kWave;
   constexpr int kItems = kBlockSortMax / T;  // keys per thread, at most
   static_assert(kItems == 1 || kItems == 2 || kItems == 4, "keys per lane");
   uint32_t np = kWave;
@@ -185,7 +184,7 @@ __device__ void block_sort_impl(uint64_t *s, uint16_t *v, uint32_t n) {
   }
   __syncthreads();
   const uint32_t lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  // 1. runs: one per wave of np / kWaves keys once np >= T (R keys per
+  // 1. runs: one per wave of 64 * np / T keys once np > T (np / T keys per
   // lane), else np / 64 runs of 64 on the first waves
   const uint32_t per = np > (uint32_t)T ? np / T : 1;
   if (per == 1) {
