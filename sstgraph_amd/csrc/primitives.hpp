@@ -40,7 +40,7 @@ int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
                    hipStream_t s, uint64_t **result);
 
 // Full sort of n 64-bit keys whose set bits all lie in the given 8-bit
-// digits: up to 2M keys, whole-key sorts of 4K-key chunks (one block each)
+// digits: up to 2^20 keys, whole-key sorts of 4K-key chunks (one block each)
 // and merge passes (one launch each); beyond that
 // radix_sort_u64 over the digits.  Same contract for keys, tmp and *result.
 int sort_keys_u64(uint64_t *keys, uint64_t *tmp, uint64_t n, const int *shifts,

@@ -58,14 +58,14 @@ def test_batch_sizes_around_the_sort_and_scan_paths():
     """Batches on both sides of the one-block LDS sort (4096 keys for an
     update, 16384 for the radix sort's own one-block path), of the one-block
     scans (4096 values) and of the merge sort (4K-key block sorts and merge
-    passes up to 2^21 keys, the radix sort beyond), inserted and then half
+    passes up to 2^20 keys, the radix sort beyond), inserted and then half
     removed; ids below 2^17 so every batch sorts on three column and three
     row digits."""
     n = 1 << 17
     rng = np.random.default_rng(11)
     g, og = S.SparseMatrixV(n), O.Graph(n)
     for k in (2, 63, 64, 65, 4095, 4096, 4097, 8192, 12289, 16383, 16384,
-              16385, 32768, 40000, 49153, 1 << 21, (1 << 21) + 1):
+              16385, 32768, 40000, 49153, 1 << 20, (1 << 20) + 1, 1 << 21):
         say("batch", k)
         s = rng.integers(0, n, k).astype(np.uint32)
         d = rng.integers(0, n, k).astype(np.uint32)
