@@ -105,13 +105,23 @@ __global__ void k_mark(const uint64_t *keys, uint64_t n, DevGraph g,
   if (insert && skip_self && s == d) keep = false;
   uint32_t p = 0;
   if (keep) {
-    uint32_t dg = g.deg[s];
-    bool exists = false;
-    if (dg) {
-      const uint32_t *row = g.pool + g.off[s];
-      p = (uint32_t)lower_bound_dev(row, (uint64_t)dg, d);
-      exists = p < dg && row[p] == d;
+    // degree and offset loaded together; the search keeps the value at its
+    // upper bound, so membership needs no load after it
+    const uint32_t dg = g.deg[s];
+    const uint32_t *row = g.pool + g.off[s];
+    uint32_t lo = 0, hi = dg, vhi = 0;  // answer in [lo, hi]; row[hi] == vhi
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      const uint32_t v = row[mid];
+      if (v < d) {
+        lo = mid + 1;
+      } else {
+        hi = mid;
+        vhi = v;
+      }
     }
+    p = lo;
+    const bool exists = p < dg && vhi == d;
     keep = insert ? !exists : exists;
   }
   flags[i] = keep ? 1 : 0;
