@@ -409,12 +409,7 @@ __global__ void k_symmetry(DevGraph g, const uint64_t *prefix, uint64_t m,
     while (e >= prefix[u + 1]) u++;
     uint32_t v = g.pool[g.off[u] + (e - prefix[u])];
     bool ok = false;
-    if (v < g.n) {
-      uint32_t dv = g.deg[v];
-      const uint32_t *row = g.pool + g.off[v];
-      uint64_t p = dv ? lower_bound_dev(row, (uint64_t)dv, (uint32_t)u) : 0;
-      ok = p < dv && row[p] == (uint32_t)u;
-    }
+    if (v < g.n) ok = row_has(g.pool + g.off[v], g.deg[v], (uint32_t)u);
     if (!ok) *flag = 0;
   }
 }

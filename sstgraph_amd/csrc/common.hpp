@@ -146,6 +146,27 @@ __device__ __forceinline__ uint64_t lower_bound_dev(const T *a, uint64_t len,
   return lo;
 }
 
+// Whether x is in the sorted row a[0..len), by a binary search that keeps
+// the value at its upper bound, so membership needs no load after the
+// search (the row's degree and offset can be loaded together beforehand).
+// *at, if given, gets the first index with a[i] >= x.
+__device__ __forceinline__ bool row_has(const uint32_t *a, uint32_t len,
+                                        uint32_t x, uint32_t *at = nullptr) {
+  uint32_t lo = 0, hi = len, vhi = 0;  // answer in [lo, hi]; a[hi] == vhi
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t v = a[mid];
+    if (v < x) {
+      lo = mid + 1;
+    } else {
+      hi = mid;
+      vhi = v;
+    }
+  }
+  if (at) *at = lo;
+  return lo < len && vhi == x;
+}
+
 // index of the segment containing flat position x, given exclusive offsets
 // off[0..k) (off[0] == 0, non-decreasing): largest s with off[s] <= x.
 __device__ __forceinline__ uint64_t find_segment(const uint64_t *off,

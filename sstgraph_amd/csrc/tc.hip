@@ -359,8 +359,7 @@ __global__ void __launch_bounds__(256)
     for (uint32_t j = lane; j < da; j += 64) {
       const uint32_t w = ra[j];
       if (w == u || w == v) continue;
-      const uint64_t p = lower_bound_dev(rb, (uint64_t)db, w);
-      if (p >= db || rb[p] != w) continue;
+      if (!row_has(rb, db, w)) continue;
       const uint64_t kuw = u < w ? ((uint64_t)u << 32) | w
                                  : ((uint64_t)w << 32) | u;
       const uint64_t kvw = v < w ? ((uint64_t)v << 32) | w

@@ -104,24 +104,10 @@ __global__ void k_mark(const uint64_t *keys, uint64_t n, DevGraph g,
   uint32_t s = (uint32_t)(k >> 32), d = (uint32_t)k;
   if (insert && skip_self && s == d) keep = false;
   uint32_t p = 0;
-  if (keep) {
-    // degree and offset loaded together; the search keeps the value at its
-    // upper bound, so membership needs no load after it
+  if (keep) {  // degree and offset loaded together, no load after the search
     const uint32_t dg = g.deg[s];
     const uint32_t *row = g.pool + g.off[s];
-    uint32_t lo = 0, hi = dg, vhi = 0;  // answer in [lo, hi]; row[hi] == vhi
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      const uint32_t v = row[mid];
-      if (v < d) {
-        lo = mid + 1;
-      } else {
-        hi = mid;
-        vhi = v;
-      }
-    }
-    p = lo;
-    const bool exists = p < dg && vhi == d;
+    const bool exists = row_has(row, dg, d, &p);
     keep = insert ? !exists : exists;
   }
   flags[i] = keep ? 1 : 0;
@@ -870,14 +856,7 @@ __global__ void k_sym_batch(const uint64_t *B, uint64_t b, DevGraph g,
   if (i >= b) return;
   uint32_t u = (uint32_t)(B[i] >> 32), v = (uint32_t)B[i];
   bool has = false;
-  if (v < g.n) {
-    uint32_t dv = g.deg[v];
-    if (dv) {
-      const uint32_t *row = g.pool + g.off[v];
-      uint64_t p = lower_bound_dev(row, (uint64_t)dv, u);
-      has = p < dv && row[p] == u;
-    }
-  }
+  if (v < g.n) has = row_has(g.pool + g.off[v], g.deg[v], u);
   if (has != (insert != 0)) *flag = 0;
 }
 
@@ -925,14 +904,7 @@ __global__ void k_has(DevGraph g, const uint32_t *rows, const uint32_t *cols,
   if (i >= n) return;
   uint32_t r = rows[i], c = cols[i];
   uint8_t h = 0;
-  if (r < g.n && c < n_cols) {
-    uint32_t dg = g.deg[r];
-    if (dg) {
-      const uint32_t *row = g.pool + g.off[r];
-      uint64_t p = lower_bound_dev(row, (uint64_t)dg, c);
-      h = p < dg && row[p] == c;
-    }
-  }
+  if (r < g.n && c < n_cols) h = row_has(g.pool + g.off[r], g.deg[r], c);
   out[i] = h;
 }
 

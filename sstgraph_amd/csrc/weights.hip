@@ -188,10 +188,7 @@ __global__ void k_wt_get(DevGraph g, WtTable t, int has_table,
   const uint32_t r = rows[i], c = cols[i];
   uint32_t w = 0;
   if (r < g.n) {
-    const uint32_t d = g.deg[r];
-    const uint32_t *row = g.pool + g.off[r];
-    const uint64_t p = lower_bound_dev(row, (uint64_t)d, c);
-    if (p < d && row[p] == c) {
+    if (row_has(g.pool + g.off[r], g.deg[r], c)) {
       w = 1;
       if (has_table) {
         const int64_t h = wt_find(t, ((uint64_t)r << 32) | c);
