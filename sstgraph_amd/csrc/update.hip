@@ -137,50 +137,58 @@ __global__ void k_segprep(const uint64_t *B, const uint32_t *B_pos,
                           uint8_t *is_big, uint8_t *is_tile,
                           uint64_t *garbage) {
   uint64_t s = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (s > nmax) return;
-  uint64_t t = ctr[kCtrT], b = ctr[kCtrB];
-  if (s >= t) {  // slot t is read as zero by the scans; later slots are not
-    if (s == t) {
-      seg_alloc[s] = 0;
-      is_big[s] = 0;
-      is_tile[s] = 0;
-    }
-    return;
-  }
-  uint64_t bs = seg_idx[s], be = (s + 1 < t) ? seg_idx[s + 1] : b;
-  uint64_t blen = be - bs;
-  uint32_t v = (uint32_t)(B[bs] >> 32);
-  uint32_t od = g.deg[v], cp = g.cap[v];
-  uint64_t p0 = 0, stage = 0, alloc = 0, work = 0;
-  if (insert) {
-    uint64_t nd = (uint64_t)od + blen;
-    if (nd > cp) {  // relocate: whole row merged into a fresh slot
-      alloc = grow_cap(nd, n_cols);
-      work = nd;
-      if (cp) atomicAdd((unsigned long long *)garbage, (unsigned long long)cp);
+  // every lane stays to the end: the abandoned slots of relocated rows are
+  // summed per wave and added with one atomic per wave (one per relocated
+  // row serialised on the counter when many rows move)
+  uint64_t gb = 0;
+  if (s <= nmax) {
+    const uint64_t t = ctr[kCtrT], b = ctr[kCtrB];
+    if (s >= t) {  // slot t is read as zero by the scans; later slots are not
+      if (s == t) {
+        seg_alloc[s] = 0;
+        is_big[s] = 0;
+        is_tile[s] = 0;
+      }
     } else {
-      p0 = od ? B_pos[bs] : 0;
-      stage = od - p0;
-      work = stage + blen;
+      uint64_t bs = seg_idx[s], be = (s + 1 < t) ? seg_idx[s + 1] : b;
+      uint64_t blen = be - bs;
+      uint32_t v = (uint32_t)(B[bs] >> 32);
+      uint32_t od = g.deg[v], cp = g.cap[v];
+      uint64_t p0 = 0, stage = 0, alloc = 0, work = 0;
+      if (insert) {
+        uint64_t nd = (uint64_t)od + blen;
+        if (nd > cp) {  // relocate: whole row merged into a fresh slot
+          alloc = grow_cap(nd, n_cols);
+          work = nd;
+          gb = cp;
+        } else {
+          p0 = od ? B_pos[bs] : 0;
+          stage = od - p0;
+          work = stage + blen;
+        }
+      } else {
+        p0 = B_pos[bs];
+        stage = od - p0;
+        work = stage;
+      }
+      // merge inputs: the moved suffix (or the whole row when relocating)
+      // plus the batch run
+      uint64_t inputs = (alloc ? (uint64_t)od : stage) + blen;
+      bool tile = inputs <= (uint64_t)kTileMax;
+      seg_v[s] = v;
+      seg_p0[s] = (uint32_t)p0;
+      seg_olddeg[s] = od;
+      seg_alloc[s] = alloc;
+      seg_stage[s] = (uint32_t)stage;
+      seg_work[s] = work;
+      seg_tile[s] = (uint32_t)inputs;
+      is_big[s] = !tile;
+      is_tile[s] = tile;
     }
-  } else {
-    p0 = B_pos[bs];
-    stage = od - p0;
-    work = stage;
   }
-  // merge inputs: the moved suffix (or the whole row when relocating) plus
-  // the batch run
-  uint64_t inputs = (alloc ? (uint64_t)od : stage) + blen;
-  bool tile = inputs <= (uint64_t)kTileMax;
-  seg_v[s] = v;
-  seg_p0[s] = (uint32_t)p0;
-  seg_olddeg[s] = od;
-  seg_alloc[s] = alloc;
-  seg_stage[s] = (uint32_t)stage;
-  seg_work[s] = work;
-  seg_tile[s] = (uint32_t)inputs;
-  is_big[s] = !tile;
-  is_tile[s] = tile;
+  gb = wave_sum(gb);
+  if ((threadIdx.x & (kWave - 1)) == 0 && gb)
+    atomicAdd((unsigned long long *)garbage, (unsigned long long)gb);
 }
 
 // Step 7 in three launches: the six running sums of the segment slots
