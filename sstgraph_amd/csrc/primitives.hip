@@ -265,13 +265,17 @@ __global__ void __launch_bounds__(kRsThreads)
   __shared__ uint32_t wtot[kRsWaves];
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
+  // consecutive tiles on one XCD: the digit runs of tiles t and t + 1 meet
+  // inside one cache line, which then fills in one L2 (with tiles dealt
+  // round-robin over the XCDs, two L2s wrote each such line in parts)
+  const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
   for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
     (&wcnt[0][0])[i] = 0;
   for (int d = threadIdx.x; d < kRsBins; d += kRsThreads)
-    gstart[d] = offs[(uint64_t)d * tiles + blockIdx.x];
+    gstart[d] = offs[(uint64_t)d * tiles + tile];
   __syncthreads();
 
-  const uint64_t base = (uint64_t)blockIdx.x * kRsTile +
+  const uint64_t base = (uint64_t)tile * kRsTile +
                         (uint64_t)w * (kWave * kRsItems);
   uint64_t key[kRsItems];
   uint32_t rank[kRsItems];
@@ -335,7 +339,7 @@ __global__ void __launch_bounds__(kRsThreads)
     }
   }
   __syncthreads();
-  const uint64_t tbase = (uint64_t)blockIdx.x * kRsTile;
+  const uint64_t tbase = (uint64_t)tile * kRsTile;
   const uint32_t valid_n =
       (uint32_t)((n - tbase) < (uint64_t)kRsTile ? (n - tbase) : kRsTile);
   for (uint32_t i = threadIdx.x; i < valid_n; i += kRsThreads) {
