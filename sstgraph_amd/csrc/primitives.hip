@@ -250,6 +250,46 @@ __global__ void __launch_bounds__(kRsThreads)
   }
 }
 
+// Block d: the exclusive prefix of digit d's tile counts hist[d * tiles + t]
+// over the tiles, in place, and the digit's total in dtot[d].  The scatter
+// adds the totals of the smaller digits itself, so one launch replaces a
+// three-launch scan of the whole table.
+__global__ void __launch_bounds__(kScanThreads)
+    k_rs_digit_scan(uint32_t *hist, uint32_t tiles, uint32_t *dtot) {
+  __shared__ uint32_t wtot[kScanThreads / kWave];
+  constexpr uint32_t kItems = 8;   // consecutive counts per thread and round
+  constexpr uint32_t kRounds = 4;  // rounds whose loads are issued together
+  constexpr uint32_t kRound = kScanThreads * kItems;
+  uint32_t *h = hist + (uint64_t)blockIdx.x * tiles;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < tiles; c0 += kRound * kRounds) {
+    uint32_t v[kRounds][kItems];
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; r++)
+#pragma unroll
+      for (uint32_t j = 0; j < kItems; j++) {
+        const uint32_t i = c0 + r * kRound + threadIdx.x * kItems + j;
+        v[r][j] = i < tiles ? h[i] : 0u;
+      }
+#pragma unroll
+    for (uint32_t r = 0; r < kRounds; r++) {
+      const uint32_t i0 = c0 + r * kRound + threadIdx.x * kItems;
+      uint32_t sum = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < kItems; j++) sum += v[r][j];
+      uint32_t tot;
+      uint32_t run = carry + block_exclusive(sum, wtot, &tot);
+#pragma unroll
+      for (uint32_t j = 0; j < kItems; j++) {
+        if (i0 + j < tiles) h[i0 + j] = run;
+        run += v[r][j];
+      }
+      carry += tot;
+    }
+  }
+  if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
 // Stable scatter of one tile.  Keys are read wave-striped (wave w, item j,
 // lane l -> tile offset w*64*ITEMS + j*64 + l), which is also their order for
 // stability.  Lanes with equal digits are found with 8 ballots; ranks are
@@ -257,7 +297,9 @@ __global__ void __launch_bounds__(kRsThreads)
 // sorted in LDS and then written out digit run by digit run (coalesced).
 __global__ void __launch_bounds__(kRsThreads)
     k_rs_scatter(const uint64_t *in, uint64_t *out, uint64_t n, int shift,
-                 const uint32_t *offs, uint32_t tiles) {
+                 const uint32_t *offs, const uint32_t *digit_tot,
+                 uint32_t tiles) {
+  static_assert(kRsThreads == kRsBins, "one thread per digit");
   __shared__ uint64_t stage[kRsTile];
   __shared__ uint32_t wcnt[kRsWaves][kRsBins];
   __shared__ uint32_t tstart[kRsBins];  // tile-local start of each digit
@@ -271,8 +313,19 @@ __global__ void __launch_bounds__(kRsThreads)
   const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
   for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
     (&wcnt[0][0])[i] = 0;
-  for (int d = threadIdx.x; d < kRsBins; d += kRsThreads)
-    gstart[d] = offs[(uint64_t)d * tiles + tile];
+  {  // global start of digit d: the tile's prefix within the digit plus the
+     // totals of the smaller digits
+    const uint32_t d = threadIdx.x;
+    const uint32_t own = offs[(uint64_t)d * tiles + tile], t = digit_tot[d];
+    const uint32_t incl = wave_inclusive_sum(t);
+    if (lane == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+#pragma unroll
+    for (int ww = 0; ww < kRsWaves; ww++)
+      if (ww < w) woff += wtot[ww];
+    gstart[d] = own + woff + incl - t;
+  }
   __syncthreads();
 
   const uint64_t base = (uint64_t)tile * kRsTile +
@@ -663,14 +716,17 @@ int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
   uint32_t tiles = (uint32_t)ceil_div(n, kRsTile);
   uint32_t *hist = nullptr;
   SSTG_TRY(ws.get(ws_base, (uint64_t)tiles * kRsBins, &hist));
+  uint32_t *dtot = nullptr;
+  SSTG_TRY(ws.get(ws_base + 1, kRsBins, &dtot));
   uint64_t *src = keys, *dst = tmp;
   for (int p = 0; p < npass; p++) {
     SSTG_LAUNCH(k_rs_hist, tiles, kRsThreads, 0, s, (const uint64_t *)src, n,
                 shifts[p], hist, tiles);
-    SSTG_TRY(scan_exclusive_u32(hist, hist, (uint64_t)tiles * kRsBins, nullptr,
-                                ws, ws_base + 1, s));
+    SSTG_LAUNCH(k_rs_digit_scan, kRsBins, kScanThreads, 0, s, hist, tiles,
+                dtot);
     SSTG_LAUNCH(k_rs_scatter, tiles, kRsThreads, 0, s, (const uint64_t *)src,
-                dst, n, shifts[p], (const uint32_t *)hist, tiles);
+                dst, n, shifts[p], (const uint32_t *)hist,
+                (const uint32_t *)dtot, tiles);
     uint64_t *t = src;
     src = dst;
     dst = t;
