@@ -225,10 +225,13 @@ __global__ void __launch_bounds__(kRsThreads)
               uint32_t tiles) {
   __shared__ uint32_t h[kRsWaves][kRsBins];
   const int w = threadIdx.x / kWave;
+  // consecutive tiles on one XCD: the counts of 16 consecutive tiles share a
+  // cache line of each digit's row, which then fills in one L2
+  const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
   for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
     (&h[0][0])[i] = 0;
   __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  uint64_t base = (uint64_t)tile * kRsTile;
   // all of a thread's keys in flight before the first count
   uint64_t kv[kRsItems];
 #pragma unroll
@@ -246,7 +249,7 @@ __global__ void __launch_bounds__(kRsThreads)
     uint32_t c = 0;
 #pragma unroll
     for (int ww = 0; ww < kRsWaves; ww++) c += h[ww][d];
-    hist[(uint64_t)d * tiles + blockIdx.x] = c;
+    hist[(uint64_t)d * tiles + tile] = c;
   }
 }
 
