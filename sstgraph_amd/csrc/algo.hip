@@ -332,19 +332,34 @@ __global__ void k_bc_last_quotients(const uint32_t *list, uint64_t count,
   if (i < count) q[list[i]] = 1.0 / q[list[i]];
 }
 
-// bm + l * words: the bitmap of level l, for l < levels; one lane per
-// vertex, a ballot per level and 64 vertices (no atomics)
+// bm + l * words: the bitmap of level l, for l < levels; a wave covers
+// kLevelRounds x 64 vertices, one lane per vertex and round, a ballot per
+// level and round (no atomics).  All of a lane's level loads are issued
+// before the first ballot (one load per thread waited for each in turn):
+// past n a lane reads vertex n - 1 and counts it on no level.
+constexpr int kLevelRounds = 8;
 __global__ void k_level_bitmaps(const uint32_t *level, uint32_t n,
                                 uint32_t levels, uint32_t *bm,
                                 uint64_t words) {
+  if (n == 0) return;
   const uint32_t lane = lane_id();
-  const uint64_t v = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  const uint32_t lv = v < n ? level[v] : kInf;
-  const uint64_t w = v >> 5;
+  const uint64_t wave = ((uint64_t)blockIdx.x * kT + threadIdx.x) / kWave;
+  const uint64_t v0 = wave * kLevelRounds * kWave + lane;
+  uint32_t lv[kLevelRounds];
+#pragma unroll
+  for (int r = 0; r < kLevelRounds; r++) {
+    const uint64_t v = v0 + (uint64_t)r * kWave;
+    const uint32_t x = level[v < n ? v : n - 1];
+    lv[r] = v < n ? x : kInf;
+  }
   for (uint32_t l = 0; l < levels; l++) {
-    const uint64_t m = __ballot(lv == l);
-    if ((lane & 31) == 0 && w < words)
-      bm[l * words + w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+#pragma unroll
+    for (int r = 0; r < kLevelRounds; r++) {
+      const uint64_t m = __ballot(lv[r] == l);
+      const uint64_t w = (v0 + (uint64_t)r * kWave) >> 5;
+      if ((lane & 31) == 0 && w < words)
+        bm[l * words + w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+    }
   }
 }
 
@@ -1012,8 +1027,9 @@ static int bc_leveled(sstg_graph *g, uint32_t src, const uint32_t *level,
   SSTG_TRY(g->ws.get(kWsAlgo + 19, run.levels * words, &bm));
   const uint32_t t0 = g->bin_hi[kBinsPull][0], t1 = g->bin_hi[kBinsPull][1],
                  t2 = g->bin_hi[kBinsPull][2];
-  SSTG_LAUNCH(k_level_bitmaps, blocks(words * 32), kT, 0, s, level, n,
-              run.levels, bm, words);
+  SSTG_LAUNCH(k_level_bitmaps,
+              ceil_div(words * 32, (uint64_t)kT * kLevelRounds), kT, 0, s,
+              level, n, run.levels, bm, words);
   SSTG_LAUNCH(k_bc_keys, blocks(reached), kT, 0, s, order, reached, level,
               (const uint32_t *)g->deg, t0, t1, t2, key);
   uint64_t *sorted = key;

@@ -302,12 +302,23 @@ __global__ void __launch_bounds__(kT)
            unsigned long long *ctr_clear, uint64_t *zero1) {
   clear_step_ctrs(ctr_clear);
   if (zero1 && blockIdx.x == 0 && threadIdx.x == 0) *zero1 = 0;
+  if (n == 0) return;  // no vertex, no word
   const uint32_t lane = lane_id();
   const uint64_t wave = ((uint64_t)blockIdx.x * kT + threadIdx.x) / kWave;
+  // every round's test issued before the first ballot (tested as they came,
+  // each round waited for its load): past n a lane tests vertex n - 1, so
+  // every load is in bounds and none sits behind a branch
+  bool in[kBitmapRounds];
+#pragma unroll
   for (int r = 0; r < kBitmapRounds; r++) {
     const uint64_t v = (wave * kBitmapRounds + r) * kWave + lane;
-    const bool in = v < n && p((uint32_t)v);
-    const uint64_t m = __ballot(in);
+    const uint32_t vc = v < n ? (uint32_t)v : n - 1;
+    in[r] = p(vc) & (v < n);
+  }
+#pragma unroll
+  for (int r = 0; r < kBitmapRounds; r++) {
+    const uint64_t v = (wave * kBitmapRounds + r) * kWave + lane;
+    const uint64_t m = __ballot(in[r]);
     const uint64_t w = v >> 5;
     if ((lane & 31) == 0 && w < words)
       bm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
