@@ -253,6 +253,67 @@ __global__ void __launch_bounds__(kRsThreads)
   }
 }
 
+// keys[i] = src[i] << 32 | dst[i]; a pair out of range sets *err and packs
+// as (0, 0).
+__global__ void k_pack_pairs(const uint32_t *src, const uint32_t *dst,
+                             uint64_t n, uint32_t n_rows, uint32_t n_cols,
+                             uint64_t *keys, uint64_t *err) {
+  const uint64_t i = (uint64_t)blockIdx.x * kRsThreads + threadIdx.x;
+  if (i >= n) return;
+  uint32_t a = src[i], b = dst[i];
+  if (a >= n_rows || b >= n_cols) {
+    *err = 1;
+    a = 0;
+    b = 0;
+  }
+  keys[i] = ((uint64_t)a << 32) | b;
+}
+
+// k_pack_pairs over k_rs_hist's tiles (the same tile order and layout),
+// counting each packed key's digit at shift as k_rs_hist would.
+__global__ void __launch_bounds__(kRsThreads)
+    k_rs_pack_hist(const uint32_t *src, const uint32_t *dst, uint64_t n,
+                   uint32_t n_rows, uint32_t n_cols, uint64_t *keys,
+                   uint64_t *err, int shift, uint32_t *hist, uint32_t tiles) {
+  __shared__ uint32_t h[kRsWaves][kRsBins];
+  const int w = threadIdx.x / kWave;
+  const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+  for (int i = threadIdx.x; i < kRsWaves * kRsBins; i += kRsThreads)
+    (&h[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)tile * kRsTile;
+  uint32_t a[kRsItems], b[kRsItems];
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    const uint64_t i = base + (uint64_t)j * kRsThreads + threadIdx.x;
+    a[j] = i < n ? src[i] : 0u;
+    b[j] = i < n ? dst[i] : 0u;
+  }
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < kRsItems; j++) {
+    const uint64_t i = base + (uint64_t)j * kRsThreads + threadIdx.x;
+    if (i < n) {
+      if (a[j] >= n_rows || b[j] >= n_cols) {
+        bad = true;
+        a[j] = 0;
+        b[j] = 0;
+      }
+      const uint64_t k = ((uint64_t)a[j] << 32) | b[j];
+      keys[i] = k;
+      atomicAdd(&h[w][(uint32_t)(k >> shift) & 255u], 1u);
+    }
+  }
+  if (bad) *err = 1;
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRsBins; d += kRsThreads) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int ww = 0; ww < kRsWaves; ww++) c += h[ww][d];
+    hist[(uint64_t)d * tiles + tile] = c;
+  }
+}
+
 // Block d: the exclusive prefix of digit d's tile counts hist[d * tiles + t]
 // over the tiles, in place, and the digit's total in dtot[d].  The scatter
 // adds the totals of the smaller digits itself, so one launch replaces a
@@ -702,6 +763,45 @@ int select_flagged_u32(const uint8_t *flags, const uint32_t *values,
                                   s);
 }
 
+// The radix passes; with pairs, pass 0's histogram comes from the packing
+// kernel, which writes the keys.
+struct PackedPairs {
+  const uint32_t *src, *dst;
+  uint32_t n_rows, n_cols;
+  uint64_t *err;
+};
+
+static int radix_passes(uint64_t *keys, uint64_t *tmp, uint64_t n,
+                        const int *shifts, int npass, Workspace &ws,
+                        int ws_base, hipStream_t s, uint64_t **result,
+                        const PackedPairs *pairs) {
+  uint32_t tiles = (uint32_t)ceil_div(n, kRsTile);
+  uint32_t *hist = nullptr;
+  SSTG_TRY(ws.get(ws_base, (uint64_t)tiles * kRsBins, &hist));
+  uint32_t *dtot = nullptr;
+  SSTG_TRY(ws.get(ws_base + 1, kRsBins, &dtot));
+  uint64_t *src = keys, *dst = tmp;
+  for (int p = 0; p < npass; p++) {
+    if (p == 0 && pairs)
+      SSTG_LAUNCH(k_rs_pack_hist, tiles, kRsThreads, 0, s, pairs->src,
+                  pairs->dst, n, pairs->n_rows, pairs->n_cols, keys,
+                  pairs->err, shifts[0], hist, tiles);
+    else
+      SSTG_LAUNCH(k_rs_hist, tiles, kRsThreads, 0, s, (const uint64_t *)src,
+                  n, shifts[p], hist, tiles);
+    SSTG_LAUNCH(k_rs_digit_scan, kRsBins, kScanThreads, 0, s, hist, tiles,
+                dtot);
+    SSTG_LAUNCH(k_rs_scatter, tiles, kRsThreads, 0, s, (const uint64_t *)src,
+                dst, n, shifts[p], (const uint32_t *)hist,
+                (const uint32_t *)dtot, tiles);
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+  }
+  *result = src;
+  return SSTG_OK;
+}
+
 int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
                    const int *shifts, int npass, Workspace &ws, int ws_base,
                    hipStream_t s, uint64_t **result) {
@@ -716,26 +816,8 @@ int radix_sort_u64(uint64_t *keys, uint64_t *tmp, uint64_t n,
                 kSsMax);
     return SSTG_OK;
   }
-  uint32_t tiles = (uint32_t)ceil_div(n, kRsTile);
-  uint32_t *hist = nullptr;
-  SSTG_TRY(ws.get(ws_base, (uint64_t)tiles * kRsBins, &hist));
-  uint32_t *dtot = nullptr;
-  SSTG_TRY(ws.get(ws_base + 1, kRsBins, &dtot));
-  uint64_t *src = keys, *dst = tmp;
-  for (int p = 0; p < npass; p++) {
-    SSTG_LAUNCH(k_rs_hist, tiles, kRsThreads, 0, s, (const uint64_t *)src, n,
-                shifts[p], hist, tiles);
-    SSTG_LAUNCH(k_rs_digit_scan, kRsBins, kScanThreads, 0, s, hist, tiles,
-                dtot);
-    SSTG_LAUNCH(k_rs_scatter, tiles, kRsThreads, 0, s, (const uint64_t *)src,
-                dst, n, shifts[p], (const uint32_t *)hist,
-                (const uint32_t *)dtot, tiles);
-    uint64_t *t = src;
-    src = dst;
-    dst = t;
-  }
-  *result = src;
-  return SSTG_OK;
+  return radix_passes(keys, tmp, n, shifts, npass, ws, ws_base, s, result,
+                      nullptr);
 }
 
 int sort_keys_u64(uint64_t *keys, uint64_t *tmp, uint64_t n, const int *shifts,
@@ -759,6 +841,24 @@ int sort_keys_u64(uint64_t *keys, uint64_t *tmp, uint64_t n, const int *shifts,
   }
   *result = src;
   return SSTG_OK;
+}
+
+int pack_sort_pairs(const uint32_t *src, const uint32_t *dst, uint64_t n,
+                    uint32_t n_rows, uint32_t n_cols, uint64_t *err,
+                    uint64_t *keys, uint64_t *tmp, const int *shifts,
+                    int npass, Workspace &ws, int ws_base, hipStream_t s,
+                    uint64_t **result) {
+  *result = keys;
+  if (n == 0) return SSTG_OK;
+  if (n > kMergeSortMax && npass > 0) {
+    if (n >= (1ull << 32) || npass > 16) return SSTG_E_INVAL;
+    const PackedPairs pairs{src, dst, n_rows, n_cols, err};
+    return radix_passes(keys, tmp, n, shifts, npass, ws, ws_base, s, result,
+                        &pairs);
+  }
+  SSTG_LAUNCH(k_pack_pairs, ceil_div(n, (uint64_t)kRsThreads), kRsThreads, 0,
+              s, src, dst, n, n_rows, n_cols, keys, err);
+  return sort_keys_u64(keys, tmp, n, shifts, npass, ws, ws_base, s, result);
 }
 
 }  // namespace sstg

@@ -47,6 +47,16 @@ int sort_keys_u64(uint64_t *keys, uint64_t *tmp, uint64_t n, const int *shifts,
                   int npass, Workspace &ws, int ws_base, hipStream_t s,
                   uint64_t **result);
 
+// keys[i] = src[i] << 32 | dst[i], sorted as sort_keys_u64 sorts them.  A
+// pair outside [0, n_rows) x [0, n_cols) sets *err to 1 and packs as (0, 0).
+// On the radix path the packing kernel also counts the first digit of its
+// tile, so the first pass reads no keys for its histogram.
+int pack_sort_pairs(const uint32_t *src, const uint32_t *dst, uint64_t n,
+                    uint32_t n_rows, uint32_t n_cols, uint64_t *err,
+                    uint64_t *keys, uint64_t *tmp, const int *shifts,
+                    int npass, Workspace &ws, int ws_base, hipStream_t s,
+                    uint64_t **result);
+
 // Workspace slots used by the primitives, starting at ws_base.
 constexpr int kPrimSlots = 8;
 

@@ -65,20 +65,6 @@ __device__ __forceinline__ uint64_t grow_cap(uint64_t d, uint32_t n_cols) {
   return slot_capacity(d, n_cols);
 }
 
-__global__ void k_pack(const uint32_t *src, const uint32_t *dst, uint64_t n,
-                       uint32_t n_rows, uint32_t n_cols, uint64_t *keys,
-                       uint64_t *ctr) {
-  uint64_t i = (uint64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  uint32_t s = src[i], d = dst[i];
-  if (s >= n_rows || d >= n_cols) {
-    ctr[kCtrErr] = 1;
-    s = 0;
-    d = 0;
-  }
-  keys[i] = ((uint64_t)s << 32) | d;
-}
-
 // ctr[kCtrErr] = 1 if any pair of the batch is out of range (the whole-batch
 // check of a batch applied in pieces)
 __global__ void k_check_range(const uint32_t *src, const uint32_t *dst,
@@ -1088,15 +1074,14 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
   SSTG_TRY(ws.get(kWsKeysB, n, &tmp));
   SSTG_TRY(ws.get(kWsFlags, n + 1, &flags));
   SSTG_HIP(hipMemsetAsync(ctr, 0, kCtrCount * sizeof(uint64_t), s));
-  SSTG_LAUNCH(k_pack, blocks(n), kT, 0, s, src, dst, n, g->n_rows, g->n_cols,
-              keys, ctr);
   int shifts[16], np = 0;
   uint32_t cb = bit_width_u32(g->n_cols ? g->n_cols - 1 : 0);
   uint32_t rb = bit_width_u32(g->n_rows ? g->n_rows - 1 : 0);
   for (uint32_t b = 0; b < cb; b += 8) shifts[np++] = (int)b;
   for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
   uint64_t *sorted = nullptr;
-  SSTG_TRY(sort_keys_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  SSTG_TRY(pack_sort_pairs(src, dst, n, g->n_rows, g->n_cols, ctr + kCtrErr,
+                           keys, tmp, shifts, np, ws, kWsPrim, s, &sorted));
   uint32_t *pos = nullptr, *B_pos = nullptr;
   SSTG_TRY(ws.get(kWsMarkPos, n, &pos));
   SSTG_TRY(ws.get(kWsBPos, n, &B_pos));
@@ -1300,15 +1285,14 @@ int graph_effective_keys(sstg_graph *g, const uint32_t *src,
   SSTG_TRY(ws.get(kWsFlags, n + 1, &flags));
   SSTG_TRY(ws.get(kWsMarkPos, n, &pos));
   SSTG_HIP(hipMemsetAsync(ctr, 0, kCtrCount * sizeof(uint64_t), s));
-  SSTG_LAUNCH(k_pack, blocks(n), kT, 0, s, src, dst, n, g->n_rows, g->n_cols,
-              keys, ctr);
   int shifts[16], np = 0;
   uint32_t cb = bit_width_u32(g->n_cols ? g->n_cols - 1 : 0);
   uint32_t rb = bit_width_u32(g->n_rows ? g->n_rows - 1 : 0);
   for (uint32_t b = 0; b < cb; b += 8) shifts[np++] = (int)b;
   for (uint32_t b = 0; b < rb; b += 8) shifts[np++] = (int)(32 + b);
   uint64_t *sorted = nullptr;
-  SSTG_TRY(sort_keys_u64(keys, tmp, n, shifts, np, ws, kWsPrim, s, &sorted));
+  SSTG_TRY(pack_sort_pairs(src, dst, n, g->n_rows, g->n_cols, ctr + kCtrErr,
+                           keys, tmp, shifts, np, ws, kWsPrim, s, &sorted));
   SSTG_LAUNCH(k_mark, blocks(n), kT, 0, s, (const uint64_t *)sorted, n,
               g->view(), insert ? 1 : 0, skip_self ? 1 : 0, flags, pos);
   SSTG_TRY(select_flagged_u64(flags, sorted, n, out, ctr + kCtrB, ws, kWsPrim,

@@ -157,6 +157,21 @@ def test_out_of_range_ids_leave_graph_unchanged():
     assert g.M() == 2 and g.has(1, 3) and g.has(2, 4)
     g.insert_batch(np.array([], np.uint32), np.array([], np.uint32))
     assert g.M() == 2
+    # long batches on both sides of the merge-sort limit (2^20 keys): above
+    # it the pairs are packed by the kernel that counts the first radix digit
+    rng = np.random.default_rng(5)
+    for n in (1 << 20, (1 << 20) + 3):
+        for at in (0, n - 1):
+            say("out of range", n, at)
+            s = rng.integers(0, 100, n, dtype=np.uint32)
+            d = rng.integers(0, 100, n, dtype=np.uint32)
+            d[at] = 100
+            with pytest.raises(S.IdOutOfRange):
+                g.insert_batch(s, d)
+            s[at], d[at] = 100, 0
+            with pytest.raises(S.IdOutOfRange):
+                g.remove_batch(s, d)
+            assert g.M() == 2 and g.has(1, 3) and g.has(2, 4)
 
 
 def test_rectangular_and_zero_key():
