@@ -876,20 +876,53 @@ __global__ void k_newcap(const uint32_t *deg, const uint32_t *cap, uint64_t n,
   cap64[i] = c;
 }
 
+// The row, source slot and destination slot of each of a thread's kChunk
+// consecutive edges from e0 (edges past m get slot 0 of the first edge's
+// row and must not be stored).  Rows are found by prefix reads only, so the
+// pool loads that follow can all be in flight at once (loaded and stored
+// one edge at a time, each store made the next edge's loads wait, as the
+// store might alias them).
+__device__ __forceinline__ void chunk_slots(const DevGraph &g,
+                                            const uint64_t *prefix, uint64_t m,
+                                            const uint64_t *dst_off,
+                                            uint64_t e0, uint64_t (&src)[kChunk],
+                                            uint64_t (&dst)[kChunk]) {
+  uint64_t v = find_segment(prefix, g.n, e0);
+  uint64_t pv = prefix[v], pn = prefix[v + 1];
+  uint64_t so = g.off[v], dso = dst_off ? dst_off[v] : pv;
+#pragma unroll
+  for (int j = 0; j < kChunk; j++) {
+    const uint64_t e = e0 + (uint64_t)j;
+    src[j] = so;
+    dst[j] = 0;
+    if (e < m) {
+      while (e >= pn) {  // on to the edge's row (rows of degree 0 skipped)
+        v++;
+        pv = pn;
+        pn = prefix[v + 1];
+        so = g.off[v];
+        dso = dst_off ? dst_off[v] : pv;
+      }
+      src[j] = so + (e - pv);
+      dst[j] = dso + (e - pv);
+    }
+  }
+}
+
 // Copies every row (deg[v] ids) to dst_base(v) + k, flat over all edges.
 // dst_off == nullptr: dense output at the row's degree prefix.
 __global__ void k_gather_rows(DevGraph g, const uint64_t *prefix, uint64_t m,
                               uint32_t *out, const uint64_t *dst_off) {
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
-  if (e >= m) return;
-  uint64_t end = e + kChunk < m ? e + kChunk : m;
-  uint64_t v = find_segment(prefix, g.n, e);
-  for (; e < end; e++) {
-    while (e >= prefix[v + 1]) v++;
-    uint64_t k = e - prefix[v];
-    uint64_t d = dst_off ? dst_off[v] + k : e;
-    out[d] = g.pool[g.off[v] + k];
-  }
+  const uint64_t e0 = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  if (e0 >= m) return;
+  uint64_t src[kChunk], dst[kChunk];
+  chunk_slots(g, prefix, m, dst_off, e0, src, dst);
+  uint32_t val[kChunk];
+#pragma unroll
+  for (int j = 0; j < kChunk; j++) val[j] = g.pool[src[j]];
+#pragma unroll
+  for (int j = 0; j < kChunk; j++)
+    if (e0 + (uint64_t)j < m) out[dst[j]] = val[j];
 }
 
 __global__ void k_has(DevGraph g, const uint32_t *rows, const uint32_t *cols,
@@ -908,15 +941,17 @@ __global__ void __launch_bounds__(kT)
     k_touch_sum(DevGraph g, const uint64_t *prefix, uint64_t m,
                 uint64_t *part) {
   __shared__ uint64_t ws[kT / kWave];
-  uint64_t e = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
+  const uint64_t e0 = ((uint64_t)blockIdx.x * kT + threadIdx.x) * kChunk;
   uint64_t acc = 0;
-  if (e < m) {
-    uint64_t end = e + kChunk < m ? e + kChunk : m;
-    uint64_t v = find_segment(prefix, g.n, e);
-    for (; e < end; e++) {
-      while (e >= prefix[v + 1]) v++;
-      acc += g.pool[g.off[v] + (e - prefix[v])];
-    }
+  if (e0 < m) {
+    uint64_t src[kChunk], dst[kChunk];
+    chunk_slots(g, prefix, m, nullptr, e0, src, dst);
+    uint32_t val[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; j++) val[j] = g.pool[src[j]];
+#pragma unroll
+    for (int j = 0; j < kChunk; j++)
+      if (e0 + (uint64_t)j < m) acc += val[j];
   }
   acc = wave_sum(acc);
   if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = acc;
