@@ -218,6 +218,7 @@ enum : int {
   kWsSmallPos = 146,
   kWsSmallRows = 147,
   kWsSmallFound = 149,  // small updates: each pair's row search
+  kWsRebuild = 150,     // 150..152 pool rebuild: capacities, offsets, prefix
 };
 
 // degree threshold sets of a graph (sstg_graph::bin_hi): the reductions

@@ -1030,10 +1030,13 @@ void graph_free(sstg_graph *g) {
 static int rebuild_impl(sstg_graph *g, uint64_t extra, bool recompute) {
   hipStream_t s = g->stream;
   const uint64_t n = g->n_rows;
+  // the per-vertex temporaries stay in the workspace between rebuilds
+  // rather than being allocated and freed by each (the copies into off and
+  // cap below are ordered on the stream before any later use of them)
   uint32_t *new_cap = nullptr;
   uint64_t *new_off = nullptr, *prefix = nullptr;
-  SSTG_HIP(hipMalloc(&new_cap, (n + 1) * sizeof(uint32_t)));
-  SSTG_HIP(hipMalloc(&new_off, (n + 1) * sizeof(uint64_t)));
+  SSTG_TRY(g->ws.get(kWsRebuild, n + 1, &new_cap));
+  SSTG_TRY(g->ws.get(kWsRebuild + 1, n + 1, &new_off));
   SSTG_LAUNCH(k_newcap, blocks(n + 1), kT, 0, s, (const uint32_t *)g->deg,
               (const uint32_t *)g->cap, n, g->n_cols, recompute ? 1 : 0,
               new_cap, new_off);
@@ -1053,7 +1056,7 @@ static int rebuild_impl(sstg_graph *g, uint64_t extra, bool recompute) {
     SSTG_HIP(hipMalloc(&new_pool, new_cap_elems * sizeof(uint32_t)));
   }
   if (g->m) {
-    SSTG_HIP(hipMalloc(&prefix, (n + 1) * sizeof(uint64_t)));
+    SSTG_TRY(g->ws.get(kWsRebuild + 2, n + 1, &prefix));
     SSTG_LAUNCH(k_deg_to_u64, blocks(n + 1), kT, 0, s, (const uint32_t *)g->deg,
                 n, prefix);
     SSTG_TRY(scan_exclusive_u64(prefix, prefix, n + 1, nullptr, g->ws,
@@ -1063,7 +1066,6 @@ static int rebuild_impl(sstg_graph *g, uint64_t extra, bool recompute) {
                 (const uint64_t *)new_off);
   }
   SSTG_HIP(hipStreamSynchronize(s));
-  hipFree(prefix);
   hipFree(g->pool);
   g->pool = new_pool;
   g->pool_cap = new_cap_elems;
@@ -1073,9 +1075,6 @@ static int rebuild_impl(sstg_graph *g, uint64_t extra, bool recompute) {
                           hipMemcpyDeviceToDevice, s));
   SSTG_HIP(hipMemcpyAsync(g->cap, new_cap, n * sizeof(uint32_t),
                           hipMemcpyDeviceToDevice, s));
-  SSTG_HIP(hipStreamSynchronize(s));
-  hipFree(new_cap);
-  hipFree(new_off);
   g->rebuilds++;
   return SSTG_OK;
 }
