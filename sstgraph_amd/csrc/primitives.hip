@@ -595,15 +595,15 @@ __global__ void __launch_bounds__(kSsThreads)
 // Keys of n <= kMergeSortMax: k_sort_chunks sorts runs of kMsChunk keys
 // (one block each, all in parallel), then each k_merge_pass doubles the
 // sorted runs in one launch.  A 10^5-key batch takes 6 launches where the
-// radix sort takes 30 (6 digit passes of histogram, three scan launches and
-// scatter).
+// radix sort takes 17 (6 digit passes of histogram, count scan and
+// scatter, the first histogram counted while packing).
 constexpr int kMpThreads = 256;
 constexpr int kMpItems = 8;
 constexpr uint32_t kMpOut = kMpThreads * kMpItems;  // outputs per block
-// above this the radix sort wins (insert at scale 24: 2e6 keys 1.00 ms
-// radix against 1.05 merge; 1e6 keys 0.72 either way; 7e5 0.61 radix
-// against 0.60 merge)
-constexpr uint64_t kMergeSortMax = 1ull << 20;
+// above this the radix sort wins (insert at scale 24, radix against merge:
+// 1e6 keys 0.670 against 0.703 ms, 7e5 0.565 against 0.587, 2^19 0.473
+// against 0.483, 3.5e5 0.384 either way, 2^18 + 1 0.332 against 0.329)
+constexpr uint64_t kMergeSortMax = 1ull << 18;
 
 // Number of a[0..la) among the first d outputs of the merge of a and b
 // (ties from a first): the smallest i with a[i] > b[d - 1 - i], or the

@@ -58,14 +58,15 @@ def test_batch_sizes_around_the_sort_and_scan_paths():
     """Batches on both sides of the one-block LDS sort (4096 keys for an
     update, 16384 for the radix sort's own one-block path), of the one-block
     scans (4096 values) and of the merge sort (4K-key block sorts and merge
-    passes up to 2^20 keys, the radix sort beyond), inserted and then half
+    passes up to 2^18 keys, the radix sort beyond), inserted and then half
     removed; ids below 2^17 so every batch sorts on three column and three
     row digits."""
     n = 1 << 17
     rng = np.random.default_rng(11)
     g, og = S.SparseMatrixV(n), O.Graph(n)
     for k in (2, 63, 64, 65, 4095, 4096, 4097, 8192, 12289, 16383, 16384,
-              16385, 32768, 40000, 49153, 1 << 20, (1 << 20) + 1, 1 << 21):
+              16385, 32768, 40000, 49153, 1 << 18, (1 << 18) + 1, 1 << 20,
+              (1 << 20) + 1, 1 << 21):
         say("batch", k)
         s = rng.integers(0, n, k).astype(np.uint32)
         d = rng.integers(0, n, k).astype(np.uint32)
@@ -157,10 +158,10 @@ def test_out_of_range_ids_leave_graph_unchanged():
     assert g.M() == 2 and g.has(1, 3) and g.has(2, 4)
     g.insert_batch(np.array([], np.uint32), np.array([], np.uint32))
     assert g.M() == 2
-    # long batches on both sides of the merge-sort limit (2^20 keys): above
+    # long batches on both sides of the merge-sort limit (2^18 keys): above
     # it the pairs are packed by the kernel that counts the first radix digit
     rng = np.random.default_rng(5)
-    for n in (1 << 20, (1 << 20) + 3):
+    for n in (1 << 18, (1 << 18) + 3):
         for at in (0, n - 1):
             say("out of range", n, at)
             s = rng.integers(0, 100, n, dtype=np.uint32)
