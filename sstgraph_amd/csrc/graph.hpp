@@ -33,6 +33,9 @@ struct sstg_graph {
   // handle has none (the copy through host_rb is used then)
   uint64_t *host_ctr = nullptr;
   uint64_t *host_ctr_dev = nullptr;
+  // the last value the host asked the device to write to the flag slot of
+  // host_ctr (kHostFlag) once a step's counters are in place
+  uint64_t host_seq = 0;
   // device-side counters block (see update.hip)
   uint64_t *dev_ctr = nullptr;
   // cached derived data: exclusive prefix of deg (n_rows + 1 entries)
@@ -109,6 +112,18 @@ int update_small(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
 // update.hip
 int graph_alloc(sstg_graph *g);
 void graph_free(sstg_graph *g);
+// host_ctr holds kHostCtrSlots words: the counter blocks from 0 and, at
+// kHostFlag, the sequence number a kernel writes after them
+constexpr int kHostCtrSlots = 32;
+constexpr int kHostFlag = 24;
+
+// Waits until the device has written seq to host_ctr[kHostFlag] (spinning on
+// the mapped word, which arrives as soon as the kernel writes it rather than
+// when the stream's completion signal is seen), then orders the host's later
+// reads after it.  Falls back to a stream synchronisation when the graph has
+// no mapped block or the flag has not come within 100 ms.
+int wait_host_flag(sstg_graph *g, uint64_t seq);
+
 int graph_update(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
                  uint64_t n, bool insert, bool skip_self, bool device_ptrs);
 int graph_rebuild(sstg_graph *g, uint64_t extra_elems);

@@ -25,6 +25,10 @@
 // Steps 8 and 9 run over a flat index space (one thread per 8 outputs), so a
 // hub row with a million neighbours and a thousand one-edge rows are balanced
 // the same way.
+#include <atomic>
+#include <chrono>
+#include <cstring>
+
 #include "graph.hpp"
 #include "primitives.hpp"
 
@@ -203,6 +207,7 @@ struct SegScan {
   uint64_t *big_idx, *big_stage, *big_work, *tile_idx, *tile_len;
   uint64_t *ctr;
   uint64_t *hctr;  // mapped host copy of the counter block, or null
+  uint64_t seq;    // written to hctr[kHostFlag] after the counters
 };
 
 // The six running sums of the K slots s0, s0 + 1, ... below hi (allocation,
@@ -372,8 +377,11 @@ __global__ void __launch_bounds__(kT) k_seg_scan(SegScan a) {
     a.big_work[carry[1]] = carry[4];
     a.tile_len[carry[2]] = carry[5];
     // the whole block, earlier launches' counters included, for the host
-    if (a.hctr)
+    if (a.hctr) {
       for (int k = 0; k < kCtrCount; k++) a.hctr[k] = a.ctr[k];
+      __threadfence_system();  // the counters before the flag, to the host
+      *(volatile uint64_t *)(a.hctr + kHostFlag) = a.seq;
+    }
   }
 }
 
@@ -997,8 +1005,10 @@ int graph_alloc(sstg_graph *g) {
   SSTG_HIP(hipMalloc(&g->dev_ctr, kCtrCount * sizeof(uint64_t)));
   SSTG_HIP(hipHostMalloc(&g->host_rb, 64 * sizeof(uint64_t),
                          hipHostMallocDefault));
-  SSTG_HIP(hipHostMalloc(&g->host_ctr, 16 * sizeof(uint64_t),
+  SSTG_HIP(hipHostMalloc(&g->host_ctr, kHostCtrSlots * sizeof(uint64_t),
                          hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(g->host_ctr, 0, kHostCtrSlots * sizeof(uint64_t));
+  g->host_seq = 0;
   SSTG_HIP(hipHostGetDevicePointer((void **)&g->host_ctr_dev, g->host_ctr, 0));
   SSTG_HIP(hipStreamSynchronize(g->stream));
   return SSTG_OK;
@@ -1076,6 +1086,24 @@ static int rebuild_impl(sstg_graph *g, uint64_t extra, bool recompute) {
   SSTG_HIP(hipMemcpyAsync(g->cap, new_cap, n * sizeof(uint32_t),
                           hipMemcpyDeviceToDevice, s));
   g->rebuilds++;
+  return SSTG_OK;
+}
+
+int wait_host_flag(sstg_graph *g, uint64_t seq) {
+  if (g->host_ctr) {
+    const volatile uint64_t *f = g->host_ctr + kHostFlag;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 1;; i++) {
+      if (*f == seq) {
+        std::atomic_thread_fence(std::memory_order_acquire);
+        return SSTG_OK;
+      }
+      if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 >
+                                 std::chrono::milliseconds(100))
+        break;
+    }
+  }
+  SSTG_HIP(hipStreamSynchronize(g->stream));
   return SSTG_OK;
 }
 
@@ -1173,6 +1201,7 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
     sc.tile_len = tile_len;
     sc.ctr = ctr;
     sc.hctr = g->host_ctr_dev;
+    sc.seq = ++g->host_seq;
     const uint64_t tiles = ceil_div(sc.n, kSegTile);
     if (tiles <= kSegOneBlock) {
       sc.per = sc.n;
@@ -1193,8 +1222,10 @@ static int update_chunk(sstg_graph *g, const uint32_t *src, const uint32_t *dst,
     SSTG_HIP(hipMemcpyAsync(g->host_rb, ctr, kCtrCount * sizeof(uint64_t),
                             hipMemcpyDeviceToHost, s));
     h = g->host_rb;
+    SSTG_HIP(hipStreamSynchronize(s));
+  } else {
+    SSTG_TRY(wait_host_flag(g, g->host_seq));
   }
-  SSTG_HIP(hipStreamSynchronize(s));
   if (h[kCtrErr]) return SSTG_E_RANGE;
   const uint64_t b = h[kCtrB], t = h[kCtrT];
   const uint64_t stage_tot = h[kCtrStage], alloc_tot = h[kCtrAlloc],
