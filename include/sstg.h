@@ -419,6 +419,8 @@ int sstg_er(uint32_t n, uint32_t seed, uint64_t first, uint64_t count,
  * own (tests, benchmarks, FFI hosts) ------------------------------------ */
 int sstg_device_alloc(uint64_t bytes, void **out);
 int sstg_device_free(void *p);
+/* The copies return once the data is in place for work on any stream
+ * (a graph's own stream does not wait for the null stream). */
 int sstg_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
 int sstg_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
 int sstg_memcpy_d2d(void *dst, const void *src, uint64_t bytes);

@@ -682,8 +682,14 @@ int sstg_device_free(void *p) {
   return SSTG_OK;
 }
 
+// The copies return once the data is in place for every stream: a graph's
+// stream does not wait for the null stream, and hipMemcpy may return before a
+// device-to-device copy (or the device side of a copy from pageable host
+// memory) has finished, so an update started right after a copy into its
+// input could read a part not yet written.
 int sstg_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
   SSTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  SSTG_HIP(hipDeviceSynchronize());
   return SSTG_OK;
 }
 
@@ -694,6 +700,7 @@ int sstg_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
 
 int sstg_memcpy_d2d(void *dst, const void *src, uint64_t bytes) {
   SSTG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  SSTG_HIP(hipDeviceSynchronize());
   return SSTG_OK;
 }
 

@@ -467,8 +467,10 @@ int sstg_multi_create(uint32_t k, const uint32_t *n_per_graph,
       hipMalloc(&M->c32a, n * sizeof(float)) ||
       hipMalloc(&M->c32b, n * sizeof(float)))
     return fail(SSTG_E_NOMEM);
-  if (hipMemcpy(M->d_vbase, vbase.data(), (k + 1) * sizeof(uint32_t),
-                hipMemcpyHostToDevice))
+  // on the stream the kernel below runs on (a copy on the null stream is
+  // not ordered before work on it)
+  if (hipMemcpyAsync(M->d_vbase, vbase.data(), (k + 1) * sizeof(uint32_t),
+                     hipMemcpyHostToDevice, g->stream))
     return fail(SSTG_E_DEVICE);
   hipLaunchKernelGGL(k_graph_of, dim3(blocks(n)), dim3(kT), 0, g->stream,
                      (const uint32_t *)M->d_vbase, k, (uint32_t)n,
